@@ -1,0 +1,181 @@
+/*
+ * quest_amd.h -- C ABI of the MI355X (gfx950) retarded-interaction step.
+ *
+ * This is the drop-in boundary for QuEST's direct pairwise history
+ * interaction and the predictor-corrector loop around it.  No C++ or HIP types
+ * cross it: plain pointers, sizes and status codes.  Complex numbers are
+ * interleaved (re, im) doubles.  Dot arrays are in the caller's order; the
+ * library reorders internally and converts back.
+ *
+ * Every entry point cites the reference interface it replaces (paths relative
+ * to the QuEST source tree).  The C++ mirror of the reference classes that
+ * sits on top of this ABI is quest_amd/host/quest.hpp.
+ *
+ * Threading: one context owns one HIP stream; a context is not thread-safe,
+ * distinct contexts are independent.
+ */
+#ifndef QUEST_AMD_H
+#define QUEST_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The C++ mirror rethrows them as the reference's exception
+ * types: QUEST_ERR_DOMAIN -> std::domain_error("unbounded history values at
+ * or before step N") (integrator.cpp:200-207); everything else ->
+ * std::runtime_error with quest_last_error(). */
+#define QUEST_OK 0
+#define QUEST_ERR_ARG 1      /* invalid argument                              */
+#define QUEST_ERR_HIP 2      /* HIP runtime error (message says which call)   */
+#define QUEST_ERR_DOMAIN 3   /* non-finite state; see quest_bad_step          */
+#define QUEST_ERR_STATE 4    /* call out of sequence for this context         */
+#define QUEST_ERR_NOMEM 5    /* device or host allocation failed              */
+
+/* Arithmetic of the O(N^2) pair sum.  FP64 everywhere, or FP32 pair
+ * arithmetic with FP64 accumulation per target (the Bloch/PC update, the
+ * delay split of the zero-delay pairs and all per-dot work stay FP64). */
+#define QUEST_PREC_FP64 0
+#define QUEST_PREC_MIXED 1
+
+/* Green's function of the history interaction.
+ * ROTATING: GreenFunction::RotatingDyadic (rotating_green_function.cpp:3-24),
+ *           prefactor rho01 (rotating_green_function.h:17-21) -- what main.cpp
+ *           wires (main.cpp:40-46).
+ * LAB:      GreenFunction::Dyadic (green_function.cpp:3-20), prefactor
+ *           2 Re rho01 (green_function.h:23-27). */
+#define QUEST_DYADIC_ROTATING 0
+#define QUEST_DYADIC_LAB 1
+
+typedef struct quest_ctx quest_ctx;
+
+/* Configuration values the path reads (configuration.h:12-19) plus the
+ * choices main.cpp hard-codes (history window 22, main.cpp:30). */
+typedef struct {
+  double c0;           /* config.c0, um/ps                                  */
+  double hbar;         /* config.hbar, meV ps                               */
+  double mu0;          /* config.mu0                                        */
+  double laser_omega;  /* config.omega, rad/ps (rotating frame)             */
+  double dt;           /* config.dt, ps                                     */
+  int interp_order;    /* config.interpolation_order, 1..6                  */
+  int history_window;  /* PC window; main.cpp:30 uses 22 (1..64)            */
+  int precision;       /* QUEST_PREC_*                                      */
+  int dyadic;          /* QUEST_DYADIC_*                                    */
+  int device;          /* HIP device ordinal                                */
+  int source_split;    /* pair-kernel source splits; 0 = autotuned default  */
+} quest_params;
+
+/* Pulse (pulse.h:10-24): amplitude, delay, width (dimensionless), freq,
+ * wavevector, polarization.  The polarization is used as given, as on the
+ * reference's read_pulse_config path (pulse.cpp:29-46). */
+typedef struct {
+  double amplitude, delay, width, freq;
+  double wavevector[3];
+  double polarization[3];
+} quest_pulse;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the construction of HistoryInteraction (history_interaction.cpp:
+ * 5-19; the coefficient table of :21-45 is not built -- the kernel evaluates
+ * the Green's function per pair on the fly) and of the DotVector it reads
+ * (quantum_dot.h:21-48: pos, freq, damping (T1, T2), dipole).
+ * pos, dip: [n][3]; freq, t1, t2: [n].  freq/t1/t2 may be NULL when only
+ * quest_hist_evaluate is used. */
+int quest_ctx_create(const quest_params *params, int n_dots, const double *pos,
+                     const double *dip, const double *freq, const double *t1,
+                     const double *t2, quest_ctx **out);
+
+void quest_ctx_destroy(quest_ctx *ctx);
+
+/* Message of the last failed call on ctx (or of the last failed
+ * quest_ctx_create on this thread when ctx is NULL). */
+const char *quest_last_error(const quest_ctx *ctx);
+
+/* ------------------------------------------ HistoryInteraction::evaluate */
+
+/* Replaces HistoryInteraction::evaluate(step) (history_interaction.cpp:47-70):
+ * out[n] (complex) = the retarded field projected on each dot's dipole, from
+ * the history of rho01.  rho01 [n] (complex) is each dot's current rho01 at
+ * `step` -- the only history slot the integrator changes between calls
+ * (integrator.cpp:142-185).  Values at earlier steps are the ones passed with
+ * the last call for that step; steps before 0 are the ground-state prehistory
+ * (rho01 = 0, main.cpp:32-36).  `step` must not decrease and may grow by at
+ * most one per call. */
+int quest_hist_evaluate(quest_ctx *ctx, int step, const double *rho01,
+                        double *out);
+
+/* --------------------------------------------- PulseInteraction::evaluate */
+
+/* Sets (or, with NULL, clears) the pulse: Pulse (pulse.h:10-24). */
+int quest_set_pulse(quest_ctx *ctx, const quest_pulse *pulse);
+
+/* Replaces PulseInteraction::evaluate(step) (pulse_interaction.cpp:9-20):
+ * out[n] (complex, imaginary part 0). */
+int quest_pulse_evaluate(quest_ctx *ctx, int step, double *out);
+
+/* ------------------------------------------- PredictorCorrector::Weights */
+
+/* Replaces PredictorCorrector::Weights(n_lambda, n_time, radius)
+ * (integrator.cpp:5-102): host FP64 complex minimum-norm least squares
+ * (one-sided Jacobi SVD).  ps, cs: [2][n_time]; row 1 and *future_coef carry
+ * the step factor (n_time - 1) / 2.  Pure host code; needs no GPU. */
+int quest_pc_weights(int n_lambda, int n_time, double radius, double *ps,
+                     double *cs, double *future_coef);
+
+/* Sets the weights the device integrator uses (n_time must equal the
+ * context's history_window).  Without this call quest_run uses
+ * quest_pc_weights(18, window, 3.15) (main.cpp:49). */
+int quest_set_pc_weights(quest_ctx *ctx, int n_time, const double *ps,
+                         const double *cs, double future_coef);
+
+/* -------------------------------------- PredictorCorrector::Integrator */
+
+/* Replaces Integrator::solve (integrator.cpp:120-127) for n_steps steps
+ * starting at the context's current step, with the interactions main.cpp
+ * wires (pulse first, then history; main.cpp:43-46) and the ground-state
+ * prehistory (main.cpp:32-36).  Everything stays on the device.  On a
+ * non-finite state returns QUEST_ERR_DOMAIN and quest_bad_step gives the
+ * first bad step (integrator.cpp:187-207). */
+int quest_run(quest_ctx *ctx, int n_steps);
+
+/* Step the next quest_run/quest_hist_evaluate call works on. */
+int quest_current_step(const quest_ctx *ctx, int *step);
+
+/* First step with a non-finite state, or -1. */
+int quest_bad_step(const quest_ctx *ctx, int *step);
+
+/* State (rho00, rho01) and derivative of every dot at the last finished step:
+ * the slot history[i][step][0] and [1] (history.h:13-14).  rho, drho: [n][2]
+ * complex; drho may be NULL. */
+int quest_get_state(quest_ctx *ctx, double *rho, double *drho);
+
+/* ------------------------------------------------------ diagnostics */
+
+/* Integer delay floor(|r_obs - r_src| / (c0 dt)) of every unordered pair in
+ * the reference's pair order (history_interaction.cpp:25-34, idx2coord
+ * :80-86), computed by the device's FP64 delay split.  delays: [n(n-1)/2]. */
+int quest_pair_delays(quest_ctx *ctx, int32_t *delays);
+
+/* Enables HIP-event timing of the pair kernel (on the context's stream). */
+int quest_set_timing(quest_ctx *ctx, int enable);
+
+/* Accumulated pair-kernel time (ms) and launch count since timing was
+ * enabled or last reset; reset != 0 clears them after reading. */
+int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
+                           int reset);
+
+/* Integer facts about the context: n_dots, ring slots, max delay, number of
+ * zero-delay (near) pairs, source splits.  info: [8] int64. */
+int quest_ctx_info(const quest_ctx *ctx, int64_t *info);
+
+/* Blocks until all work queued on the context's stream is done. */
+int quest_sync(quest_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,96 @@
+// quest_internal.hpp -- internal declarations shared by the C ABI and the
+// HIP kernels.  Not part of the public interface (include/quest_amd.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/quest_amd.h"
+
+namespace quest {
+
+constexpr int kMaxOrder = 6;       // interpolation orders 1..6 are instantiated
+constexpr int kMaxWindow = 64;     // PC history window limit
+constexpr int kBlock = 256;        // threads per block of the per-dot and pair kernels
+constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
+
+int pc_weights(int n_lambda, int n_time, double radius, double *ps, double *cs,
+               double *future_coef);
+
+// Scalars every kernel of a context shares (all FP64; kernels convert).
+struct Consts {
+  int n;              // dots
+  int order;          // M
+  int kstride;        // complex entries per ring slot (M + 1 rounded up to even)
+  int ring_slots;     // power of two > max delay
+  int ring_mask;
+  int splits;         // source splits of the pair kernel
+  int src_per_split;
+  int tiles;          // target tiles of kBlock dots
+  int state_slots;    // state ring: power of two > max(window, order + 2)
+  int state_mask;
+  double c, dt, cdt, icdt, omega, theta;   // theta = omega dt
+  double k_green;     // mu0 / (4 pi hbar)
+  double hbar, laser_omega;
+  // combination constants of the pair term, see pair_term()
+  double cc, oo, oc, m_cdt, m_o2dt, two_idt2;
+  // monomial coefficients of the Lagrange basis: vinv[k][m] = [x^k] L_m(x)
+  double vinv[(kMaxOrder + 1) * (kMaxOrder + 1)];
+  int dyadic;         // QUEST_DYADIC_*
+};
+
+struct PulseParams {
+  int on;
+  double amplitude, delay, width, freq;
+  double k[3], pol[3];
+};
+
+// Device buffers of one context (sorted dot order).
+struct DeviceState {
+  double4 *pos64 = nullptr, *dip64 = nullptr;   // (x, y, z, 0)
+  float4 *pos32 = nullptr, *dip32 = nullptr;
+  double *freq = nullptr, *t1 = nullptr, *t2 = nullptr;
+  void *ring = nullptr;                // R2 [n][ring_slots][kstride]
+  double2 *y0 = nullptr, *y1 = nullptr;  // state ring [state_slots][n]: rho00, rho01
+  double2 *d0 = nullptr, *d1 = nullptr;  // derivative ring
+  double2 *cur0 = nullptr, *cur1 = nullptr;  // iterate double buffer [2][n]
+  double2 *yc0 = nullptr, *yc1 = nullptr;    // corrector base [n]
+  double2 *field = nullptr;            // fixed part of the field at this step [n]
+  double2 *partial = nullptr;          // [splits][n]
+  double2 *phase = nullptr;            // e^{i theta s}, s in [0, phase_len)
+  double2 *out = nullptr;              // [n] staging of per-call results
+  int *near_start = nullptr;           // [n + 1]
+  int *near_src = nullptr;             // [near]
+  int *near_tgt = nullptr;             // [near]
+  int *near_n = nullptr;               // [near] FP64 integer delay
+  double2 *near_coef = nullptr;        // [near][order + 1]
+  double *pcw = nullptr;               // ps[2W] then cs[2W]
+  int *bad_step = nullptr;             // first non-finite step (INT_MAX = none)
+  int *counts = nullptr;               // scratch [n + 1]
+  int *perm_dev = nullptr;             // inverse permutation (original -> sorted)
+};
+
+// ---------------------------------------------------------------- launchers
+// All launch on `s` and return the hipError_t of the launch.
+hipError_t launch_fill_ground_state(const Consts &k, DeviceState &d, hipStream_t s);
+hipError_t launch_clear_ring(const Consts &k, DeviceState &d, size_t ring_bytes, hipStream_t s);
+hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s);
+hipError_t launch_pc_begin(const Consts &k, DeviceState &d, const PulseParams &pl,
+                           int window, int step, hipStream_t s);
+hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, int window,
+                          double future_coef, int step, int iter, hipStream_t s);
+hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s);
+hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, int step, hipStream_t s);
+hipError_t launch_hist_near(const Consts &k, DeviceState &d, int step, hipStream_t s);
+hipError_t launch_pulse(const Consts &k, DeviceState &d, const PulseParams &pl, int step,
+                        hipStream_t s);
+hipError_t launch_near_count(const Consts &k, DeviceState &d, int precision, hipStream_t s);
+hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipStream_t s);
+hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s);
+hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int32_t *out, int64_t first,
+                              int64_t count, hipStream_t s);
+
+}  // namespace quest

@@ -1,0 +1,817 @@
+// quest_kernels.hip -- gfx950 kernels of the retarded-interaction step.
+//
+// Data layout in HBM (all in the library's Morton-sorted dot order):
+//   pos/dip        (x, y, z, 0) per dot, FP64 and FP32 copies
+//   ring           R2 [n][ring_slots][kstride]: for every source dot and every
+//                  finished step s (slot s & ring_mask) the monomial
+//                  coefficients c_k of its Lagrange interpolant of rho01
+//                  through steps s, s-1, ..., s-M, multiplied by e^{i theta s}
+//                  (rotating frame), in FP64 or FP32
+//   state ring     double2 [state_slots][n] rho00, rho01 and derivatives, for
+//                  the predictor-corrector window
+//
+// The pair sum is a gather: one thread per target dot, sources walked in
+// order.  Source data are wave-uniform (scalar loads); the ring read is a
+// per-lane gather of kstride complex values.  No MFMA: the work is a
+// gather-and-accumulate with ~90 VALU operations per ordered pair.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "quest_internal.hpp"
+
+namespace quest {
+namespace {
+
+// ------------------------------------------------------------------ helpers
+
+template <typename R> struct V2;
+template <> struct V2<double> { using type = double2; };
+template <> struct V2<float> { using type = float2; };
+template <typename R> struct V4;
+template <> struct V4<double> { using type = double4; };
+template <> struct V4<float> { using type = float4; };
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b)
+{
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+
+// FP32 delay split of a pair.  Written with explicit fused operations and
+// contraction off so that every kernel that classifies a pair (the pair sum
+// and the zero-delay list builder) gets bit-identical results.
+__device__ __forceinline__ float tau_f32(float dx, float dy, float dz, float icdt,
+                                         float &r2, float &ir)
+{
+#pragma clang fp contract(off)
+  r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  ir = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * ir;
+  return r * icdt;
+}
+
+// FP64 delay split, bit-identical to the reference's
+//   n = trunc(sqrt((dx^2 + dy^2) + dz^2) / (c0 dt))   (no fused operations)
+// (history_interaction.cpp:31-32, split_double :88-97).  A fast estimate from
+// the hardware reciprocal square root (two Newton steps, relative error a few
+// ulp) decides the integer part unless the estimate lies within 1e-9 of an
+// integer; then the correctly rounded sqrt and division are evaluated.
+// Returns n, the fraction x and 1/r.
+__device__ __forceinline__ int split_f64(double dx, double dy, double dz, double cdt,
+                                         double icdt, double &x, double &ir, double &r2)
+{
+#pragma clang fp contract(off)
+  r2 = (dx * dx + dy * dy) + dz * dz;
+  double y = __builtin_amdgcn_rsq(r2);
+  {
+    const double h = r2 * y;
+    const double e = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+  }
+  {
+    const double h = r2 * y;
+    const double e = __builtin_fma(-h, y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+  }
+  ir = y;
+  const double tau = (r2 * y) * icdt;
+  double fl = __builtin_floor(tau);
+  double fr = tau - fl;
+  if(__builtin_expect(fr < 1e-9 || fr > 1.0 - 1e-9, 0)) {
+    const double r = __builtin_sqrt(r2);
+    const double te = r / cdt;
+    fl = __builtin_floor(te);
+    fr = te - fl;
+  }
+  x = fr;
+  // NaN (coincident dots) -> 0; keeps the conversion defined
+  return static_cast<int>(__builtin_fmax(fl, 0.0));
+}
+
+// sin and cos of u in [0, 0.23): Taylor series (Horner in u^2), truncated
+// where the next term is below the format's rounding error.
+__device__ __forceinline__ void sincos_small(float u, float &s, float &c)
+{
+  const float u2 = u * u;
+  s = u * __builtin_fmaf(u2, __builtin_fmaf(u2, 1.0f / 120.0f, -1.0f / 6.0f), 1.0f);
+  c = __builtin_fmaf(u2, __builtin_fmaf(u2, __builtin_fmaf(u2, -1.0f / 720.0f, 1.0f / 24.0f), -0.5f), 1.0f);
+}
+__device__ __forceinline__ void sincos_small(double u, double &s, double &c)
+{
+  const double u2 = u * u;
+  double ps = -1.0 / 39916800.0;
+  ps = __builtin_fma(ps, u2, 1.0 / 362880.0);
+  ps = __builtin_fma(ps, u2, -1.0 / 5040.0);
+  ps = __builtin_fma(ps, u2, 1.0 / 120.0);
+  ps = __builtin_fma(ps, u2, -1.0 / 6.0);
+  ps = __builtin_fma(ps, u2, 1.0);
+  s = u * ps;
+  double pc = 1.0 / 479001600.0;
+  pc = __builtin_fma(pc, u2, -1.0 / 3628800.0);
+  pc = __builtin_fma(pc, u2, 1.0 / 40320.0);
+  pc = __builtin_fma(pc, u2, -1.0 / 720.0);
+  pc = __builtin_fma(pc, u2, 1.0 / 24.0);
+  pc = __builtin_fma(pc, u2, -0.5);
+  c = __builtin_fma(pc, u2, 1.0);
+}
+
+// Value q0 = p(x), slope q1 = p'(x) and half curvature q2 = p''(x)/2 of the
+// degree-M polynomial sum_k c_k x^k (simultaneous Horner).
+template <typename R, int M>
+__device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q0r, R &q0i,
+                                        R &q1r, R &q1i, R &q2r, R &q2i)
+{
+  q0r = c[M].x;
+  q0i = c[M].y;
+  q1r = 0;
+  q1i = 0;
+  q2r = 0;
+  q2i = 0;
+  if constexpr(M >= 1) {
+    q1r = q0r;
+    q1i = q0i;
+    q0r = __builtin_fma(q0r, x, c[M - 1].x);
+    q0i = __builtin_fma(q0i, x, c[M - 1].y);
+  }
+  if constexpr(M >= 2) {
+    q2r = q1r;
+    q2i = q1i;
+    q1r = __builtin_fma(q1r, x, q0r);
+    q1i = __builtin_fma(q1i, x, q0i);
+    q0r = __builtin_fma(q0r, x, c[M - 2].x);
+    q0i = __builtin_fma(q0i, x, c[M - 2].y);
+  }
+#pragma unroll
+  for(int k = M - 3; k >= 0; --k) {
+    q2r = __builtin_fma(q2r, x, q1r);
+    q2i = __builtin_fma(q2i, x, q1i);
+    q1r = __builtin_fma(q1r, x, q0r);
+    q1i = __builtin_fma(q1i, x, q0i);
+    q0r = __builtin_fma(q0r, x, c[k].x);
+    q0i = __builtin_fma(q0i, x, c[k].y);
+  }
+}
+
+// Pair constants converted to the pair arithmetic's type.
+template <typename R> struct PairK {
+  R icdt, cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta;
+  double cdt;
+  __host__ __device__ explicit PairK(const Consts &k)
+      : icdt((R)k.icdt), cc((R)k.cc), oo((R)k.oo), oc((R)k.oc), m_cdt((R)k.m_cdt),
+        m_o2dt((R)k.m_o2dt), two_idt2((R)k.two_idt2), theta((R)k.theta), cdt(k.cdt) {}
+};
+
+// The field at a target from one source, given the geometry and the source's
+// interpolant at the retarded time (before the global factor -K e^{-i theta step}):
+//   G . = e^{-i theta x} [ T0 p + T1 (p' + i w p) + T2 (p'' + 2 i w p' - w^2 p) ]
+// with time derivatives p' = -q1/dt, p'' = 2 q2/dt^2, T0 = c^2 g3/r,
+// T1 = c g3, T2 = g1, g3 = (d_a.d_b - 3 B)/r^2, g1 = (d_a.d_b - B)/r,
+// B = (d_a.dr)(d_b.dr)/r^2 (rotating_green_function.cpp:13-20 projected on the
+// dipoles, history_interaction.cpp:40-43).  LAB drops the rotation and the
+// i w mixing (green_function.cpp:12-17).
+template <typename R, bool LAB>
+__device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R dz,
+                                          const typename V4<R>::type &da,
+                                          const typename V4<R>::type &db, R x, R q0r, R q0i,
+                                          R q1r, R q1i, R q2r, R q2i, R &outr, R &outi)
+{
+  const R ir2 = ir * ir;
+  const R ba = da.x * dx + da.y * dy + da.z * dz;
+  const R bb = db.x * dx + db.y * dy + db.z * dz;
+  const R aab = da.x * db.x + da.y * db.y + da.z * db.z;
+  const R bq = ba * bb * ir2;
+  const R g3 = (aab - 3 * bq) * ir2;
+  const R g1 = (aab - bq) * ir;
+  const R a1 = K.m_cdt * g3;
+  const R a2 = K.two_idt2 * g1;
+  if constexpr(LAB) {
+    const R a0 = K.cc * ir * g3;
+    outr = a0 * q0r + a1 * q1r + a2 * q2r;
+    outi = a0 * q0i + a1 * q1i + a2 * q2i;
+  } else {
+    const R a0 = K.cc * ir * g3 - K.oo * g1;
+    const R b0 = K.oc * g3;
+    const R b1 = K.m_o2dt * g1;
+    const R cr = a0 * q0r - b0 * q0i + a1 * q1r - b1 * q1i + a2 * q2r;
+    const R ci = a0 * q0i + b0 * q0r + a1 * q1i + b1 * q1r + a2 * q2i;
+    R sn, cs;
+    sincos_small(K.theta * x, sn, cs);
+    outr = cr * cs + ci * sn;
+    outi = ci * cs - cr * sn;
+  }
+}
+
+// ------------------------------------------------------------ pair kernels
+
+// Past-only pair sum: every ordered pair (a <- b), b != a, whose integer delay
+// n >= 1 reads finished steps only, so this sum is formed once per step.  The
+// zero-delay pairs are excluded here and evaluated at every corrector
+// iteration from the near list.  Grid: tiles * splits blocks of kBlock
+// threads; block id -> (split = id % splits, tile = id / splits), so with 8
+// splits each XCD (round-robin block dispatch) walks one source range.
+constexpr int kChunk = 64;   // sources summed in the pair type before the FP64 fold
+
+template <int M, bool LAB>
+__global__ __launch_bounds__(kBlock) void pair_past_f64(
+    const double4 *__restrict__ pos, const double4 *__restrict__ dip,
+    const double2 *__restrict__ ring, double2 *__restrict__ partial, Consts k, int step)
+{
+  const PairK<double> K(k);
+  const int split = blockIdx.x % k.splits;
+  const int tile = blockIdx.x / k.splits;
+  const int tgt = tile * kBlock + threadIdx.x;
+  const int a = tgt < k.n ? tgt : k.n - 1;
+  const double4 pa = pos[a], da = dip[a];
+  const int b0 = split * k.src_per_split;
+  const int b1 = min(k.n, b0 + k.src_per_split);
+  double accr = 0, acci = 0;
+  for(int b = b0; b < b1; ++b) {
+    const double4 pb = pos[b], db = dip[b];
+    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+    double x, ir, r2;
+    const int n = split_f64(dx, dy, dz, K.cdt, K.icdt, x, ir, r2);
+    const bool valid = (b != a) && (n >= 1);
+    const int slot = (step - n) & k.ring_mask;
+    const double2 *c = ring + (static_cast<size_t>(b) * k.ring_slots + slot) * k.kstride;
+    double2 cv[M + 1];
+#pragma unroll
+    for(int i = 0; i <= M; ++i) cv[i] = c[i];
+    double q0r, q0i, q1r, q1i, q2r, q2i;
+    horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
+    double tr, ti;
+    pair_term<double, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
+    accr += valid ? tr : 0.0;
+    acci += valid ? ti : 0.0;
+  }
+  if(tgt < k.n) partial[static_cast<size_t>(split) * k.n + tgt] = make_double2(accr, acci);
+}
+
+template <int M, bool LAB>
+__global__ __launch_bounds__(kBlock) void pair_past_f32(
+    const float4 *__restrict__ pos, const float4 *__restrict__ dip,
+    const float2 *__restrict__ ring, double2 *__restrict__ partial, Consts k, int step)
+{
+  const PairK<float> K(k);
+  const int split = blockIdx.x % k.splits;
+  const int tile = blockIdx.x / k.splits;
+  const int tgt = tile * kBlock + threadIdx.x;
+  const int a = tgt < k.n ? tgt : k.n - 1;
+  const float4 pa = pos[a], da = dip[a];
+  const int b0 = split * k.src_per_split;
+  const int b1 = min(k.n, b0 + k.src_per_split);
+  double accr = 0, acci = 0;
+  for(int bt = b0; bt < b1; bt += kChunk) {
+    const int be = min(b1, bt + kChunk);
+    float sr = 0, si = 0;
+    for(int b = bt; b < be; ++b) {
+      const float4 pb = pos[b], db = dip[b];
+      const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+      float r2, ir;
+      const float tau = tau_f32(dx, dy, dz, K.icdt, r2, ir);
+      const float fl = __builtin_floorf(tau);
+      const float x = tau - fl;
+      const int n = static_cast<int>(__builtin_fmaxf(fl, 0.0f));
+      const bool valid = (b != a) && (n >= 1);
+      const int slot = (step - n) & k.ring_mask;
+      const float2 *c = ring + (static_cast<size_t>(b) * k.ring_slots + slot) * k.kstride;
+      float2 cv[M + 1];
+#pragma unroll
+      for(int i = 0; i <= M; ++i) cv[i] = c[i];
+      float q0r, q0i, q1r, q1i, q2r, q2i;
+      horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
+      float tr, ti;
+      pair_term<float, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
+      sr += valid ? tr : 0.0f;
+      si += valid ? ti : 0.0f;
+    }
+    accr += static_cast<double>(sr);
+    acci += static_cast<double>(si);
+  }
+  if(tgt < k.n) partial[static_cast<size_t>(split) * k.n + tgt] = make_double2(accr, acci);
+}
+
+// ------------------------------------------------------- per-dot kernels
+
+// Field scale of the pair sum at `step`: -K e^{-i theta step} (rotating) or K (lab).
+__device__ __forceinline__ double2 hist_scale(const Consts &k, const double2 *phase, int step)
+{
+  if(k.dyadic == QUEST_DYADIC_LAB) return make_double2(k.k_green, 0.0);
+  const double2 ph = phase[step];
+  return make_double2(-k.k_green * ph.x, k.k_green * ph.y);
+}
+
+__device__ __forceinline__ double2 reduce_partials(const Consts &k, const double2 *partial, int a)
+{
+  double2 s = make_double2(0.0, 0.0);
+  for(int j = 0; j < k.splits; ++j) s = cadd(s, partial[static_cast<size_t>(j) * k.n + a]);
+  return s;
+}
+
+// Pulse::operator() projected on the dipole (pulse.cpp:15-20,
+// pulse_interaction.cpp:11-16, gaussian math_utils.cpp:27-30).
+__device__ __forceinline__ double2 pulse_field(const PulseParams &pl, double hbar, double4 r,
+                                               double4 d, double t)
+{
+  if(!pl.on) return make_double2(0.0, 0.0);
+  const double kr = (pl.k[0] * r.x + pl.k[1] * r.y) + pl.k[2] * r.z;
+  const double arg = kr - pl.freq * (t - pl.delay);
+  const double u = arg / pl.width;
+  const double g = exp(-(u * u) / 2);
+  const double e0 = (pl.amplitude / 2 * pl.pol[0]) * g;
+  const double e1 = (pl.amplitude / 2 * pl.pol[1]) * g;
+  const double e2 = (pl.amplitude / 2 * pl.pol[2]) * g;
+  return make_double2(((e0 * d.x + e1 * d.y) + e2 * d.z) / hbar, 0.0);
+}
+
+// QuantumDot::liouville_rhs (quantum_dot.cpp:10-19).
+__device__ __forceinline__ void bloch_rhs(double2 r0, double2 r1, double2 om, double laser,
+                                          double freq, double t1, double t2, double2 &m0,
+                                          double2 &m1)
+{
+  // z = om conj(r1) - conj(om) r1
+  const double2 a = make_double2(om.x * r1.x + om.y * r1.y, om.y * r1.x - om.x * r1.y);
+  const double2 b = make_double2(om.x * r1.x + om.y * r1.y, om.x * r1.y - om.y * r1.x);
+  const double2 z = make_double2(a.x - b.x, a.y - b.y);
+  // -i z - (r0 - 1) / t1
+  m0 = make_double2(z.y - (r0.x - 1.0) / t1, -z.x - r0.y / t1);
+  // w = om (1 - 2 r0) + r1 (laser - freq);  -i w - r1 / t2
+  const double2 f = make_double2(1.0 - 2.0 * r0.x, -2.0 * r0.y);
+  const double det = laser - freq;
+  const double2 w = make_double2(om.x * f.x - om.y * f.y + r1.x * det,
+                                 om.x * f.y + om.y * f.x + r1.y * det);
+  m1 = make_double2(w.y - r1.x / t2, -w.x - r1.y / t2);
+}
+
+__device__ __forceinline__ bool finite2(double2 v) { return isfinite(v.x) && isfinite(v.y); }
+
+// rho01 of dot b at time t, seen from the evaluation of step `step`: the
+// current iterate for t == step, the state ring for finished steps (the ring
+// holds the ground state, rho01 = 0, for t < 0).
+__device__ __forceinline__ double2 rho01_at(const Consts &k, const double2 *cur1,
+                                            const double2 *y1, int b, int t, int step)
+{
+  if(t == step) return cur1[b];
+  return y1[static_cast<size_t>(t & k.state_mask) * k.n + b];
+}
+
+// Zero-delay (near) pairs of target a, evaluated with the reference's
+// per-pair coefficients (history_interaction.cpp:56-66, one direction).
+__device__ __forceinline__ double2 near_sum(const Consts &k, const DeviceState &d,
+                                            const double2 *cur1, int a, int step)
+{
+  double2 acc = make_double2(0.0, 0.0);
+  const int e0 = d.near_start[a], e1 = d.near_start[a + 1];
+  for(int e = e0; e < e1; ++e) {
+    const int b = d.near_src[e];
+    const int nn = d.near_n[e];
+    const double2 *cf = d.near_coef + static_cast<size_t>(e) * (k.order + 1);
+    for(int m = 0; m <= k.order; ++m) {
+      double2 rho = rho01_at(k, cur1, d.y1, b, step - nn - m, step);
+      if(k.dyadic == QUEST_DYADIC_LAB) rho = make_double2(2.0 * rho.x, 0.0);
+      acc = cadd(acc, cmul(rho, cf[m]));
+    }
+  }
+  return acc;
+}
+
+// Writes the interpolant coefficients of slot s for source a:
+// c_k = e^{i theta s} sum_m vinv[k][m] rho01(s - m)  (rotating), or of
+// 2 Re rho01 without the phase (lab).
+template <typename R2>
+__device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase, R2 *ring,
+                                           const double2 *y1, double2 rho_s, int a, int s)
+{
+  double2 smp[kMaxOrder + 1];
+  smp[0] = rho_s;
+  for(int m = 1; m <= k.order; ++m) smp[m] = y1[static_cast<size_t>((s - m) & k.state_mask) * k.n + a];
+  const bool lab = k.dyadic == QUEST_DYADIC_LAB;
+  if(lab)
+    for(int m = 0; m <= k.order; ++m) smp[m] = make_double2(2.0 * smp[m].x, 0.0);
+  const double2 ph = lab ? make_double2(1.0, 0.0) : phase[s];
+  R2 *dst = ring + (static_cast<size_t>(a) * k.ring_slots + (s & k.ring_mask)) * k.kstride;
+  for(int kk = 0; kk <= k.order; ++kk) {
+    double2 c = make_double2(0.0, 0.0);
+    for(int m = 0; m <= k.order; ++m) {
+      const double v = k.vinv[kk * (kMaxOrder + 1) + m];
+      c.x += v * smp[m].x;
+      c.y += v * smp[m].y;
+    }
+    c = cmul(c, ph);
+    dst[kk].x = c.x;
+    dst[kk].y = c.y;
+  }
+}
+
+// Ground state in every state-ring slot (main.cpp:32-36), zero derivatives.
+__global__ void fill_ground_state_kernel(Consts k, double2 *y0, double2 *y1, double2 *d0,
+                                         double2 *d1, int *bad_step)
+{
+  const size_t total = static_cast<size_t>(k.state_slots) * k.n;
+  for(size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < total;
+      i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    y0[i] = make_double2(1.0, 0.0);
+    y1[i] = make_double2(0.0, 0.0);
+    d0[i] = make_double2(0.0, 0.0);
+    d1[i] = make_double2(0.0, 0.0);
+  }
+  if(blockIdx.x == 0 && threadIdx.x == 0) *bad_step = INT_MAX;
+}
+
+__global__ void clear_words_kernel(uint4 *p, size_t n16)
+{
+  for(size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n16;
+      i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = make_uint4(0, 0, 0, 0);
+}
+
+// Start of a step (whole-run mode): fixed field (pulse + past pair sum), the
+// predictor (integrator.cpp:142-154) and the step-invariant part of the
+// corrector (integrator.cpp:156-169).
+__global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState d, PulseParams pl,
+                                                          int window, int step)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  const double dt = k.dt;
+  double2 hist = make_double2(0.0, 0.0);
+  if(k.n > 1) hist = cmul(reduce_partials(k, d.partial, a), hist_scale(k, d.phase, step));
+  const double2 pulse = pulse_field(pl, k.hbar, d.pos64[a], d.dip64[a], step * dt);
+  d.field[a] = cadd(pulse, hist);
+
+  const double *ps0 = d.pcw, *ps1 = d.pcw + window;
+  const double *cs0 = d.pcw + 2 * window, *cs1 = d.pcw + 3 * window;
+  double2 p0 = make_double2(0.0, 0.0), p1 = p0, c0 = p0, c1 = p0;
+  const int start = step - window;
+  for(int h = 0; h < window; ++h) {
+    const size_t o = static_cast<size_t>((start + h) & k.state_mask) * k.n + a;
+    const double2 y0 = d.y0[o], y1 = d.y1[o], g0 = d.d0[o], g1 = d.d1[o];
+    const double pw = ps0[h], pd = ps1[h] * dt, cw = cs0[h], cd = cs1[h] * dt;
+    p0.x += y0.x * pw + g0.x * pd;
+    p0.y += y0.y * pw + g0.y * pd;
+    p1.x += y1.x * pw + g1.x * pd;
+    p1.y += y1.y * pw + g1.y * pd;
+    c0.x += y0.x * cw + g0.x * cd;
+    c0.y += y0.y * cw + g0.y * cd;
+    c1.x += y1.x * cw + g1.x * cd;
+    c1.y += y1.y * cw + g1.y * cd;
+  }
+  d.cur0[a] = p0;
+  d.cur1[a] = p1;
+  d.yc0[a] = c0;
+  d.yc1[a] = c1;
+}
+
+// One evaluation of the step (integrator.cpp:171-185) and the next corrector
+// update (integrator.cpp:156-169) for iterations 0..9; iteration 10 commits
+// the step to the state ring, checks finiteness (integrator.cpp:187-207) and
+// writes the source's interpolant slot for the next steps' pair sums.
+template <typename R2>
+__global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d, R2 *ring,
+                                                         double future_coef, int step, int iter)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  const int rd = iter & 1, wr = rd ^ 1;
+  const double2 *cur0 = d.cur0 + static_cast<size_t>(rd) * k.n;
+  const double2 *cur1 = d.cur1 + static_cast<size_t>(rd) * k.n;
+  const double2 y0 = cur0[a], y1 = cur1[a];
+  double2 om = d.field[a];
+  if(d.near_start[a + 1] > d.near_start[a]) om = cadd(om, near_sum(k, d, cur1, a, step));
+  double2 m0, m1;
+  bloch_rhs(y0, y1, om, k.laser_omega, d.freq[a], d.t1[a], d.t2[a], m0, m1);
+  if(iter < kNumCorrectorSteps) {
+    const double f = future_coef * k.dt;
+    d.cur0[static_cast<size_t>(wr) * k.n + a] = make_double2(f * m0.x + d.yc0[a].x, f * m0.y + d.yc0[a].y);
+    d.cur1[static_cast<size_t>(wr) * k.n + a] = make_double2(f * m1.x + d.yc1[a].x, f * m1.y + d.yc1[a].y);
+    return;
+  }
+  const size_t o = static_cast<size_t>(step & k.state_mask) * k.n + a;
+  d.y0[o] = y0;
+  d.y1[o] = y1;
+  d.d0[o] = m0;
+  d.d1[o] = m1;
+  if(!(finite2(y0) && finite2(y1) && finite2(m0) && finite2(m1))) atomicMin(d.bad_step, step);
+  build_slot(k, d.phase, ring, d.y1, y1, a, step);
+}
+
+// Per-call mode: the values passed for `step` become final -- state ring
+// slot and interpolant slot of every source.
+template <typename R2>
+__global__ __launch_bounds__(kBlock) void hist_commit_kernel(Consts k, DeviceState d, R2 *ring,
+                                                             int step)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  const double2 y1 = d.cur1[a];
+  d.y1[static_cast<size_t>(step & k.state_mask) * k.n + a] = y1;
+  build_slot(k, d.phase, ring, d.y1, y1, a, step);
+}
+
+// Per-call mode: the pair sum of the finished history, scaled.
+__global__ __launch_bounds__(kBlock) void hist_reduce_kernel(Consts k, DeviceState d, int step)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  double2 hist = make_double2(0.0, 0.0);
+  if(k.n > 1) hist = cmul(reduce_partials(k, d.partial, a), hist_scale(k, d.phase, step));
+  d.field[a] = hist;
+}
+
+// Per-call mode: out = past pair sum + zero-delay pairs with the current rho01.
+__global__ __launch_bounds__(kBlock) void hist_near_kernel(Consts k, DeviceState d, int step)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  double2 om = d.field[a];
+  if(d.near_start[a + 1] > d.near_start[a]) om = cadd(om, near_sum(k, d, d.cur1, a, step));
+  d.out[a] = om;
+}
+
+__global__ __launch_bounds__(kBlock) void pulse_kernel(Consts k, DeviceState d, PulseParams pl,
+                                                       int step)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  d.out[a] = pulse_field(pl, k.hbar, d.pos64[a], d.dip64[a], step * k.dt);
+}
+
+// ------------------------------------------------------- zero-delay list
+
+// Number (FILL = false) or list (FILL = true, in source order) of the sources
+// of each target whose pair the pair kernel of this precision classifies as
+// zero-delay.
+template <bool F64, bool FILL>
+__global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState d)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  int cnt = 0;
+  int base = FILL ? d.near_start[a] : 0;
+  if constexpr(F64) {
+    const double4 pa = d.pos64[a];
+    for(int b = 0; b < k.n; ++b) {
+      const double4 pb = d.pos64[b];
+      double x, ir, r2;
+      const int n = split_f64(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, k.cdt, k.icdt, x, ir, r2);
+      if(b != a && n < 1) {
+        if(FILL) d.near_src[base + cnt] = b;
+        ++cnt;
+      }
+    }
+  } else {
+    const float4 pa = d.pos32[a];
+    const float icdt = static_cast<float>(k.icdt);
+    for(int b = 0; b < k.n; ++b) {
+      const float4 pb = d.pos32[b];
+      float r2, ir;
+      const float tau = tau_f32(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, icdt, r2, ir);
+      const int n = static_cast<int>(__builtin_fmaxf(__builtin_floorf(tau), 0.0f));
+      if(b != a && n < 1) {
+        if(FILL) d.near_src[base + cnt] = b;
+        ++cnt;
+      }
+    }
+  }
+  if(!FILL) d.counts[a] = cnt;
+  else
+    for(int e = 0; e < cnt; ++e) d.near_tgt[base + e] = a;
+}
+
+// Lagrange weights at x on nodes 0..M and their first and second x-derivatives,
+// as products (no division by x - m, so x on a node is regular).
+__device__ void lagrange_products(double x, int M, double *l0, double *l1, double *l2)
+{
+  for(int b = 0; b <= M; ++b) {
+    double den = 1;
+    for(int m = 0; m <= M; ++m)
+      if(m != b) den *= static_cast<double>(b - m);
+    double v = 1, d1 = 0, d2 = 0;
+    for(int m = 0; m <= M; ++m) {
+      if(m == b) continue;
+      const double f = x - m;
+      d2 = d2 * f + 2 * d1;
+      d1 = d1 * f + v;
+      v = v * f;
+    }
+    l0[b] = v / den;
+    l1[b] = d1 / den;
+    l2[b] = d2 / den;
+  }
+}
+
+// FP64 per-pair coefficients of the zero-delay list: the reference's
+// coefficients[pair][m] (history_interaction.cpp:29-43 with
+// lagrange_set.cpp:17-41 and rotating_green_function.cpp:3-24 /
+// green_function.cpp:3-20), target a <- source b.
+__global__ __launch_bounds__(kBlock) void near_coef_kernel(Consts k, DeviceState d, int total)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= total) return;
+  const int a = d.near_tgt[e], b = d.near_src[e];
+  const double4 pa = d.pos64[a], pb = d.pos64[b], da = d.dip64[a], db = d.dip64[b];
+  const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+  double x, ir, r2;
+  const int n = split_f64(dx, dy, dz, k.cdt, k.icdt, x, ir, r2);
+  d.near_n[e] = n;
+  const int M = k.order;
+  double l0[kMaxOrder + 1], l1[kMaxOrder + 1], l2[kMaxOrder + 1];
+  lagrange_products(x, M, l0, l1, l2);
+  const double r = sqrt(r2);
+  const double ba = da.x * dx + da.y * dy + da.z * dz;
+  const double bb = db.x * dx + db.y * dy + db.z * dz;
+  const double aab = da.x * db.x + da.y * db.y + da.z * db.z;
+  const double bq = ba * bb / r2;
+  const double T0 = k.c * k.c * (aab - 3 * bq) / (r2 * r);
+  const double T1 = k.c * (aab - 3 * bq) / r2;
+  const double T2 = (aab - bq) / r;
+  const double w = k.omega;
+  double2 ph = make_double2(1.0, 0.0);
+  if(k.dyadic != QUEST_DYADIC_LAB) {
+    double s, c;
+    sincos(-w * r / k.c, &s, &c);
+    ph = make_double2(-k.k_green * c, -k.k_green * s);
+  } else {
+    ph = make_double2(k.k_green, 0.0);
+  }
+  double2 *out = d.near_coef + static_cast<size_t>(e) * (M + 1);
+  for(int m = 0; m <= M; ++m) {
+    // time-derivative weights: w1 = -L'(x)/dt, w2 = L''(x)/dt^2
+    const double w0 = l0[m], w1 = -l1[m] / k.dt, w2 = l2[m] / (k.dt * k.dt);
+    double2 g;
+    if(k.dyadic == QUEST_DYADIC_LAB) g = make_double2(T0 * w0 + T1 * w1 + T2 * w2, 0.0);
+    else
+      g = make_double2(T0 * w0 + T1 * w1 + T2 * (w2 - w * w * w0), T1 * w * w0 + T2 * 2.0 * w * w1);
+    out[m] = cmul(ph, g);
+  }
+}
+
+// Diagnostic: FP64 integer delays of unordered pairs [first, first + count) in
+// the reference's lower-triangle order (history_interaction.cpp:80-86).
+__global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_t first,
+                                   int64_t count)
+{
+  for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
+      i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t idx = first + i;
+    int64_t row = static_cast<int64_t>(floor((sqrt(static_cast<double>(1 + 8 * idx)) + 1) / 2.0));
+    while(row * (row - 1) / 2 > idx) --row;
+    while((row + 1) * row / 2 <= idx) ++row;
+    const int64_t col = idx - row * (row - 1) / 2;
+    const int rs = d.perm_dev[row], cs = d.perm_dev[col];
+    const double4 pr = d.pos64[rs], pc = d.pos64[cs];
+    double x, ir, r2;
+    out[i] = split_f64(pc.x - pr.x, pc.y - pr.y, pc.z - pr.z, k.cdt, k.icdt, x, ir, r2);
+  }
+}
+
+inline dim3 dot_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
+
+template <int M>
+hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s)
+{
+  const dim3 grid(k.tiles * k.splits), block(kBlock);
+  const bool lab = k.dyadic == QUEST_DYADIC_LAB;
+  if(precision == QUEST_PREC_FP64) {
+    if(lab)
+      hipLaunchKernelGGL((pair_past_f64<M, true>), grid, block, 0, s, d.pos64, d.dip64,
+                         static_cast<const double2 *>(d.ring), d.partial, k, step);
+    else
+      hipLaunchKernelGGL((pair_past_f64<M, false>), grid, block, 0, s, d.pos64, d.dip64,
+                         static_cast<const double2 *>(d.ring), d.partial, k, step);
+  } else {
+    if(lab)
+      hipLaunchKernelGGL((pair_past_f32<M, true>), grid, block, 0, s, d.pos32, d.dip32,
+                         static_cast<const float2 *>(d.ring), d.partial, k, step);
+    else
+      hipLaunchKernelGGL((pair_past_f32<M, false>), grid, block, 0, s, d.pos32, d.dip32,
+                         static_cast<const float2 *>(d.ring), d.partial, k, step);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_fill_ground_state(const Consts &k, DeviceState &d, hipStream_t s)
+{
+  const size_t total = static_cast<size_t>(k.state_slots) * k.n;
+  const int blocks = static_cast<int>(std::min<size_t>((total + kBlock - 1) / kBlock, 4096));
+  hipLaunchKernelGGL(fill_ground_state_kernel, dim3(blocks), dim3(kBlock), 0, s, k, d.y0, d.y1,
+                     d.d0, d.d1, d.bad_step);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_ring(const Consts &k, DeviceState &d, size_t ring_bytes, hipStream_t s)
+{
+  (void)k;
+  const size_t n16 = ring_bytes / 16;
+  const int blocks = static_cast<int>(std::min<size_t>((n16 + kBlock - 1) / kBlock, 8192));
+  hipLaunchKernelGGL(clear_words_kernel, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s,
+                     static_cast<uint4 *>(d.ring), n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s)
+{
+  switch(k.order) {
+  case 1: return launch_pair_m<1>(k, d, precision, step, s);
+  case 2: return launch_pair_m<2>(k, d, precision, step, s);
+  case 3: return launch_pair_m<3>(k, d, precision, step, s);
+  case 4: return launch_pair_m<4>(k, d, precision, step, s);
+  case 5: return launch_pair_m<5>(k, d, precision, step, s);
+  case 6: return launch_pair_m<6>(k, d, precision, step, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_pc_begin(const Consts &k, DeviceState &d, const PulseParams &pl, int window,
+                           int step, hipStream_t s)
+{
+  hipLaunchKernelGGL(pc_begin_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, pl, window, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, int window,
+                          double future_coef, int step, int iter, hipStream_t s)
+{
+  (void)window;
+  if(precision == QUEST_PREC_FP64)
+    hipLaunchKernelGGL(pc_iter_kernel<double2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
+                       static_cast<double2 *>(d.ring), future_coef, step, iter);
+  else
+    hipLaunchKernelGGL(pc_iter_kernel<float2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
+                       static_cast<float2 *>(d.ring), future_coef, step, iter);
+  return hipGetLastError();
+}
+
+hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s)
+{
+  if(precision == QUEST_PREC_FP64)
+    hipLaunchKernelGGL(hist_commit_kernel<double2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
+                       static_cast<double2 *>(d.ring), step);
+  else
+    hipLaunchKernelGGL(hist_commit_kernel<float2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
+                       static_cast<float2 *>(d.ring), step);
+  return hipGetLastError();
+}
+
+hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, int step, hipStream_t s)
+{
+  hipLaunchKernelGGL(hist_reduce_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_hist_near(const Consts &k, DeviceState &d, int step, hipStream_t s)
+{
+  hipLaunchKernelGGL(hist_near_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_pulse(const Consts &k, DeviceState &d, const PulseParams &pl, int step,
+                        hipStream_t s)
+{
+  hipLaunchKernelGGL(pulse_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, pl, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_near_count(const Consts &k, DeviceState &d, int precision, hipStream_t s)
+{
+  if(precision == QUEST_PREC_FP64)
+    hipLaunchKernelGGL((near_scan_kernel<true, false>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
+  else
+    hipLaunchKernelGGL((near_scan_kernel<false, false>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipStream_t s)
+{
+  if(precision == QUEST_PREC_FP64)
+    hipLaunchKernelGGL((near_scan_kernel<true, true>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
+  else
+    hipLaunchKernelGGL((near_scan_kernel<false, true>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s)
+{
+  if(near_total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(near_coef_kernel, dim3((near_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     s, k, d, near_total);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int32_t *out, int64_t first,
+                              int64_t count, hipStream_t s)
+{
+  const int blocks = static_cast<int>(std::min<int64_t>((count + kBlock - 1) / kBlock, 16384));
+  hipLaunchKernelGGL(pair_delays_kernel, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s, k, d,
+                     out, first, count);
+  return hipGetLastError();
+}
+
+}  // namespace quest

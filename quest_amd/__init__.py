@@ -1,0 +1,11 @@
+"""MI355X (gfx950) implementation of QuEST's retarded-interaction step.
+
+The product is libquest_amd.so (C ABI: include/quest_amd.h, HIP kernels in
+csrc/); this package is its Python front end (capi) and the seeded system
+generators (systems).
+"""
+from .capi import (PREC_FP64, PREC_MIXED, DYADIC_LAB, DYADIC_ROTATING, Context, QuestError,
+                   UnboundedSolution, load, pc_weights)
+
+__all__ = ["PREC_FP64", "PREC_MIXED", "DYADIC_LAB", "DYADIC_ROTATING", "Context", "QuestError",
+           "UnboundedSolution", "load", "pc_weights"]

@@ -1,0 +1,242 @@
+"""ctypes bindings of libquest_amd.so (the C ABI in include/quest_amd.h).
+
+The Python mirror of the reference's interaction and integrator interface:
+
+* ``Context.hist_evaluate(step, rho01)`` -- HistoryInteraction::evaluate
+  (src/interactions/history_interaction.cpp:47-70)
+* ``Context.pulse_evaluate(step)``      -- PulseInteraction::evaluate
+  (src/interactions/pulse_interaction.cpp:9-20)
+* ``Context.run(n_steps)``              -- PredictorCorrector::Integrator::solve
+  (src/integrator.cpp:120-127), whole run on the device
+* ``pc_weights(18, 22, 3.15)``          -- PredictorCorrector::Weights
+  (src/integrator.cpp:85-102)
+
+Errors: a non-finite state raises ``UnboundedSolution`` (a ``ValueError``,
+the Python spelling of the reference's std::domain_error) with the
+reference's message "unbounded history values at or before step N"
+(integrator.cpp:200-207); every other failure raises ``QuestError``
+(RuntimeError).  There is no CPU fallback: without the built library every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libquest_amd.so")
+
+QUEST_OK, QUEST_ERR_ARG, QUEST_ERR_HIP, QUEST_ERR_DOMAIN, QUEST_ERR_STATE, QUEST_ERR_NOMEM = range(6)
+PREC_FP64, PREC_MIXED = 0, 1
+DYADIC_ROTATING, DYADIC_LAB = 0, 1
+
+# Every symbol include/quest_amd.h declares.
+EXPORTS = [
+    "quest_ctx_create", "quest_ctx_destroy", "quest_last_error", "quest_hist_evaluate",
+    "quest_set_pulse", "quest_pulse_evaluate", "quest_pc_weights", "quest_set_pc_weights",
+    "quest_run", "quest_current_step", "quest_bad_step", "quest_get_state",
+    "quest_pair_delays", "quest_set_timing", "quest_pair_kernel_time", "quest_ctx_info",
+    "quest_sync",
+]
+
+
+class QuestError(RuntimeError):
+    pass
+
+
+class UnboundedSolution(ValueError):
+    """std::domain_error of Integrator::throw_if_unbounded_solution."""
+
+    def __init__(self, msg, step):
+        super().__init__(msg)
+        self.step = step
+
+
+class Params(C.Structure):
+    _fields_ = [("c0", C.c_double), ("hbar", C.c_double), ("mu0", C.c_double),
+                ("laser_omega", C.c_double), ("dt", C.c_double), ("interp_order", C.c_int),
+                ("history_window", C.c_int), ("precision", C.c_int), ("dyadic", C.c_int),
+                ("device", C.c_int), ("source_split", C.c_int)]
+
+
+class Pulse(C.Structure):
+    _fields_ = [("amplitude", C.c_double), ("delay", C.c_double), ("width", C.c_double),
+                ("freq", C.c_double), ("wavevector", C.c_double * 3),
+                ("polarization", C.c_double * 3)]
+
+
+_lib = None
+
+
+def load():
+    """The loaded library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise QuestError(f"{LIB_PATH} is missing: build it with `python -m quest_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    vp = C.c_void_p
+    L.quest_ctx_create.argtypes = [C.POINTER(Params), C.c_int, dp, dp, dp, dp, dp, C.POINTER(vp)]
+    L.quest_ctx_destroy.argtypes = [vp]
+    L.quest_ctx_destroy.restype = None
+    L.quest_last_error.argtypes = [vp]
+    L.quest_last_error.restype = C.c_char_p
+    L.quest_hist_evaluate.argtypes = [vp, C.c_int, dp, dp]
+    L.quest_set_pulse.argtypes = [vp, C.POINTER(Pulse)]
+    L.quest_pulse_evaluate.argtypes = [vp, C.c_int, dp]
+    L.quest_pc_weights.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp, dp]
+    L.quest_set_pc_weights.argtypes = [vp, C.c_int, dp, dp, C.c_double]
+    L.quest_run.argtypes = [vp, C.c_int]
+    L.quest_current_step.argtypes = [vp, ip]
+    L.quest_bad_step.argtypes = [vp, ip]
+    L.quest_get_state.argtypes = [vp, dp, dp]
+    L.quest_pair_delays.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.quest_set_timing.argtypes = [vp, C.c_int]
+    L.quest_pair_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int]
+    L.quest_ctx_info.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.quest_sync.argtypes = [vp]
+    for name in EXPORTS:
+        if name not in ("quest_ctx_destroy", "quest_last_error"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def pc_weights(n_lambda: int = 18, n_time: int = 22, radius: float = 3.15):
+    """PredictorCorrector::Weights (integrator.cpp:85-102): (ps [2][n_time], cs, future_coef)."""
+    L = load()
+    ps = np.zeros((2, n_time))
+    cs = np.zeros((2, n_time))
+    fc = C.c_double()
+    rc = L.quest_pc_weights(n_lambda, n_time, radius, _dp(ps), _dp(cs), C.byref(fc))
+    if rc != QUEST_OK:
+        raise QuestError(L.quest_last_error(None).decode())
+    return ps, cs, fc.value
+
+
+class Context:
+    """One dot system on one GPU (a quest_ctx)."""
+
+    def __init__(self, system, consts, precision=PREC_FP64, dyadic=DYADIC_ROTATING,
+                 order=4, window=22, device=0, source_split=0, pulse=None):
+        L = load()
+        self._L = L
+        self.ptr = None
+        pos = np.ascontiguousarray(system["pos"], dtype=np.float64)
+        dip = np.ascontiguousarray(system["dip"], dtype=np.float64)
+        n = pos.shape[0]
+        self.n = n
+        freq = np.ascontiguousarray(system.get("freq", np.zeros(n)), dtype=np.float64)
+        t1 = np.ascontiguousarray(system.get("t1", np.ones(n)), dtype=np.float64)
+        t2 = np.ascontiguousarray(system.get("t2", np.ones(n)), dtype=np.float64)
+        p = Params(consts["c0"], consts["hbar"], consts["mu0"], consts["laser_omega"],
+                   consts["dt"], order, window, precision, dyadic, device, source_split)
+        out = C.c_void_p()
+        rc = L.quest_ctx_create(C.byref(p), n, _dp(pos), _dp(dip), _dp(freq), _dp(t1), _dp(t2),
+                                C.byref(out))
+        if rc != QUEST_OK:
+            raise QuestError(f"quest_ctx_create: {L.quest_last_error(None).decode()}")
+        self.ptr = out
+        if pulse is not None:
+            self.set_pulse(pulse)
+
+    def _check(self, rc):
+        if rc == QUEST_OK:
+            return
+        msg = self._L.quest_last_error(self.ptr).decode()
+        if rc == QUEST_ERR_DOMAIN:
+            bad = C.c_int()
+            self._L.quest_bad_step(self.ptr, C.byref(bad))
+            raise UnboundedSolution(msg, bad.value)
+        raise QuestError(msg)
+
+    def close(self):
+        if self.ptr:
+            self._L.quest_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_pulse(self, pl):
+        if pl is None:
+            self._check(self._L.quest_set_pulse(self.ptr, None))
+            return
+        s = Pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                  (C.c_double * 3)(*pl["wavevector"]), (C.c_double * 3)(*pl["polarization"]))
+        self._check(self._L.quest_set_pulse(self.ptr, C.byref(s)))
+
+    def set_pc_weights(self, ps, cs, fc):
+        ps = np.ascontiguousarray(ps, dtype=np.float64)
+        cs = np.ascontiguousarray(cs, dtype=np.float64)
+        self._check(self._L.quest_set_pc_weights(self.ptr, ps.shape[1], _dp(ps), _dp(cs), float(fc)))
+
+    def hist_evaluate(self, step: int, rho01) -> np.ndarray:
+        r = np.ascontiguousarray(rho01, dtype=np.complex128)
+        out = np.zeros(self.n, dtype=np.complex128)
+        self._check(self._L.quest_hist_evaluate(self.ptr, step, _dp(r.view(np.float64)),
+                                                _dp(out.view(np.float64))))
+        return out
+
+    def pulse_evaluate(self, step: int) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.complex128)
+        self._check(self._L.quest_pulse_evaluate(self.ptr, step, _dp(out.view(np.float64))))
+        return out
+
+    def run(self, n_steps: int) -> None:
+        self._check(self._L.quest_run(self.ptr, int(n_steps)))
+
+    def state(self):
+        """(rho [n][2], drho [n][2]) complex at the last finished step."""
+        rho = np.zeros((self.n, 2), dtype=np.complex128)
+        drho = np.zeros((self.n, 2), dtype=np.complex128)
+        self._check(self._L.quest_get_state(self.ptr, _dp(rho.view(np.float64)),
+                                            _dp(drho.view(np.float64))))
+        return rho, drho
+
+    @property
+    def step(self) -> int:
+        s = C.c_int()
+        self._check(self._L.quest_current_step(self.ptr, C.byref(s)))
+        return s.value
+
+    def pair_delays(self) -> np.ndarray:
+        out = np.zeros(max(self.n * (self.n - 1) // 2, 1), dtype=np.int32)
+        self._check(self._L.quest_pair_delays(self.ptr, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out[: self.n * (self.n - 1) // 2]
+
+    def set_timing(self, enable: bool) -> None:
+        self._check(self._L.quest_set_timing(self.ptr, int(bool(enable))))
+
+    def pair_kernel_time(self, reset: bool = False):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self._check(self._L.quest_pair_kernel_time(self.ptr, C.byref(ms), C.byref(cnt), int(reset)))
+        return ms.value, cnt.value
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 8)()
+        self._check(self._L.quest_ctx_info(self.ptr, a))
+        keys = ["n", "ring_slots", "max_delay_bound", "near_pairs", "splits", "tiles",
+                "state_slots", "precision"]
+        return dict(zip(keys, list(a)))
+
+    def sync(self) -> None:
+        self._check(self._L.quest_sync(self.ptr))

@@ -1,0 +1,184 @@
+"""GPU parity: the HIP library (through the C ABI) against the FP64 CPU oracle.
+
+Tolerances (norm-wise relative error max|gpu - oracle| / max|oracle|):
+  FP64 field of one evaluation          1e-12
+  FP64 trajectories                     1e-10
+  mixed field of one evaluation         5e-5   (FP32 pair arithmetic)
+  mixed vs FP64 trajectories, |d rho|   1e-6   absolute
+Integer delays are compared bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import dense_system, oracle_params, oracle_pulse, rel_err, short_pulse
+
+pytestmark = pytest.mark.gpu
+
+
+def _smooth_history(n, steps, seed):
+    """rho01[i][t], t in [0, steps): slowly varying complex amplitudes."""
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(0.05, 0.5, size=n) * np.exp(1j * rng.uniform(0, 2 * np.pi, size=n))
+    nu = rng.uniform(-80, 80, size=n)          # rad/ps in the rotating frame
+    t = np.arange(steps) * 1e-4
+    ramp = 1 - np.exp(-t / 2e-3)
+    return amp[:, None] * ramp[None, :] * np.exp(1j * nu[:, None] * t[None, :])
+
+
+def _oracle_y(rho01, window=22):
+    n, steps = rho01.shape
+    y = np.zeros((n, window + steps, 2), dtype=np.complex128)
+    y[:, :, 0] = 1.0
+    y[:, window:, 1] = rho01
+    return y
+
+
+@pytest.mark.parametrize("precision,dyadic,order,n,side", [
+    (0, 0, 4, 64, 0.25),
+    (0, 0, 3, 64, 0.25),
+    (0, 1, 4, 64, 0.25),
+    (0, 0, 4, 1000, 1.2),
+    (1, 0, 4, 64, 0.25),
+    (1, 0, 4, 1000, 1.2),
+])
+def test_hist_evaluate_matches_oracle(oracle_lib, precision, dyadic, order, n, side):
+    from quest_amd import Context, systems
+    oracle = oracle_lib
+    consts = systems.consts()
+    sysd = dense_system(n, side, seed=11 + n)
+    steps = 70
+    rho01 = _smooth_history(n, steps, seed=5)
+    p = oracle_params(oracle, consts, order=order)
+    delays, coef = oracle.build_table(p, sysd["pos"], sysd["dip"], kind=dyadic)
+    assert (delays == 0).sum() >= 1, "the planted zero-delay pair is missing"
+    y = _oracle_y(rho01)
+    tol = 1e-12 if precision == 0 else 5e-5
+    worst = 0.0
+    with Context(sysd, consts, precision=precision, dyadic=dyadic, order=order) as ctx:
+        for s in range(steps):
+            for it in range(2):   # the integrator calls evaluate repeatedly per step
+                cur = rho01[:, s] * (1.0 + 0.01 * it)
+                yy = y.copy()
+                yy[:, 22 + s, 1] = cur
+                got = ctx.hist_evaluate(s, cur)
+                want = oracle.evaluate_history(p, delays, coef, yy, s, kind=dyadic)
+                if s > 0:
+                    worst = max(worst, rel_err(got, want))
+            # the final value of step s is the unscaled one
+            ctx.hist_evaluate(s, rho01[:, s])
+    print(f"precision={precision} dyadic={dyadic} order={order} n={n}: worst rel err {worst:.3e}")
+    assert worst < tol
+
+
+def test_pulse_matches_oracle(oracle_lib):
+    from quest_amd import Context, systems
+    oracle = oracle_lib
+    consts = systems.consts()
+    sysd = systems.system(128, seed=9)
+    pl = systems.pulse(math.pi / 2)
+    with Context(sysd, consts, pulse=pl) as ctx:
+        for step in (0, 3000, 5000, 5021, 7000):
+            got = ctx.pulse_evaluate(step)
+            want = oracle.pulse_evaluate(oracle_pulse(oracle, pl), consts["hbar"], sysd["pos"],
+                                         sysd["dip"], step * consts["dt"])
+            assert rel_err(got, want) < 1e-13
+            assert np.all(got.imag == 0)
+
+
+def _run_gpu(sysd, consts, pl, weights, steps, precision=0, chunks=1):
+    from quest_amd import Context
+    ps, cs, fc = weights
+    with Context(sysd, consts, precision=precision, pulse=pl) as ctx:
+        ctx.set_pc_weights(ps, cs, fc)
+        per = steps // chunks
+        for _ in range(chunks):
+            ctx.run(per)
+        assert ctx.step == steps
+        return ctx.state()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_run_matches_oracle(oracle_lib, ref_weights, mode):
+    from quest_amd import systems
+    oracle = oracle_lib
+    consts = systems.consts()
+    sysd = dense_system(48, 0.22, seed=21)
+    pl = short_pulse()
+    steps = 160 if mode == 0 else 400
+    ps, cs, fc = ref_weights
+    ref = oracle.solve(oracle_params(oracle, consts), sysd, oracle_pulse(oracle, pl), ps, cs, fc,
+                       steps, mode=mode)
+    assert ref["status"] == 0
+    rho, drho = _run_gpu(sysd, consts, pl, ref_weights, steps, chunks=2)
+    e_rho = rel_err(rho, ref["y"])
+    e_d = rel_err(drho, ref["yp"])
+    print(f"mode {mode}: rho rel err {e_rho:.3e}, drho rel err {e_d:.3e}, "
+          f"max|rho01| {np.abs(ref['y'][:, 1]).max():.3f}")
+    assert np.abs(ref["y"][:, 1]).max() > 0.1, "the pulse did not drive the system"
+    assert e_rho < 1e-10 and e_d < 1e-9
+
+
+def test_run_mixed_close_to_fp64(ref_weights):
+    from quest_amd import systems
+    consts = systems.consts()
+    sysd = dense_system(256, 0.5, seed=31)
+    pl = short_pulse()
+    r64, _ = _run_gpu(sysd, consts, pl, ref_weights, 400, precision=0)
+    r32, _ = _run_gpu(sysd, consts, pl, ref_weights, 400, precision=1)
+    err = float(np.abs(r32 - r64).max())
+    print(f"mixed vs fp64 max|d rho| = {err:.3e}")
+    assert err < 1e-6
+
+
+def test_pair_delays_bit_exact():
+    """FP64 delay split of every pair at N = 4096 vs an IEEE numpy restatement of
+    history_interaction.cpp:29-34 (sqrt((dx^2 + dy^2) + dz^2) / (c0 dt), trunc)."""
+    from quest_amd import Context, systems
+    sysd, _ = systems.workload("n4096")
+    consts = systems.consts()
+    pos = sysd["pos"]
+    rows, cols = np.tril_indices(pos.shape[0], -1)
+    dr = pos[cols] - pos[rows]
+    r2 = (dr[:, 0] * dr[:, 0] + dr[:, 1] * dr[:, 1]) + dr[:, 2] * dr[:, 2]
+    want = np.trunc(np.sqrt(r2) / (consts["c0"] * consts["dt"])).astype(np.int32)
+    with Context(sysd, consts) as ctx:
+        got = ctx.pair_delays()
+    assert got.shape == want.shape
+    mism = int((got != want).sum())
+    assert mism == 0, f"{mism} delays differ"
+
+
+def test_unbounded_solution_reports_first_bad_step(oracle_lib, ref_weights):
+    """A dot with a negative T2 grows without bound; the step the device
+    reports equals the oracle's (integrator.cpp:187-207)."""
+    from quest_amd import UnboundedSolution, systems
+    oracle = oracle_lib
+    consts = systems.consts()
+    sysd = dense_system(16, 0.2, seed=41)
+    sysd["t2"][5] = -2e-5
+    pl = short_pulse()
+    ps, cs, fc = ref_weights
+    ref = oracle.solve(oracle_params(oracle, consts), sysd, oracle_pulse(oracle, pl), ps, cs, fc,
+                       400, mode=1)
+    assert ref["status"] == 1
+    with pytest.raises(UnboundedSolution) as ei:
+        _run_gpu(sysd, consts, pl, ref_weights, 400)
+    assert ei.value.step == ref["bad_step"]
+    assert str(ei.value) == f"unbounded history values at or before step {ref['bad_step']}"
+
+
+def test_per_call_sequence_errors():
+    from quest_amd import Context, QuestError, systems
+    sysd = systems.system(32, seed=3)
+    with Context(sysd, systems.consts()) as ctx:
+        z = np.zeros(32, dtype=np.complex128)
+        ctx.hist_evaluate(0, z)
+        ctx.hist_evaluate(1, z)
+        with pytest.raises(QuestError):
+            ctx.hist_evaluate(0, z)
+        with pytest.raises(QuestError):
+            ctx.hist_evaluate(3, z)
+        with pytest.raises(QuestError):
+            ctx.run(1)

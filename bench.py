@@ -1,0 +1,144 @@
+"""Benchmark: pair-interactions/s of the retarded-interaction step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload n65536]
+
+One step = one full predictor-corrector timestep of every dot (pair sum,
+pulse, predictor, 10 corrector iterations, Bloch RHS), as Integrator::solve
+does it (src/integrator.cpp:120-185).  The workload is N = 65536 dots at
+512 um^-3 with random dipoles, 2 rad/ps Gaussian detuning and a pi/2 pulse
+(dt = 1e-4 ps, order-4 interpolation, 2000-step run); `value` is the
+mixed-precision path (FP32 pair arithmetic, FP64 accumulation and FP64 state)
+and the same run in plain FP64 is reported beside it, with the max |d rho|
+between the two.  Inputs live in HBM before the timed region starts.
+
+The CPU baseline is the FP64 oracle's reference method (pair table plus 11
+scatter sums per step, history_interaction.cpp:47-70 + integrator.cpp:129-140)
+on one host core, over a bounded sample (5 steps of the N = 4096 system; the
+table of N = 65536 would need 180 GB).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+F_ALG = 105            # flop per ordered pair per step (algorithmic count)
+FP32_PEAK = 157.3      # TFLOP/s, MI355X vector FP32 (datasheet)
+FP64_PEAK = 78.6       # TFLOP/s, MI355X vector FP64
+
+
+def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights):
+    from quest_amd import Context
+    ps, cs, fc = weights
+    with Context(sysd, consts, precision=precision, pulse=pulse) as ctx:
+        ctx.set_pc_weights(ps, cs, fc)
+        info = ctx.info()
+        if warmup:
+            ctx.run(warmup)
+        ctx.sync()
+        ctx.set_timing(True)
+        ctx.pair_kernel_time(reset=True)
+        t0 = time.perf_counter()
+        ctx.run(steps)          # returns after the stream has drained
+        ctx.sync()
+        secs = time.perf_counter() - t0
+        kms, launches = ctx.pair_kernel_time()
+        rho, _ = ctx.state()
+    return {"seconds": secs, "pair_ms": kms / max(launches, 1), "launches": launches,
+            "rho": rho, "info": info}
+
+
+def cpu_baseline(steps=5):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import pc_weights
+    from quest_amd import systems
+    oracle.build()
+    oracle.set_threads(1)
+    sysd, _ = systems.workload("n4096")
+    c = systems.consts()
+    pl = systems.pulse(math.pi / 2)
+    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
+    ps, cs, fc = pc_weights.weights()
+    r = oracle.time_reference_steps(
+        p, sysd, oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                              pl["wavevector"], pl["polarization"]), ps, cs, fc, steps)
+    n = sysd["pos"].shape[0]
+    return {"value": n * n * steps / r["step_seconds"], "unit": "pair-interactions/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"FP64 oracle, reference method (pair table + 11 scatter sums/step), "
+                       f"N={n} system of the same generator, {steps} steps in "
+                       f"{r['step_seconds']:.2f} s on 1 core (table build {r['setup_seconds']:.1f} s "
+                       f"excluded); N=65536 needs a 180 GB table")}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="n65536")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        return 2
+
+    from quest_amd import pc_weights, systems
+    sysd, full_steps = systems.workload(args.workload)
+    consts = systems.consts()
+    pulse = systems.pulse(math.pi / 2)
+    weights = pc_weights(18, 22, 3.15)
+    n = sysd["pos"].shape[0]
+
+    mixed = run_gpu(sysd, consts, pulse, 1, args.warmup, args.steps, weights)
+    fp64 = None if args.no_fp64 else run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights)
+
+    pairs = float(n) * n
+    value = pairs * args.steps / mixed["seconds"]
+    ms_per_step = 1e3 * mixed["seconds"] / args.steps
+    ordered = float(n) * (n - 1)
+    achieved = F_ALG * ordered / (mixed["pair_ms"] * 1e-3) / 1e12
+    out = {
+        "metric": "pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N=65536, ρ err≤1e-6",
+        "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair arithmetic + f64 accumulation/state",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: N={n} dots at 512 um^-3, random dipoles, "
+                               f"2 rad/ps detuning, pi/2 pulse, dt=1e-4 ps, order 4, "
+                               f"{full_steps}-step run", "n_dots": n,
+                   "source_splits": mixed["info"]["splits"],
+                   "ring_slots": mixed["info"]["ring_slots"],
+                   "zero_delay_pairs": mixed["info"]["near_pairs"]},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK, "traffic": None,
+                     "kernel": "pair_past_f32<4,false>", "kernel_ms": mixed["pair_ms"],
+                     "flop_per_pair": F_ALG},
+    }
+    if fp64 is not None:
+        v64 = pairs * args.steps / fp64["seconds"]
+        a64 = F_ALG * ordered / (fp64["pair_ms"] * 1e-3) / 1e12
+        out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
+                       "pair_kernel_ms": fp64["pair_ms"], "achieved_tflops": a64,
+                       "frac_of_fp64_peak": a64 / FP64_PEAK}
+        out["mixed_over_fp64"] = value / v64
+        out["max_abs_drho_vs_fp64"] = float(np.abs(mixed["rho"] - fp64["rho"]).max())
+        out["max_abs_rho01"] = float(np.abs(fp64["rho"][:, 1]).max())
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -293,6 +293,18 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
   k.m_o2dt = -2 * p->laser_omega / p->dt;
   k.two_idt2 = 2 / (p->dt * p->dt);
   lagrange_monomials(p->interp_order, k.vinv);
+  // FP32 positions are stored relative to the box centre; each coordinate is
+  // off by at most half an ulp of the largest centred coordinate, so a pair
+  // distance by at most sqrt(3) ulp.  The arithmetic (squares, sum, rsqrt,
+  // two products, the rounded 1/(c0 dt)) adds at most ~3.5e-7 relative.
+  double ctr[3], half = 0;
+  for(int a = 0; a < 3; ++a) {
+    ctr[a] = 0.5 * (lo[a] + hi[a]);
+    half = std::max(half, 0.5 * (hi[a] - lo[a]));
+  }
+  const double ulp = half > 0 ? std::ldexp(1.0, std::ilogb(half) - 23) : 0x1p-126;
+  k.eps_rel = 8e-7;
+  k.eps_abs = 2.0 * std::sqrt(3.0) * ulp / cdt + 2e-6;
   k.dyadic = p->dyadic;
 
   // sorted host copies
@@ -303,8 +315,8 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
     const int i = c->perm[s];
     p64[s] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
     d64[s] = make_double4(dip[3 * i], dip[3 * i + 1], dip[3 * i + 2], 0.0);
-    p32[s] = make_float4(static_cast<float>(p64[s].x), static_cast<float>(p64[s].y),
-                         static_cast<float>(p64[s].z), 0.0f);
+    p32[s] = make_float4(static_cast<float>(p64[s].x - ctr[0]), static_cast<float>(p64[s].y - ctr[1]),
+                         static_cast<float>(p64[s].z - ctr[2]), 0.0f);
     d32[s] = make_float4(static_cast<float>(d64[s].x), static_cast<float>(d64[s].y),
                          static_cast<float>(d64[s].z), 0.0f);
     if(freq) f[s] = freq[i];

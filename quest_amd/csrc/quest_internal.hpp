@@ -37,6 +37,8 @@ struct Consts {
   double hbar, laser_omega;
   // combination constants of the pair term, see pair_term()
   double cc, oo, oc, m_cdt, m_o2dt, two_idt2;
+  // error bound of the FP32 delay estimate: eps = eps_rel tau + eps_abs
+  double eps_rel, eps_abs;
   // monomial coefficients of the Lagrange basis: vinv[k][m] = [x^k] L_m(x)
   double vinv[(kMaxOrder + 1) * (kMaxOrder + 1)];
   int dyadic;         // QUEST_DYADIC_*
@@ -51,7 +53,7 @@ struct PulseParams {
 // Device buffers of one context (sorted dot order).
 struct DeviceState {
   double4 *pos64 = nullptr, *dip64 = nullptr;   // (x, y, z, 0)
-  float4 *pos32 = nullptr, *dip32 = nullptr;
+  float4 *pos32 = nullptr, *dip32 = nullptr;      // pos32 relative to the box centre
   double *freq = nullptr, *t1 = nullptr, *t2 = nullptr;
   void *ring = nullptr;                // R2 [n][ring_slots][kstride]
   double2 *y0 = nullptr, *y1 = nullptr;  // state ring [state_slots][n]: rho00, rho01

@@ -92,6 +92,34 @@ __device__ __forceinline__ int split_f64(double dx, double dy, double dz, double
   return static_cast<int>(__builtin_fmax(fl, 0.0));
 }
 
+// Delay split of the mixed-precision path: the FP32 estimate decides the
+// integer part unless its fraction lies within the estimate's error bound
+// eps = eps_rel tau + eps_abs (position rounding plus six roundings, with a
+// factor two of margin) of an integer; then the exact FP64 split decides.  The
+// integer delays are therefore the reference's in both precisions, and the
+// pair kernel and the zero-delay list builder classify every pair alike.
+__device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float icdt, float eps_rel,
+                                           float eps_abs, const double4 &pa64,
+                                           const double4 *__restrict__ pos64, int b, double cdt,
+                                           double icdt64, float &x, float &ir)
+{
+  float r2;
+  const float tau = tau_f32(dx, dy, dz, icdt, r2, ir);
+  const float fl = __builtin_floorf(tau);
+  const float fr = tau - fl;
+  const float eps = __builtin_fmaf(eps_rel, tau, eps_abs);
+  if(__builtin_expect(fr < eps || fr > 1.0f - eps, 0)) {
+    const double4 pb = pos64[b];
+    double xd, ird, r2d;
+    const int n = split_f64(pa64.x - pb.x, pa64.y - pb.y, pa64.z - pb.z, cdt, icdt64, xd, ird, r2d);
+    x = static_cast<float>(xd);
+    return n;
+  }
+  x = fr;
+  // NaN (coincident dots) -> 0; keeps the conversion defined
+  return static_cast<int>(__builtin_fmaxf(fl, 0.0f));
+}
+
 // sin and cos of u in [0, 0.23): Taylor series (Horner in u^2), truncated
 // where the next term is below the format's rounding error.
 __device__ __forceinline__ void sincos_small(float u, float &s, float &c)
@@ -158,11 +186,12 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
 
 // Pair constants converted to the pair arithmetic's type.
 template <typename R> struct PairK {
-  R icdt, cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta;
-  double cdt;
+  R icdt, cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, eps_abs;
+  double cdt, icdt64;
   __host__ __device__ explicit PairK(const Consts &k)
       : icdt((R)k.icdt), cc((R)k.cc), oo((R)k.oo), oc((R)k.oc), m_cdt((R)k.m_cdt),
-        m_o2dt((R)k.m_o2dt), two_idt2((R)k.two_idt2), theta((R)k.theta), cdt(k.cdt) {}
+        m_o2dt((R)k.m_o2dt), two_idt2((R)k.two_idt2), theta((R)k.theta),
+        eps_rel((R)k.eps_rel), eps_abs((R)k.eps_abs), cdt(k.cdt), icdt64(k.icdt) {}
 };
 
 // The field at a target from one source, given the geometry and the source's
@@ -253,7 +282,8 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
 template <int M, bool LAB>
 __global__ __launch_bounds__(kBlock) void pair_past_f32(
     const float4 *__restrict__ pos, const float4 *__restrict__ dip,
-    const float2 *__restrict__ ring, double2 *__restrict__ partial, Consts k, int step)
+    const double4 *__restrict__ pos64, const float2 *__restrict__ ring,
+    double2 *__restrict__ partial, Consts k, int step)
 {
   const PairK<float> K(k);
   const int split = blockIdx.x % k.splits;
@@ -261,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void pair_past_f32(
   const int tgt = tile * kBlock + threadIdx.x;
   const int a = tgt < k.n ? tgt : k.n - 1;
   const float4 pa = pos[a], da = dip[a];
+  const double4 pa64 = pos64[a];
   const int b0 = split * k.src_per_split;
   const int b1 = min(k.n, b0 + k.src_per_split);
   double accr = 0, acci = 0;
@@ -270,11 +301,9 @@ __global__ __launch_bounds__(kBlock) void pair_past_f32(
     for(int b = bt; b < be; ++b) {
       const float4 pb = pos[b], db = dip[b];
       const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-      float r2, ir;
-      const float tau = tau_f32(dx, dy, dz, K.icdt, r2, ir);
-      const float fl = __builtin_floorf(tau);
-      const float x = tau - fl;
-      const int n = static_cast<int>(__builtin_fmaxf(fl, 0.0f));
+      float x, ir;
+      const int n = split_mixed(dx, dy, dz, K.icdt, K.eps_rel, K.eps_abs, pa64, pos64, b, K.cdt,
+                                K.icdt64, x, ir);
       const bool valid = (b != a) && (n >= 1);
       const int slot = (step - n) & k.ring_mask;
       const float2 *c = ring + (static_cast<size_t>(b) * k.ring_slots + slot) * k.kstride;
@@ -564,12 +593,13 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
     }
   } else {
     const float4 pa = d.pos32[a];
-    const float icdt = static_cast<float>(k.icdt);
+    const double4 pa64 = d.pos64[a];
+    const PairK<float> K(k);
     for(int b = 0; b < k.n; ++b) {
       const float4 pb = d.pos32[b];
-      float r2, ir;
-      const float tau = tau_f32(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, icdt, r2, ir);
-      const int n = static_cast<int>(__builtin_fmaxf(__builtin_floorf(tau), 0.0f));
+      float x, ir;
+      const int n = split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.icdt, K.eps_rel,
+                                K.eps_abs, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
       if(b != a && n < 1) {
         if(FILL) d.near_src[base + cnt] = b;
         ++cnt;
@@ -684,10 +714,10 @@ hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, int ste
                          static_cast<const double2 *>(d.ring), d.partial, k, step);
   } else {
     if(lab)
-      hipLaunchKernelGGL((pair_past_f32<M, true>), grid, block, 0, s, d.pos32, d.dip32,
+      hipLaunchKernelGGL((pair_past_f32<M, true>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
                          static_cast<const float2 *>(d.ring), d.partial, k, step);
     else
-      hipLaunchKernelGGL((pair_past_f32<M, false>), grid, block, 0, s, d.pos32, d.dip32,
+      hipLaunchKernelGGL((pair_past_f32<M, false>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
                          static_cast<const float2 *>(d.ring), d.partial, k, step);
   }
   return hipGetLastError();

@@ -17,14 +17,14 @@ from conftest import dense_system, oracle_params, oracle_pulse, rel_err, short_p
 pytestmark = pytest.mark.gpu
 
 
-def _smooth_history(n, steps, seed):
-    """rho01[i][t], t in [0, steps): slowly varying complex amplitudes."""
-    rng = np.random.default_rng(seed)
-    amp = rng.uniform(0.05, 0.5, size=n) * np.exp(1j * rng.uniform(0, 2 * np.pi, size=n))
-    nu = rng.uniform(-80, 80, size=n)          # rad/ps in the rotating frame
-    t = np.arange(steps) * 1e-4
-    ramp = 1 - np.exp(-t / 2e-3)
-    return amp[:, None] * ramp[None, :] * np.exp(1j * nu[:, None] * t[None, :])
+def _physical_history(oracle, sysd, consts, steps, weights):
+    """rho01[i][t], t in [0, steps): an oracle trajectory under a short pulse
+    centred in the window (smooth in time, as every integrated history is)."""
+    ps, cs, fc = weights
+    pl = short_pulse(delay=steps * 0.6e-4, sigma_t=steps * 0.15e-4)
+    r = oracle.solve(oracle_params(oracle, consts), sysd, oracle_pulse(oracle, pl), ps, cs, fc,
+                     steps, mode=1, record_every=1)
+    return r["rec"][:, :, 1].T.copy()
 
 
 def _oracle_y(rho01, window=22):
@@ -43,13 +43,14 @@ def _oracle_y(rho01, window=22):
     (1, 0, 4, 64, 0.25),
     (1, 0, 4, 1000, 1.2),
 ])
-def test_hist_evaluate_matches_oracle(oracle_lib, precision, dyadic, order, n, side):
+def test_hist_evaluate_matches_oracle(oracle_lib, ref_weights, precision, dyadic, order, n, side):
     from quest_amd import Context, systems
     oracle = oracle_lib
     consts = systems.consts()
     sysd = dense_system(n, side, seed=11 + n)
     steps = 70
-    rho01 = _smooth_history(n, steps, seed=5)
+    rho01 = _physical_history(oracle, sysd, consts, steps, ref_weights)
+    assert np.abs(rho01).max() > 0.05
     p = oracle_params(oracle, consts, order=order)
     delays, coef = oracle.build_table(p, sysd["pos"], sysd["dip"], kind=dyadic)
     assert (delays == 0).sum() >= 1, "the planted zero-delay pair is missing"

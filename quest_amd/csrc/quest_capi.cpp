@@ -268,7 +268,7 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
   Consts &k = c->k;
   k.n = n;
   k.order = p->interp_order;
-  k.kstride = (p->interp_order + 2) & ~1;
+  k.kstride = p->interp_order + 1;  // slot_stride<M>()
   k.ring_slots = next_pow2(c->dmax_bound + 1);
   k.ring_mask = k.ring_slots - 1;
   k.tiles = (n + kBlock - 1) / kBlock;

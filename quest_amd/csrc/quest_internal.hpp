@@ -24,7 +24,7 @@ int pc_weights(int n_lambda, int n_time, double radius, double *ps, double *cs,
 struct Consts {
   int n;              // dots
   int order;          // M
-  int kstride;        // complex entries per ring slot (M + 1 rounded up to even)
+  int kstride;        // complex entries per ring slot (M + 1)
   int ring_slots;     // power of two > max delay
   int ring_mask;
   int splits;         // source splits of the pair kernel

@@ -2,7 +2,7 @@
 //
 // Data layout in HBM (all in the library's Morton-sorted dot order):
 //   pos/dip        (x, y, z, 0) per dot, FP64 and FP32 copies
-//   ring           R2 [n][ring_slots][kstride]: for every source dot and every
+//   ring           R2 [n][ring_slots][M + 1]: for every source dot and every
 //                  finished step s (slot s & ring_mask) the monomial
 //                  coefficients c_k of its Lagrange interpolant of rho01
 //                  through steps s, s-1, ..., s-M, multiplied by e^{i theta s}
@@ -147,6 +147,14 @@ __device__ __forceinline__ void sincos_small(double u, double &s, double &c)
   c = __builtin_fma(pc, u2, 1.0);
 }
 
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Complex entries per ring slot: M + 1, unpadded -- the pair sum is bound by
+// how many cache lines the concurrently walked ring windows span, not by
+// load alignment.
+template <int M> constexpr int slot_stride() { return M + 1; }
+
 // Value q0 = p(x), slope q1 = p'(x) and half curvature q2 = p''(x)/2 of the
 // degree-M polynomial sum_k c_k x^k (simultaneous Horner).
 template <typename R, int M>
@@ -162,25 +170,25 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
   if constexpr(M >= 1) {
     q1r = q0r;
     q1i = q0i;
-    q0r = __builtin_fma(q0r, x, c[M - 1].x);
-    q0i = __builtin_fma(q0i, x, c[M - 1].y);
+    q0r = fmaT(q0r, x, c[M - 1].x);
+    q0i = fmaT(q0i, x, c[M - 1].y);
   }
   if constexpr(M >= 2) {
     q2r = q1r;
     q2i = q1i;
-    q1r = __builtin_fma(q1r, x, q0r);
-    q1i = __builtin_fma(q1i, x, q0i);
-    q0r = __builtin_fma(q0r, x, c[M - 2].x);
-    q0i = __builtin_fma(q0i, x, c[M - 2].y);
+    q1r = fmaT(q1r, x, q0r);
+    q1i = fmaT(q1i, x, q0i);
+    q0r = fmaT(q0r, x, c[M - 2].x);
+    q0i = fmaT(q0i, x, c[M - 2].y);
   }
 #pragma unroll
   for(int k = M - 3; k >= 0; --k) {
-    q2r = __builtin_fma(q2r, x, q1r);
-    q2i = __builtin_fma(q2i, x, q1i);
-    q1r = __builtin_fma(q1r, x, q0r);
-    q1i = __builtin_fma(q1i, x, q0i);
-    q0r = __builtin_fma(q0r, x, c[k].x);
-    q0i = __builtin_fma(q0i, x, c[k].y);
+    q2r = fmaT(q2r, x, q1r);
+    q2i = fmaT(q2i, x, q1i);
+    q1r = fmaT(q1r, x, q0r);
+    q1i = fmaT(q1i, x, q0i);
+    q0r = fmaT(q0r, x, c[k].x);
+    q0i = fmaT(q0i, x, c[k].y);
   }
 }
 
@@ -257,6 +265,7 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
   const double4 pa = pos[a], da = dip[a];
   const int b0 = split * k.src_per_split;
   const int b1 = min(k.n, b0 + k.src_per_split);
+  const size_t src_stride = static_cast<size_t>(k.ring_slots) * slot_stride<M>();
   double accr = 0, acci = 0;
   for(int b = b0; b < b1; ++b) {
     const double4 pb = pos[b], db = dip[b];
@@ -264,8 +273,8 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
     double x, ir, r2;
     const int n = split_f64(dx, dy, dz, K.cdt, K.icdt, x, ir, r2);
     const bool valid = (b != a) && (n >= 1);
-    const int slot = (step - n) & k.ring_mask;
-    const double2 *c = ring + (static_cast<size_t>(b) * k.ring_slots + slot) * k.kstride;
+    const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
+    const double2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
     double2 cv[M + 1];
 #pragma unroll
     for(int i = 0; i <= M; ++i) cv[i] = c[i];
@@ -294,6 +303,7 @@ __global__ __launch_bounds__(kBlock) void pair_past_f32(
   const double4 pa64 = pos64[a];
   const int b0 = split * k.src_per_split;
   const int b1 = min(k.n, b0 + k.src_per_split);
+  const size_t src_stride = static_cast<size_t>(k.ring_slots) * slot_stride<M>();
   double accr = 0, acci = 0;
   for(int bt = b0; bt < b1; bt += kChunk) {
     const int be = min(b1, bt + kChunk);
@@ -305,8 +315,8 @@ __global__ __launch_bounds__(kBlock) void pair_past_f32(
       const int n = split_mixed(dx, dy, dz, K.icdt, K.eps_rel, K.eps_abs, pa64, pos64, b, K.cdt,
                                 K.icdt64, x, ir);
       const bool valid = (b != a) && (n >= 1);
-      const int slot = (step - n) & k.ring_mask;
-      const float2 *c = ring + (static_cast<size_t>(b) * k.ring_slots + slot) * k.kstride;
+      const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
+      const float2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
       float2 cv[M + 1];
 #pragma unroll
       for(int i = 0; i <= M; ++i) cv[i] = c[i];

@@ -56,6 +56,40 @@ def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights):
             "rho": rho, "info": info}
 
 
+def copy_probe(nbytes: int) -> None:
+    """One device-to-device copy of nbytes (the runtime's copy kernel): the
+    calibration point for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, C.c_size_t(nbytes)) or hip.hipDeviceSynchronize():
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) or hip.hipDeviceSynchronize():  # DeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from profiles/pair_kernels_n65536_pmc.md
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "pair_kernels_n65536_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| `" + kernel + "`") and "traffic_bytes" not in line:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[-1])
+            except ValueError:
+                return None
+    return None
+
+
 def cpu_baseline(steps=5):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -88,6 +122,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="n65536")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -100,6 +136,8 @@ def main(argv=None) -> int:
     weights = pc_weights(18, 22, 3.15)
     n = sysd["pos"].shape[0]
 
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
     mixed = run_gpu(sysd, consts, pulse, 1, args.warmup, args.steps, weights)
     fp64 = None if args.no_fp64 else run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights)
 
@@ -121,7 +159,8 @@ def main(argv=None) -> int:
                    "ring_slots": mixed["info"]["ring_slots"],
                    "zero_delay_pairs": mixed["info"]["near_pairs"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": None,
+                     "frac": achieved / FP32_PEAK,
+                     "traffic": profiled_traffic("pair_past_f32<4, false>") if args.workload == "n65536" else None,
                      "kernel": "pair_past_f32<4,false>", "kernel_ms": mixed["pair_ms"],
                      "flop_per_pair": F_ALG},
     }

@@ -37,8 +37,8 @@ def common_flags():
             "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 
 
-def compile_one(src: str, obj: str) -> None:
-    cmd = [hipcc()] + common_flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+def compile_one(src: str, obj: str, extra=()) -> None:
+    cmd = [hipcc()] + common_flags() + list(extra) + ["-c", os.path.join(CSRC, src), "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile of {src} failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
@@ -48,27 +48,57 @@ def compile_one(src: str, obj: str) -> None:
         sys.stderr.write("\n".join(errs[:40]) + "\n")
 
 
-def build(jobs: int = 4) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    objdir = os.path.join(LIBDIR, "obj")
+def build(jobs: int = 4, out: str = LIB, defines=()) -> str:
+    libdir = os.path.dirname(out)
+    os.makedirs(libdir, exist_ok=True)
+    objdir = os.path.join(libdir, "obj")
     os.makedirs(objdir, exist_ok=True)
     objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    extra = [f"-D{d}" for d in defines]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        list(ex.map(lambda so: compile_one(*so), zip(SOURCES, objs)))
-    tmp = LIB + ".tmp"
+        list(ex.map(lambda so: compile_one(*so, extra), zip(SOURCES, objs)))
+    tmp = out + ".tmp"
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
+
+
+BINDIR = os.path.join(HERE, "bin")
+PROGRAMS = {  # output name -> source (host C++ over the C ABI, linked to the library)
+    "quest_hip": os.path.join(HERE, "host", "quest_main.cpp"),
+    "mirror_test": os.path.join(ROOT, "tests", "cpp", "mirror_test.cpp"),
+}
+
+
+def build_programs(lib: str = LIB) -> list:
+    """The C++ driver (`quest_hip`, main.cpp's replacement) and the mirror's test."""
+    os.makedirs(BINDIR, exist_ok=True)
+    out = []
+    libdir = os.path.dirname(lib)
+    for name, src in PROGRAMS.items():
+        exe = os.path.join(BINDIR, name)
+        cmd = [hipcc(), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+               "-o", exe + ".tmp", f"-L{libdir}", "-lquest_amd", "-Wl,-rpath,$ORIGIN/../lib"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"build of {name} failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--out", default=LIB, help="library path (default: in-tree quest_amd/lib)")
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="extra macro definition")
     args = ap.parse_args(argv)
-    print(build(args.jobs))
+    print(build(args.jobs, args.out, args.defines))
+    if args.out == LIB:
+        print("\n".join(build_programs()))
     return 0
 
 

@@ -26,7 +26,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libquest_amd.so")
+LIB_PATH = os.environ.get("QUEST_AMD_LIB") or os.path.join(HERE, "lib", "libquest_amd.so")
 
 QUEST_OK, QUEST_ERR_ARG, QUEST_ERR_HIP, QUEST_ERR_DOMAIN, QUEST_ERR_STATE, QUEST_ERR_NOMEM = range(6)
 PREC_FP64, PREC_MIXED = 0, 1

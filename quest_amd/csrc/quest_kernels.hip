@@ -250,6 +250,9 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
 // iteration from the near list.  Grid: tiles * splits blocks of kBlock
 // threads; block id -> (split = id % splits, tile = id / splits), so with 8
 // splits each XCD (round-robin block dispatch) walks one source range.
+#ifndef QUEST_PAIR_UNROLL
+#define QUEST_PAIR_UNROLL 1
+#endif
 constexpr int kChunk = 64;   // sources summed in the pair type before the FP64 fold
 
 template <int M, bool LAB>
@@ -308,6 +311,7 @@ __global__ __launch_bounds__(kBlock) void pair_past_f32(
   for(int bt = b0; bt < b1; bt += kChunk) {
     const int be = min(b1, bt + kChunk);
     float sr = 0, si = 0;
+#pragma unroll QUEST_PAIR_UNROLL
     for(int b = bt; b < be; ++b) {
       const float4 pb = pos[b], db = dip[b];
       const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;

@@ -1,0 +1,193 @@
+// quest_main.cpp -- the `quest` driver (src/main.cpp:17-63) over the GPU
+// library.  Reads the reference's three input files and writes output.dat
+// in the reference's format.
+//
+//   quest_hip [--config input.cfg] [--precision fp64|mixed] [--host-loop]
+//             [--output output.dat] [--device N] [--help] [--version]
+//
+// input.cfg is the reference's INI file (configuration.cpp:17-42):
+//   [files]      qd_path (dots.cfg), pulse_path (pulse.cfg)
+//   [constants]  c0, hbar, mu0, laser_frequency       (all required)
+//   [parameters] num_particles, total_time, timestep,
+//                interpolation_order                  (all required)
+// num_timesteps = ceil(total_time / timestep) (configuration.cpp:37-40).
+// By default the whole predictor-corrector loop runs on the device;
+// --host-loop keeps the reference's host Integrator and evaluates the
+// interactions on the device at every predictor/corrector evaluation.
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+
+#include "quest.hpp"
+
+namespace {
+
+struct CommandLineException : public std::exception {
+  const char *what() const noexcept override { return "Have a nice day! :)"; }
+};
+
+std::string trim(const std::string &s)
+{
+  const size_t a = s.find_first_not_of(" \t\r\n");
+  if(a == std::string::npos) return "";
+  const size_t b = s.find_last_not_of(" \t\r\n");
+  return s.substr(a, b - a + 1);
+}
+
+// INI reader with boost::program_options' conventions: "[section]" headers,
+// "key = value" lines, '#' starts a comment.
+std::map<std::string, std::string> read_ini(std::istream &is)
+{
+  std::map<std::string, std::string> kv;
+  std::string line, section;
+  while(std::getline(is, line)) {
+    const size_t hash = line.find('#');
+    if(hash != std::string::npos) line = line.substr(0, hash);
+    line = trim(line);
+    if(line.empty()) continue;
+    if(line.front() == '[' && line.back() == ']') {
+      section = trim(line.substr(1, line.size() - 2));
+      continue;
+    }
+    const size_t eq = line.find('=');
+    if(eq == std::string::npos) throw std::runtime_error("malformed configuration line: " + line);
+    kv[section + "." + trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+  }
+  return kv;
+}
+
+double req_double(const std::map<std::string, std::string> &kv, const std::string &key)
+{
+  auto it = kv.find(key);
+  if(it == kv.end()) throw std::runtime_error("the option '--" + key + "' is required but missing");
+  return std::stod(it->second);
+}
+
+int req_int(const std::map<std::string, std::string> &kv, const std::string &key)
+{
+  auto it = kv.find(key);
+  if(it == kv.end()) throw std::runtime_error("the option '--" + key + "' is required but missing");
+  return std::stoi(it->second);
+}
+
+struct Options {
+  std::string config_path = "input.cfg";
+  std::string output = "output.dat";
+  bool host_loop = false;
+};
+
+// configuration.cpp:7-81
+Options parse_configs(int argc, char *argv[])
+{
+  Options o;
+  for(int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    auto next = [&]() -> std::string {
+      if(i + 1 >= argc) throw std::runtime_error("missing value for " + a);
+      return argv[++i];
+    };
+    if(a == "--help") {
+      std::cout << "QuEST options (MI355X build):\n"
+                   "  --help                 print this help message\n"
+                   "  -v [ --version ]       print version string\n"
+                   "  -c [ --config ] arg    path to configuration file (default input.cfg)\n"
+                   "  --precision arg        fp64 (default) or mixed\n"
+                   "  --host-loop            host predictor-corrector, device interactions\n"
+                   "  --output arg           output file (default output.dat)\n"
+                   "  --device arg           HIP device ordinal (default 0)\n";
+      throw CommandLineException();
+    } else if(a == "--version" || a == "-v") {
+      std::cout << "Quantum Electromagnetics Simulation Toolbox (QuEST), MI355X build" << std::endl;
+      throw CommandLineException();
+    } else if(a == "--config" || a == "-c") {
+      o.config_path = next();
+    } else if(a == "--precision") {
+      const std::string v = next();
+      if(v == "fp64") config.precision = QUEST_PREC_FP64;
+      else if(v == "mixed") config.precision = QUEST_PREC_MIXED;
+      else throw std::runtime_error("--precision must be fp64 or mixed");
+    } else if(a == "--host-loop") {
+      o.host_loop = true;
+    } else if(a == "--output") {
+      o.output = next();
+    } else if(a == "--device") {
+      config.device = std::stoi(next());
+    } else {
+      throw std::runtime_error("unrecognised option '" + a + "'");
+    }
+  }
+  std::ifstream ifs(o.config_path);
+  if(!ifs) {
+    std::cerr << "ERROR: " << o.config_path << " not found" << std::endl;
+    throw CommandLineException();
+  }
+  const auto kv = read_ini(ifs);
+  if(kv.count("files.qd_path")) config.qd_path = kv.at("files.qd_path");
+  if(kv.count("files.pulse_path")) config.pulse_path = kv.at("files.pulse_path");
+  config.c0 = req_double(kv, "constants.c0");
+  config.hbar = req_double(kv, "constants.hbar");
+  config.mu0 = req_double(kv, "constants.mu0");
+  config.omega = req_double(kv, "constants.laser_frequency");
+  config.num_particles = req_int(kv, "parameters.num_particles");
+  config.dt = req_double(kv, "parameters.timestep");
+  config.total_time = req_double(kv, "parameters.total_time");
+  config.num_timesteps = static_cast<int>(std::ceil(config.total_time / config.dt));
+  config.interpolation_order = req_int(kv, "parameters.interpolation_order");
+  return o;
+}
+
+}  // namespace
+
+// main.cpp:17-63
+int main(int argc, char *argv[])
+{
+  try {
+    std::cout << std::setprecision(12) << std::scientific;
+    const Options opt = parse_configs(argc, argv);
+
+    std::cout << "Initializing..." << std::endl;
+    auto qds = std::make_shared<DotVector>(import_dots(config.qd_path));
+    qds->resize(config.num_particles);
+    auto rhs_funs = rhs_functions(*qds, config.omega);
+
+    auto history = History::make_shared_history(config.num_particles, 22, config.num_timesteps);
+    for(int t = -22; t <= 0; ++t)
+      for(int sol_idx = 0; sol_idx < config.num_particles; ++sol_idx)
+        (*history)[sol_idx][t][0] = {cmplx(1, 0), cmplx(0, 0)};  // ground state
+
+    auto pulse1 = std::make_shared<Pulse>(read_pulse_config(config.pulse_path));
+    auto rotating_dyadic =
+        std::make_shared<GreenFunction::RotatingDyadic>(config.mu0, config.c0, config.hbar, config.omega);
+
+    std::cout << "Solving..." << std::endl;
+    if(opt.host_loop) {
+      std::vector<std::shared_ptr<Interaction>> interactions{
+          std::make_shared<PulseInteraction>(qds, pulse1),
+          std::make_shared<HistoryInteraction>(qds, history, rotating_dyadic, config.interpolation_order)};
+      PredictorCorrector::Integrator integrator(config.dt, 18, 22, 3.15, history, rhs_funs,
+                                                std::move(interactions));
+      integrator.solve();
+    } else {
+      PredictorCorrector::DeviceIntegrator integrator(qds, pulse1, rotating_dyadic,
+                                                      config.interpolation_order, history);
+      integrator.set_weights(PredictorCorrector::Weights(18, 22, 3.15));
+      integrator.solve(1);
+    }
+
+    std::cout << "Writing output..." << std::endl;
+    History::write_history(history, opt.output);
+  } catch(CommandLineException &e) {
+    // --help / --version / missing configuration file: nothing more to do
+  } catch(std::domain_error &e) {
+    std::cerr << "domain_error: " << e.what() << std::endl;
+    return 2;
+  } catch(std::exception &e) {
+    std::cerr << "error: " << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,104 @@
+"""The C++ driver quest_hip (main.cpp's replacement over the GPU library) on
+the reference's own example inputs and on a denser synthetic system, in both
+the device loop and the host loop (reference integrator + GPU interactions),
+against the FP64 oracle.  Tolerance: 1e-10 norm-wise relative in rho."""
+import math
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import dense_system, oracle_params, oracle_pulse, rel_err
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "quest_amd", "bin", "quest_hip")
+EXAMPLES = os.path.join(ROOT, "tests", "golden", "examples")
+_num = re.compile(r"\(([^,]+),([^)]+)\)")
+
+
+def read_output(path, n):
+    rows = []
+    for line in open(path):
+        vals = [complex(float(a), float(b)) for a, b in _num.findall(line)]
+        assert len(vals) == 2 * n
+        rows.append(np.array(vals).reshape(n, 2))
+    return np.array(rows)
+
+
+def read_dots(path, n):
+    a = np.loadtxt(path)[:n]
+    return {"pos": a[:, 0:3], "freq": a[:, 3], "t1": a[:, 4], "t2": a[:, 5], "dip": a[:, 6:9]}
+
+
+def run_cli(workdir, extra):
+    r = subprocess.run([EXE] + extra, cwd=workdir, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r
+
+
+def oracle_traj(oracle, sysd, consts, pl, steps, order, mode):
+    """The oracle with the weights the driver uses (PredictorCorrector::Weights
+    of the library, integrator.cpp:85-102)."""
+    from quest_amd import pc_weights
+    ps, cs, fc = pc_weights(18, 22, 3.15)
+    r = oracle.solve(oracle_params(oracle, consts, order=order), sysd, oracle_pulse(oracle, pl),
+                     ps, cs, fc, steps, mode=mode, record_every=1)
+    assert r["status"] == 0
+    return r["rec"]
+
+
+@pytest.mark.parametrize("loop", ["device", "host"])
+def test_reference_example_inputs(oracle_lib, tmp_path, loop):
+    """examples/{input,dots,pulse}.cfg as shipped: N = 2, dt = 5e-3 ps, 4000
+    steps, order 3, a pi pulse (every pair is a zero-delay pair here)."""
+    for f in ("input.cfg", "dots.cfg", "pulse.cfg"):
+        shutil.copy(os.path.join(EXAMPLES, f), tmp_path / f)
+    run_cli(tmp_path, ["--host-loop"] if loop == "host" else [])
+    got = read_output(tmp_path / "output.dat", 2)
+    consts = {"c0": 299.792458, "hbar": 0.65821193, "mu0": 2.0133545e-04,
+              "laser_omega": 2278.9013, "dt": 0.5e-2}
+    a = np.loadtxt(os.path.join(EXAMPLES, "pulse.cfg"))
+    pl = {"amplitude": a[0], "delay": a[1], "width": a[2], "freq": a[3],
+          "wavevector": list(a[4:7]), "polarization": list(a[7:10])}
+    want = oracle_traj(oracle_lib, read_dots(os.path.join(EXAMPLES, "dots.cfg"), 2),
+                       consts, pl, 4000, 3, mode=0)
+    assert got.shape == want.shape
+    err = rel_err(got, want)
+    print(f"{loop} loop: rel err {err:.3e}, final rho00 {want[-1, :, 0].real}")
+    assert abs(want[-1, 0, 0].real - 0.5) < 0.5   # the pi pulse moved the state
+    assert err < 1e-10
+
+
+@pytest.mark.parametrize("loop,precision", [("device", "fp64"), ("host", "fp64"), ("device", "mixed")])
+def test_dense_system(oracle_lib, tmp_path, loop, precision):
+    n, steps = 40, 300
+    sysd = dense_system(n, 0.2, seed=77)
+    with open(tmp_path / "dots.cfg", "w") as f:
+        for i in range(n):
+            f.write(" ".join(repr(float(v)) for v in (*sysd["pos"][i], sysd["freq"][i], sysd["t1"][i],
+                                                       sysd["t2"][i], *sysd["dip"][i])) + "\n")
+    sigma = 0.005
+    omega = 2278.9
+    amp = (math.pi / 2) * 0.65821193 / (5.2917721e-4 * sigma * math.sqrt(2 * math.pi))
+    pl = {"amplitude": amp, "delay": 0.015, "width": omega * sigma, "freq": omega,
+          "wavevector": [omega / 299.792458, 0.0, 0.0], "polarization": [0.0, 0.0, 1.0]}
+    with open(tmp_path / "pulse.cfg", "w") as f:
+        f.write(" ".join(repr(float(v)) for v in (pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                                                   *pl["wavevector"], *pl["polarization"])) + "\n")
+    with open(tmp_path / "input.cfg", "w") as f:
+        f.write(f"[parameters]\n num_particles = {n}\n total_time = {steps * 1e-4}\n timestep = 1e-4\n"
+                f" interpolation_order = 4\n[constants]\n c0 = 299.792458\n hbar = 0.65821193\n"
+                f" mu0 = 2.0133545e-04\n laser_frequency = {omega}\n")
+    args = ["--precision", precision] + (["--host-loop"] if loop == "host" else [])
+    run_cli(tmp_path, args)
+    got = read_output(tmp_path / "output.dat", n)
+    consts = {"c0": 299.792458, "hbar": 0.65821193, "mu0": 2.0133545e-04, "laser_omega": omega, "dt": 1e-4}
+    want = oracle_traj(oracle_lib, sysd, consts, pl, steps, 4, mode=1)
+    assert got.shape[0] == steps
+    err = rel_err(got, want)
+    print(f"{loop} loop {precision}: rel err {err:.3e}, max|rho01| {np.abs(want[:, :, 1]).max():.3f}")
+    assert err < (1e-10 if precision == "fp64" else 1e-6)

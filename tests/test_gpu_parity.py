@@ -183,3 +183,29 @@ def test_per_call_sequence_errors():
             ctx.hist_evaluate(3, z)
         with pytest.raises(QuestError):
             ctx.run(1)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-6)])
+def test_n4096_full_run_matches_oracle_fixture(precision, tol):
+    """The N = 4096 system (512 um^-3, pi/2 pulse), all 5000 steps, against the
+    oracle trajectory in tests/golden/n4096_traj.npz (made by
+    tests/golden/make_golden.py with the same weights), every 500 steps."""
+    import os
+    from quest_amd import Context, systems
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "n4096_traj.npz"))
+    sysd, steps = systems.workload("n4096")
+    assert steps == int(g["steps"])
+    every = int(g["record_every"])
+    worst = 0.0
+    with Context(sysd, systems.consts(), precision=precision, pulse=systems.pulse(math.pi / 2)) as ctx:
+        ctx.set_pc_weights(g["ps"], g["cs"], float(g["fc"]))
+        for r in range(steps // every):
+            ctx.run(every)
+            rho, drho = ctx.state()
+            e = (np.abs(rho - g["rec"][r]).max() / np.abs(g["rec"][r]).max() if precision == 0
+                 else np.abs(rho - g["rec"][r]).max())
+            worst = max(worst, float(e))
+        assert np.abs(rho - g["y"]).max() == np.abs(rho - g["rec"][-1]).max()
+    print(f"N=4096 x {steps} steps, precision {precision}: worst error {worst:.3e} "
+          f"(max|rho01| {np.abs(g['y'][:, 1]).max():.3f})")
+    assert worst < tol

@@ -155,7 +155,9 @@ def main(argv=None) -> int:
         "config": {"workload": f"{args.workload}: N={n} dots at 512 um^-3, random dipoles, "
                                f"2 rad/ps detuning, pi/2 pulse, dt=1e-4 ps, order 4, "
                                f"{full_steps}-step run", "n_dots": n,
-                   "source_splits": mixed["info"]["splits"],
+                   "source_chunk": mixed["info"]["source_chunk"],
+                   "source_chunks_per_tile": mixed["info"]["splits"],
+                   "persistent_grid": mixed["info"]["grid"],
                    "ring_slots": mixed["info"]["ring_slots"],
                    "zero_delay_pairs": mixed["info"]["near_pairs"]},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",

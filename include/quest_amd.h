@@ -64,7 +64,9 @@ typedef struct {
   int precision;       /* QUEST_PREC_*                                      */
   int dyadic;          /* QUEST_DYADIC_*                                    */
   int device;          /* HIP device ordinal                                */
-  int source_split;    /* pair-kernel source splits; 0 = autotuned default  */
+  int source_chunk;    /* sources per pair-kernel work item; 0 = default
+                          (8 chunks of the largest system, halved until the
+                          work list fills the GPU); see quest_autotune      */
 } quest_params;
 
 /* Pulse (pulse.h:10-24): amplitude, delay, width (dimensionless), freq,
@@ -88,6 +90,18 @@ int quest_ctx_create(const quest_params *params, int n_dots, const double *pos,
                      const double *dip, const double *freq, const double *t1,
                      const double *t2, quest_ctx **out);
 
+/* Many independent systems in one context (no coupling between systems):
+ * system s has n_dots[s] dots; pos, dip, freq, t1, t2 are the systems'
+ * arrays concatenated in system order, and every per-dot array in and out of
+ * the other entry points uses the same order.  Each pair-sum launch covers
+ * all systems (one persistent kernel over a work list of (system, target
+ * tile, source chunk) items) and quest_run replays one captured graph per
+ * step for all of them.  No counterpart in the reference, which runs one
+ * system per process (main.cpp:17-63); for callers with many small systems. */
+int quest_batch_create(const quest_params *params, int n_systems, const int *n_dots,
+                       const double *pos, const double *dip, const double *freq,
+                       const double *t1, const double *t2, quest_ctx **out);
+
 void quest_ctx_destroy(quest_ctx *ctx);
 
 /* Message of the last failed call on ctx (or of the last failed
@@ -109,8 +123,12 @@ int quest_hist_evaluate(quest_ctx *ctx, int step, const double *rho01,
 
 /* --------------------------------------------- PulseInteraction::evaluate */
 
-/* Sets (or, with NULL, clears) the pulse: Pulse (pulse.h:10-24). */
+/* Sets (or, with NULL, clears) the pulse of every system: Pulse
+ * (pulse.h:10-24). */
 int quest_set_pulse(quest_ctx *ctx, const quest_pulse *pulse);
+
+/* One pulse per system of a batch context (pulses: [n_systems]). */
+int quest_set_pulses(quest_ctx *ctx, const quest_pulse *pulses);
 
 /* Replaces PulseInteraction::evaluate(step) (pulse_interaction.cpp:9-20):
  * out[n] (complex, imaginary part 0). */
@@ -167,8 +185,16 @@ int quest_set_timing(quest_ctx *ctx, int enable);
 int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
                            int reset);
 
-/* Integer facts about the context: n_dots, ring slots, max delay, number of
- * zero-delay (near) pairs, source splits.  info: [8] int64. */
+/* Times the pair kernel (`launches` launches each, after a warm-up) for
+ * source-chunk lengths around the default and keeps the fastest; returns it
+ * in *chunk_len.  Changes no simulation state.  No counterpart in the
+ * reference, which has no tunable kernel. */
+int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);
+
+/* Integer facts about the context.  info: [12] int64 = n_dots, ring slots,
+ * delay bound, zero-delay pairs, work items, persistent grid, state slots,
+ * precision, systems, source-chunk length, chunks of the largest system,
+ * step graph captured (0/1). */
 int quest_ctx_info(const quest_ctx *ctx, int64_t *info);
 
 /* Blocks until all work queued on the context's stream is done. */

@@ -38,7 +38,7 @@ EXPORTS = [
     "quest_set_pulse", "quest_pulse_evaluate", "quest_pc_weights", "quest_set_pc_weights",
     "quest_run", "quest_current_step", "quest_bad_step", "quest_get_state",
     "quest_pair_delays", "quest_set_timing", "quest_pair_kernel_time", "quest_ctx_info",
-    "quest_sync",
+    "quest_sync", "quest_batch_create", "quest_set_pulses", "quest_autotune",
 ]
 
 
@@ -58,7 +58,7 @@ class Params(C.Structure):
     _fields_ = [("c0", C.c_double), ("hbar", C.c_double), ("mu0", C.c_double),
                 ("laser_omega", C.c_double), ("dt", C.c_double), ("interp_order", C.c_int),
                 ("history_window", C.c_int), ("precision", C.c_int), ("dyadic", C.c_int),
-                ("device", C.c_int), ("source_split", C.c_int)]
+                ("device", C.c_int), ("source_chunk", C.c_int)]
 
 
 class Pulse(C.Structure):
@@ -99,6 +99,10 @@ def load():
     L.quest_pair_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int]
     L.quest_ctx_info.argtypes = [vp, C.POINTER(C.c_int64)]
     L.quest_sync.argtypes = [vp]
+    L.quest_batch_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(C.c_int), dp, dp, dp, dp,
+                                     dp, C.POINTER(vp)]
+    L.quest_set_pulses.argtypes = [vp, C.POINTER(Pulse)]
+    L.quest_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     for name in EXPORTS:
         if name not in ("quest_ctx_destroy", "quest_last_error"):
             getattr(L, name).restype = C.c_int
@@ -122,31 +126,55 @@ def pc_weights(n_lambda: int = 18, n_time: int = 22, radius: float = 3.15):
     return ps, cs, fc.value
 
 
+def _pulse_struct(pl):
+    return Pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                 (C.c_double * 3)(*pl["wavevector"]), (C.c_double * 3)(*pl["polarization"]))
+
+
 class Context:
-    """One dot system on one GPU (a quest_ctx)."""
+    """One dot system -- or, given a list of systems, a batch of independent
+    systems (quest_batch_create) -- on one GPU (a quest_ctx).  Per-dot arrays
+    of a batch are the systems' arrays concatenated in order."""
 
     def __init__(self, system, consts, precision=PREC_FP64, dyadic=DYADIC_ROTATING,
-                 order=4, window=22, device=0, source_split=0, pulse=None):
+                 order=4, window=22, device=0, source_chunk=0, pulse=None):
         L = load()
         self._L = L
         self.ptr = None
-        pos = np.ascontiguousarray(system["pos"], dtype=np.float64)
-        dip = np.ascontiguousarray(system["dip"], dtype=np.float64)
-        n = pos.shape[0]
-        self.n = n
-        freq = np.ascontiguousarray(system.get("freq", np.zeros(n)), dtype=np.float64)
-        t1 = np.ascontiguousarray(system.get("t1", np.ones(n)), dtype=np.float64)
-        t2 = np.ascontiguousarray(system.get("t2", np.ones(n)), dtype=np.float64)
+        batch = isinstance(system, (list, tuple))
+        systems = system if batch else [system]
+        sizes = [np.asarray(s["pos"]).shape[0] for s in systems]
+
+        def cat(key, default, width):
+            return np.ascontiguousarray(np.concatenate(
+                [np.asarray(s[key] if key in s else default(n), dtype=np.float64).reshape(n * width)
+                 for s, n in zip(systems, sizes)]))
+
+        pos = cat("pos", lambda n: np.zeros((n, 3)), 3)
+        dip = cat("dip", lambda n: np.zeros((n, 3)), 3)
+        freq = cat("freq", np.zeros, 1)
+        t1 = cat("t1", np.ones, 1)
+        t2 = cat("t2", np.ones, 1)
+        self.n = int(sum(sizes))
+        self.sizes = sizes
         p = Params(consts["c0"], consts["hbar"], consts["mu0"], consts["laser_omega"],
-                   consts["dt"], order, window, precision, dyadic, device, source_split)
+                   consts["dt"], order, window, precision, dyadic, device, source_chunk)
         out = C.c_void_p()
-        rc = L.quest_ctx_create(C.byref(p), n, _dp(pos), _dp(dip), _dp(freq), _dp(t1), _dp(t2),
-                                C.byref(out))
+        if batch:
+            arr = (C.c_int * len(sizes))(*sizes)
+            rc = L.quest_batch_create(C.byref(p), len(sizes), arr, _dp(pos), _dp(dip), _dp(freq),
+                                      _dp(t1), _dp(t2), C.byref(out))
+        else:
+            rc = L.quest_ctx_create(C.byref(p), self.n, _dp(pos), _dp(dip), _dp(freq), _dp(t1),
+                                    _dp(t2), C.byref(out))
         if rc != QUEST_OK:
             raise QuestError(f"quest_ctx_create: {L.quest_last_error(None).decode()}")
         self.ptr = out
         if pulse is not None:
-            self.set_pulse(pulse)
+            if isinstance(pulse, (list, tuple)):
+                self.set_pulses(pulse)
+            else:
+                self.set_pulse(pulse)
 
     def _check(self, rc):
         if rc == QUEST_OK:
@@ -179,9 +207,19 @@ class Context:
         if pl is None:
             self._check(self._L.quest_set_pulse(self.ptr, None))
             return
-        s = Pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
-                  (C.c_double * 3)(*pl["wavevector"]), (C.c_double * 3)(*pl["polarization"]))
+        s = _pulse_struct(pl)
         self._check(self._L.quest_set_pulse(self.ptr, C.byref(s)))
+
+    def set_pulses(self, pulses):
+        """One pulse per system of a batch."""
+        arr = (Pulse * len(pulses))(*[_pulse_struct(pl) for pl in pulses])
+        self._check(self._L.quest_set_pulses(self.ptr, arr))
+
+    def autotune(self, launches: int = 3) -> int:
+        """quest_autotune: the fastest source-chunk length (kept)."""
+        out = C.c_int()
+        self._check(self._L.quest_autotune(self.ptr, int(launches), C.byref(out)))
+        return out.value
 
     def set_pc_weights(self, ps, cs, fc):
         ps = np.ascontiguousarray(ps, dtype=np.float64)
@@ -232,10 +270,10 @@ class Context:
         return ms.value, cnt.value
 
     def info(self) -> dict:
-        a = (C.c_int64 * 8)()
+        a = (C.c_int64 * 12)()
         self._check(self._L.quest_ctx_info(self.ptr, a))
-        keys = ["n", "ring_slots", "max_delay_bound", "near_pairs", "splits", "tiles",
-                "state_slots", "precision"]
+        keys = ["n", "ring_slots", "max_delay_bound", "near_pairs", "work_items", "grid",
+                "state_slots", "precision", "systems", "source_chunk", "splits", "graph"]
         return dict(zip(keys, list(a)))
 
     def sync(self) -> None:

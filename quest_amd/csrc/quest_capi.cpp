@@ -1,17 +1,19 @@
 // quest_capi.cpp -- the C ABI (include/quest_amd.h) over the gfx950 kernels.
 //
-// A context owns the device copy of one dot system in Morton order, the
-// interpolant ring of the pair sum, the predictor-corrector state ring and one
-// HIP stream.  Two ways to drive it, one per context:
+// A context owns the device copy of one or more independent dot systems
+// (each in Morton order, concatenated), the interpolant ring of the pair sum,
+// the predictor-corrector state ring and one HIP stream.  Two ways to drive
+// it, one per context:
 //   per-call  quest_hist_evaluate   -- HistoryInteraction::evaluate replacement
 //   whole-run quest_run             -- Integrator::solve replacement, all steps
-//                                      on the device
+//                                      on the device, one captured graph per step
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -24,8 +26,10 @@ using namespace quest;
 struct quest_ctx {
   quest_params P{};
   int n = 0;
-  std::vector<int> perm;  // perm[sorted] = caller index
-  std::vector<int> inv;   // inv[caller] = sorted index
+  int n_sys = 0;
+  std::vector<int> sys_begin;  // [n_sys + 1] dot offsets of the systems
+  std::vector<int> perm;       // perm[sorted] = caller index
+  std::vector<int> inv;        // inv[caller] = sorted index
   Consts k{};
   DeviceState d{};
   hipStream_t stream = nullptr;
@@ -34,20 +38,25 @@ struct quest_ctx {
   int near_total = 0;
   int phase_len = 0;
   int mode = 0;           // 0 unset, 1 per-call, 2 whole-run
-  bool started = false;
   int cur_step = 0;       // whole-run: next step; per-call: step of the current values (-1 = none)
   int bad = -1;
   bool pc_uploaded = false;
   std::vector<double> pcw;  // ps[2W], cs[2W]
   double fc = 0;
-  PulseParams pulse{};
+  std::vector<PulseParams> pulses;
   std::string err;
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double pair_ms = 0;
   int64_t pair_launches = 0;
-  double2 *h_stage = nullptr;  // pinned [n]
+  double2 *h_stage = nullptr;  // pinned [2n]
+  int chunk_len = 0;           // sources per pair-kernel work item
+  int partial_rows = 0;        // rows allocated in d.partial
+  int cu_count = 0;
+  int blocks_per_cu = 1;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
 };
 
 namespace {
@@ -75,12 +84,12 @@ int next_pow2(int v)
   return p;
 }
 
-// Morton (Z-order) permutation of the dots: 21 bits per axis over the
-// bounding box, stable in the caller's order for equal keys.
-std::vector<int> morton_order(int n, const double *pos)
+// Morton (Z-order) permutation of dots [begin, end): 21 bits per axis over the
+// system's bounding box, stable for equal keys.  Returns caller indices.
+std::vector<int> morton_order(int begin, int end, const double *pos)
 {
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for(int i = 0; i < n; ++i)
+  for(int i = begin; i < end; ++i)
     for(int a = 0; a < 3; ++a) {
       lo[a] = std::min(lo[a], pos[3 * i + a]);
       hi[a] = std::max(hi[a], pos[3 * i + a]);
@@ -94,12 +103,13 @@ std::vector<int> morton_order(int n, const double *pos)
     v = (v | v << 2) & 0x1249249249249249ULL;
     return v;
   };
+  const int n = end - begin;
   std::vector<uint64_t> key(n);
   for(int i = 0; i < n; ++i) {
     uint64_t q[3];
     for(int a = 0; a < 3; ++a) {
       const double span = hi[a] - lo[a];
-      const double t = span > 0 ? (pos[3 * i + a] - lo[a]) / span : 0.0;
+      const double t = span > 0 ? (pos[3 * (begin + i) + a] - lo[a]) / span : 0.0;
       q[a] = static_cast<uint64_t>(std::min(std::max(t, 0.0), 1.0) * 2097151.0);
     }
     key[i] = spread(q[0]) | spread(q[1]) << 1 | spread(q[2]) << 2;
@@ -107,6 +117,7 @@ std::vector<int> morton_order(int n, const double *pos)
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for(int &o : order) o += begin;
   return order;
 }
 
@@ -132,12 +143,89 @@ void lagrange_monomials(int M, double *vinv)
   }
 }
 
-int default_splits(int n, int tiles)
+void drop_graph(quest_ctx *c)
 {
-  int s = (2048 + tiles - 1) / tiles;
-  s = std::min(s, std::max(1, n / 64));
-  if(s >= 8) s = (s + 7) / 8 * 8;
-  return std::max(1, s);
+  if(c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if(c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
+int count_work(const quest_ctx *c, int chunk_len)
+{
+  int64_t items = 0;
+  for(int s = 0; s < c->n_sys; ++s) {
+    const int ns = c->sys_begin[s + 1] - c->sys_begin[s];
+    if(ns < 2) continue;
+    items += static_cast<int64_t>((ns + kBlock - 1) / kBlock) * ((ns + chunk_len - 1) / chunk_len);
+  }
+  return static_cast<int>(std::min<int64_t>(items, INT_MAX));
+}
+
+// Work list of the pair kernel for a given source-chunk length: for every
+// system, every target tile of kBlock dots, every chunk of chunk_len sources
+// (chunks of a tile consecutive, see pair_past_*).
+int build_work(quest_ctx *c, int chunk_len)
+{
+  std::vector<int4> work;
+  std::vector<int> row;
+  std::vector<int> chunks(c->n_sys, 0);
+  int max_chunks = 1;
+  for(int s = 0; s < c->n_sys; ++s) {
+    const int b = c->sys_begin[s], e = c->sys_begin[s + 1], ns = e - b;
+    if(ns < 2) continue;
+    const int tiles = (ns + kBlock - 1) / kBlock, nch = (ns + chunk_len - 1) / chunk_len;
+    chunks[s] = nch;
+    max_chunks = std::max(max_chunks, nch);
+    for(int t = 0; t < tiles; ++t)
+      for(int q = 0; q < nch; ++q) {
+        work.push_back(make_int4(b + t * kBlock, std::min(e, b + (t + 1) * kBlock), b + q * chunk_len,
+                                 std::min(e, b + (q + 1) * chunk_len)));
+        row.push_back(q);
+      }
+  }
+  DeviceState &d = c->d;
+  drop_graph(c);  // the graph's pair kernel holds the old list
+  if(d.work) HIPC(c, hipFree(d.work));
+  if(d.work_row) HIPC(c, hipFree(d.work_row));
+  d.work = nullptr;
+  d.work_row = nullptr;
+  const size_t nw = std::max<size_t>(work.size(), 1);
+  HIPC(c, hipMalloc(&d.work, sizeof(int4) * nw));
+  HIPC(c, hipMalloc(&d.work_row, sizeof(int) * nw));
+  if(!work.empty()) {
+    HIPC(c, hipMemcpyAsync(d.work, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(d.work_row, row.data(), sizeof(int) * row.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  HIPC(c, hipMemcpyAsync(d.sys_chunks, chunks.data(), sizeof(int) * c->n_sys, hipMemcpyHostToDevice, c->stream));
+  if(max_chunks > c->partial_rows) {
+    if(d.partial) HIPC(c, hipFree(d.partial));
+    d.partial = nullptr;
+    HIPC(c, hipMalloc(&d.partial, sizeof(double2) * static_cast<size_t>(c->n) * max_chunks));
+    c->partial_rows = max_chunks;
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->chunk_len = chunk_len;
+  c->k.n_work = static_cast<int>(work.size());
+  c->k.max_chunks = max_chunks;
+  // persistent grid: one full wave of resident blocks (a multiple of 8, see
+  // pair_past_*), at most the item count rounded up to 8
+  const int resident = std::max(8, c->cu_count * c->blocks_per_cu / 8 * 8);
+  const int items8 = (c->k.n_work + 7) / 8 * 8;
+  c->k.grid = std::max(8, std::min(resident, items8));
+  return QUEST_OK;
+}
+
+// Default chunk length: 8 chunks of the largest system (one source range per
+// XCD), halved while the work list has fewer items than resident blocks.
+int default_chunk(const quest_ctx *c)
+{
+  int nmax = 2;
+  for(int s = 0; s < c->n_sys; ++s) nmax = std::max(nmax, c->sys_begin[s + 1] - c->sys_begin[s]);
+  int len = std::max(64, (nmax + 7) / 8);
+  const int resident = c->cu_count * c->blocks_per_cu;
+  while(len > 64 && count_work(c, len) < resident) len = std::max(64, (len + 1) / 2);
+  return len;
 }
 
 int ensure_phase(quest_ctx *c, int upto)
@@ -151,6 +239,7 @@ int ensure_phase(quest_ctx *c, int upto)
     const long double a = fmodl(theta * s, two_pi);
     h[s] = make_double2(static_cast<double>(cosl(a)), static_cast<double>(sinl(a)));
   }
+  drop_graph(c);  // the graph's kernels hold the old table pointer
   if(c->d.phase) HIPC(c, hipFree(c->d.phase));
   c->d.phase = nullptr;
   HIPC(c, hipMalloc(&c->d.phase, sizeof(double2) * len));
@@ -164,7 +253,6 @@ int start_state(quest_ctx *c)
 {
   HIPC(c, launch_fill_ground_state(c->k, c->d, c->stream));
   HIPC(c, launch_clear_ring(c->k, c->d, c->ring_bytes, c->stream));
-  c->started = true;
   c->bad = -1;
   return QUEST_OK;
 }
@@ -182,6 +270,15 @@ int ensure_pc(quest_ctx *c)
                          c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   c->pc_uploaded = true;
+  drop_graph(c);  // future_coef is a kernel argument of the graph
+  return QUEST_OK;
+}
+
+int upload_pulses(quest_ctx *c)
+{
+  HIPC(c, hipMemcpyAsync(c->d.pulses, c->pulses.data(), sizeof(PulseParams) * c->n_sys,
+                         hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
   return QUEST_OK;
 }
 
@@ -207,9 +304,10 @@ int collect_timing(quest_ctx *c)
   return QUEST_OK;
 }
 
-int pair_sum(quest_ctx *c, int step)
+// The pair kernel, bracketed by HIP events on the context's stream when timing.
+int pair_sum(quest_ctx *c)
 {
-  if(c->n < 2) return QUEST_OK;
+  if(c->k.n_work == 0) return QUEST_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if(c->timing) {
     e0 = pool_event(c);
@@ -217,18 +315,55 @@ int pair_sum(quest_ctx *c, int step)
     if(!e0 || !e1) return fail(c, QUEST_ERR_HIP, "hipEventCreate failed");
     HIPC(c, hipEventRecord(e0, c->stream));
   }
-  HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, step, c->stream));
+  HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));
   if(c->timing) HIPC(c, hipEventRecord(e1, c->stream));
+  return QUEST_OK;
+}
+
+// One whole step: pair sum, predictor, 10 x (evaluate, correct), final
+// evaluation and commit, step counter advance.
+int enqueue_step(quest_ctx *c)
+{
+  int rc = pair_sum(c);
+  if(rc != QUEST_OK) return rc;
+  HIPC(c, launch_pc_begin(c->k, c->d, c->P.history_window, c->stream));
+  for(int it = 0; it <= kNumCorrectorSteps; ++it)
+    HIPC(c, launch_pc_iter(c->k, c->d, c->P.precision, c->fc, it, c->stream));
+  HIPC(c, launch_advance_step(c->d, c->stream));
+  return QUEST_OK;
+}
+
+// One step captured as a graph (kernel launches only; no memsets), replayed
+// for every step: the kernels read the step number from the device counter.
+int ensure_graph(quest_ctx *c)
+{
+  if(c->graph_exec) return QUEST_OK;
+  HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const bool was_timing = c->timing;
+  c->timing = false;
+  const int rc = enqueue_step(c);
+  c->timing = was_timing;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if(rc != QUEST_OK) {
+    if(g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if(e != hipSuccess) return fail(c, QUEST_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  c->graph = g;
+  HIPC(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
   return QUEST_OK;
 }
 
 void free_device(quest_ctx *c)
 {
+  drop_graph(c);
   DeviceState &d = c->d;
   void *ptrs[] = {d.pos64, d.dip64, d.pos32, d.dip32, d.freq, d.t1, d.t2, d.ring, d.y0, d.y1,
                   d.d0, d.d1, d.cur0, d.cur1, d.yc0, d.yc1, d.field, d.partial, d.phase, d.out,
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
-                  d.bad_step, d.counts, d.perm_dev};
+                  d.bad_step, d.counts, d.perm_dev, d.sys_of, d.sys_range, d.sys_chunks,
+                  d.pulses, d.work, d.work_row, d.step};
   for(void *p : ptrs)
     if(p) (void)hipFree(p);
   d = DeviceState{};
@@ -240,27 +375,48 @@ void free_device(quest_ctx *c)
   c->stream = nullptr;
 }
 
-int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const double *dip,
-           const double *freq, const double *t1, const double *t2)
+int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, const double *pos,
+           const double *dip, const double *freq, const double *t1, const double *t2)
 {
   c->P = *p;
+  c->n_sys = n_sys;
+  c->sys_begin.assign(n_sys + 1, 0);
+  for(int s = 0; s < n_sys; ++s) c->sys_begin[s + 1] = c->sys_begin[s] + sizes[s];
+  const int n = c->sys_begin[n_sys];
   c->n = n;
   HIPC(c, hipSetDevice(p->device));
   HIPC(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPC(c, hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, p->device));
 
-  c->perm = morton_order(n, pos);
+  // Morton order inside every system; systems stay contiguous
+  c->perm.clear();
+  c->perm.reserve(n);
+  for(int s = 0; s < n_sys; ++s) {
+    const std::vector<int> o = morton_order(c->sys_begin[s], c->sys_begin[s + 1], pos);
+    c->perm.insert(c->perm.end(), o.begin(), o.end());
+  }
   c->inv.assign(n, 0);
   for(int s = 0; s < n; ++s) c->inv[c->perm[s]] = s;
 
-  // bounding-box diagonal bounds every pair's delay
+  // the largest bounding-box diagonal of a system bounds every pair's delay
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for(int i = 0; i < n; ++i)
+  double diag = 0;
+  for(int s = 0; s < n_sys; ++s) {
+    if(c->sys_begin[s + 1] == c->sys_begin[s]) continue;
+    double slo[3] = {INFINITY, INFINITY, INFINITY}, shi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(int i = c->sys_begin[s]; i < c->sys_begin[s + 1]; ++i)
+      for(int a = 0; a < 3; ++a) {
+        slo[a] = std::min(slo[a], pos[3 * i + a]);
+        shi[a] = std::max(shi[a], pos[3 * i + a]);
+      }
+    double dd = 0;
     for(int a = 0; a < 3; ++a) {
-      lo[a] = std::min(lo[a], pos[3 * i + a]);
-      hi[a] = std::max(hi[a], pos[3 * i + a]);
+      dd += (shi[a] - slo[a]) * (shi[a] - slo[a]);
+      lo[a] = std::min(lo[a], slo[a]);
+      hi[a] = std::max(hi[a], shi[a]);
     }
-  const double diag = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
-                                (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    diag = std::max(diag, std::sqrt(dd));
+  }
   const double cdt = p->c0 * p->dt;
   if(!(diag / cdt < 1e6)) return fail(c, QUEST_ERR_ARG, "system extent / (c0 dt) too large");
   c->dmax_bound = static_cast<int>(std::floor(diag / cdt)) + 2;
@@ -272,9 +428,6 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
   k.ring_slots = next_pow2(c->dmax_bound + 1);
   k.ring_mask = k.ring_slots - 1;
   k.tiles = (n + kBlock - 1) / kBlock;
-  k.splits = p->source_split > 0 ? p->source_split : default_splits(n, k.tiles);
-  k.src_per_split = (n + k.splits - 1) / k.splits;
-  k.splits = (n + k.src_per_split - 1) / k.src_per_split;
   k.state_slots = next_pow2(std::max(p->history_window + 1, p->interp_order + 3));
   k.state_mask = k.state_slots - 1;
   k.c = p->c0;
@@ -293,24 +446,33 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
   k.m_o2dt = -2 * p->laser_omega / p->dt;
   k.two_idt2 = 2 / (p->dt * p->dt);
   lagrange_monomials(p->interp_order, k.vinv);
-  // FP32 positions are stored relative to the box centre; each coordinate is
-  // off by at most half an ulp of the largest centred coordinate, so a pair
-  // distance by at most sqrt(3) ulp.  The arithmetic (squares, sum, rsqrt,
-  // two products, the rounded 1/(c0 dt)) adds at most ~3.5e-7 relative.
-  double ctr[3], half = 0;
-  for(int a = 0; a < 3; ++a) {
-    ctr[a] = 0.5 * (lo[a] + hi[a]);
-    half = std::max(half, 0.5 * (hi[a] - lo[a]));
-  }
+  k.dyadic = p->dyadic;
+  k.n_systems = n_sys;
+  // FP32 positions are stored relative to the centre of all systems' boxes;
+  // each coordinate is off by at most half an ulp of the largest centred
+  // coordinate, so a pair distance by at most sqrt(3) ulp.  The arithmetic
+  // (squares, sum, rsqrt, two products, the rounded 1/(c0 dt)) adds at most
+  // ~3.5e-7 relative.
+  double ctr[3] = {0, 0, 0}, half = 0;
+  if(diag > 0 || n > 0)
+    for(int a = 0; a < 3; ++a) {
+      ctr[a] = 0.5 * (lo[a] + hi[a]);
+      half = std::max(half, 0.5 * (hi[a] - lo[a]));
+    }
   const double ulp = half > 0 ? std::ldexp(1.0, std::ilogb(half) - 23) : 0x1p-126;
   k.eps_rel = 8e-7;
   k.eps_abs = 2.0 * std::sqrt(3.0) * ulp / cdt + 2e-6;
-  k.dyadic = p->dyadic;
 
   // sorted host copies
   std::vector<double4> p64(n), d64(n);
   std::vector<float4> p32(n), d32(n);
   std::vector<double> f(n, 0.0), a1(n, 1.0), a2(n, 1.0);
+  std::vector<int> sys_of(n);
+  std::vector<int2> ranges(n_sys);
+  for(int s = 0; s < n_sys; ++s) {
+    ranges[s] = make_int2(c->sys_begin[s], c->sys_begin[s + 1]);
+    for(int i = c->sys_begin[s]; i < c->sys_begin[s + 1]; ++i) sys_of[i] = s;
+  }
   for(int s = 0; s < n; ++s) {
     const int i = c->perm[s];
     p64[s] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
@@ -351,13 +513,17 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
   HIPC(c, alloc(reinterpret_cast<void **>(&d.yc0), sizeof(double2) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.yc1), sizeof(double2) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.field), sizeof(double2) * nn));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.partial), sizeof(double2) * nn * k.splits));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.out), sizeof(double2) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.near_start), sizeof(int) * (nn + 1)));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.counts), sizeof(int) * (nn + 1)));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.pcw), sizeof(double) * 4 * kMaxWindow));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.bad_step), sizeof(int)));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.step), sizeof(int)));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.perm_dev), sizeof(int) * nn));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_of), sizeof(int) * nn));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_range), sizeof(int2) * n_sys));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_chunks), sizeof(int) * n_sys));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.pulses), sizeof(PulseParams) * n_sys));
   HIPC(c, hipHostMalloc(reinterpret_cast<void **>(&c->h_stage), sizeof(double2) * nn * 2));
 
   HIPC(c, hipMemcpyAsync(d.pos64, p64.data(), sizeof(double4) * nn, hipMemcpyHostToDevice, c->stream));
@@ -368,6 +534,16 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
   HIPC(c, hipMemcpyAsync(d.t1, a1.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.t2, a2.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.perm_dev, c->inv.data(), sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(d.sys_of, sys_of.data(), sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(d.sys_range, ranges.data(), sizeof(int2) * n_sys, hipMemcpyHostToDevice, c->stream));
+  c->pulses.assign(n_sys, PulseParams{});
+  int rc = upload_pulses(c);
+  if(rc != QUEST_OK) return rc;
+
+  // pair-kernel work list sized for one wave of resident blocks
+  c->blocks_per_cu = pair_blocks_per_cu(k, p->precision);
+  rc = build_work(c, p->source_chunk > 0 ? p->source_chunk : default_chunk(c));
+  if(rc != QUEST_OK) return rc;
 
   // zero-delay list: count, scan on the host, fill in source order, coefficients
   HIPC(c, launch_near_count(k, d, p->precision, c->stream));
@@ -386,7 +562,7 @@ int create(quest_ctx *c, const quest_params *p, int n, const double *pos, const 
   HIPC(c, launch_near_fill(k, d, p->precision, c->stream));
   HIPC(c, launch_near_coef(k, d, c->near_total, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  int rc = ensure_phase(c, 1024);
+  rc = ensure_phase(c, 1024);
   if(rc != QUEST_OK) return rc;
   return start_state(c);
 }
@@ -406,7 +582,53 @@ int check_params(const quest_params *p, int n)
   if(!(p->dt > 0) || !(p->c0 > 0) || !(p->hbar > 0) || !std::isfinite(p->mu0) ||
      !std::isfinite(p->laser_omega))
     return fail(nullptr, QUEST_ERR_ARG, "constants must be finite, dt, c0, hbar > 0");
-  if(p->source_split < 0) return fail(nullptr, QUEST_ERR_ARG, "source_split must be >= 0");
+  if(p->source_chunk < 0) return fail(nullptr, QUEST_ERR_ARG, "source_chunk must be >= 0");
+  return QUEST_OK;
+}
+
+PulseParams to_params(const quest_pulse *pl)
+{
+  PulseParams q{};
+  if(!pl) return q;
+  q.on = 1;
+  q.amplitude = pl->amplitude;
+  q.delay = pl->delay;
+  q.width = pl->width;
+  q.freq = pl->freq;
+  for(int a = 0; a < 3; ++a) {
+    q.k[a] = pl->wavevector[a];
+    q.pol[a] = pl->polarization[a];
+  }
+  return q;
+}
+
+int create_checked(const quest_params *params, int n_sys, const int *sizes, const double *pos,
+                   const double *dip, const double *freq, const double *t1, const double *t2,
+                   quest_ctx **out)
+{
+  if(!out) return fail(nullptr, QUEST_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int64_t n = 0;
+  for(int s = 0; s < n_sys; ++s) {
+    if(sizes[s] < 0) return fail(nullptr, QUEST_ERR_ARG, "system sizes must be >= 0");
+    n += sizes[s];
+  }
+  if(n > INT_MAX / 2) return fail(nullptr, QUEST_ERR_ARG, "too many dots");
+  int rc = check_params(params, static_cast<int>(n));
+  if(rc != QUEST_OK) return rc;
+  if(!pos || !dip) return fail(nullptr, QUEST_ERR_ARG, "pos and dip are required");
+  for(int64_t i = 0; i < 3 * n; ++i)
+    if(!std::isfinite(pos[i])) return fail(nullptr, QUEST_ERR_ARG, "non-finite position");
+  quest_ctx *c = new(std::nothrow) quest_ctx;
+  if(!c) return fail(nullptr, QUEST_ERR_NOMEM, "context allocation failed");
+  rc = create(c, params, n_sys, sizes, pos, dip, freq, t1, t2);
+  if(rc != QUEST_OK) {
+    g_create_error = c->err;
+    free_device(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
   return QUEST_OK;
 }
 
@@ -418,24 +640,16 @@ int quest_ctx_create(const quest_params *params, int n_dots, const double *pos,
                      const double *dip, const double *freq, const double *t1,
                      const double *t2, quest_ctx **out)
 {
-  if(!out) return fail(nullptr, QUEST_ERR_ARG, "out is NULL");
-  *out = nullptr;
-  int rc = check_params(params, n_dots);
-  if(rc != QUEST_OK) return rc;
-  if(!pos || !dip) return fail(nullptr, QUEST_ERR_ARG, "pos and dip are required");
-  for(int i = 0; i < 3 * n_dots; ++i)
-    if(!std::isfinite(pos[i])) return fail(nullptr, QUEST_ERR_ARG, "non-finite position");
-  quest_ctx *c = new(std::nothrow) quest_ctx;
-  if(!c) return fail(nullptr, QUEST_ERR_NOMEM, "context allocation failed");
-  rc = create(c, params, n_dots, pos, dip, freq, t1, t2);
-  if(rc != QUEST_OK) {
-    g_create_error = c->err;
-    free_device(c);
-    delete c;
-    return rc;
-  }
-  *out = c;
-  return QUEST_OK;
+  const int sizes[1] = {n_dots};
+  return create_checked(params, 1, sizes, pos, dip, freq, t1, t2, out);
+}
+
+int quest_batch_create(const quest_params *params, int n_systems, const int *n_dots,
+                       const double *pos, const double *dip, const double *freq,
+                       const double *t1, const double *t2, quest_ctx **out)
+{
+  if(n_systems < 1 || !n_dots) return fail(nullptr, QUEST_ERR_ARG, "n_systems >= 1 and n_dots required");
+  return create_checked(params, n_systems, n_dots, pos, dip, freq, t1, t2, out);
 }
 
 void quest_ctx_destroy(quest_ctx *ctx)
@@ -471,9 +685,10 @@ int quest_hist_evaluate(quest_ctx *c, int step, const double *rho01, double *out
   if(rc != QUEST_OK) return rc;
   if(step != c->cur_step) {
     if(c->cur_step >= 0) HIPC(c, launch_hist_commit(c->k, c->d, c->P.precision, c->cur_step, c->stream));
-    rc = pair_sum(c, step);
+    HIPC(c, launch_set_step(c->d, step, c->stream));
+    rc = pair_sum(c);
     if(rc != QUEST_OK) return rc;
-    HIPC(c, launch_hist_reduce(c->k, c->d, step, c->stream));
+    HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
     c->cur_step = step;
   }
   const int n = c->n;
@@ -482,7 +697,7 @@ int quest_hist_evaluate(quest_ctx *c, int step, const double *rho01, double *out
     c->h_stage[s] = make_double2(rho01[2 * i], rho01[2 * i + 1]);
   }
   HIPC(c, hipMemcpyAsync(c->d.cur1, c->h_stage, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
-  HIPC(c, launch_hist_near(c->k, c->d, step, c->stream));
+  HIPC(c, launch_hist_near(c->k, c->d, c->stream));
   HIPC(c, hipMemcpyAsync(c->h_stage + n, c->d.out, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   for(int s = 0; s < n; ++s) {
@@ -497,29 +712,30 @@ int quest_hist_evaluate(quest_ctx *c, int step, const double *rho01, double *out
 int quest_set_pulse(quest_ctx *c, const quest_pulse *pl)
 {
   if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
-  if(!pl) {
-    c->pulse = PulseParams{};
-    return QUEST_OK;
-  }
-  PulseParams &q = c->pulse;
-  q.on = 1;
-  q.amplitude = pl->amplitude;
-  q.delay = pl->delay;
-  q.width = pl->width;
-  q.freq = pl->freq;
-  for(int a = 0; a < 3; ++a) {
-    q.k[a] = pl->wavevector[a];
-    q.pol[a] = pl->polarization[a];
-  }
-  return QUEST_OK;
+  HIPC(c, hipSetDevice(c->P.device));
+  c->pulses.assign(c->n_sys, to_params(pl));
+  return upload_pulses(c);
+}
+
+int quest_set_pulses(quest_ctx *c, const quest_pulse *pulses)
+{
+  if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
+  if(!pulses) return fail(c, QUEST_ERR_ARG, "pulses is required (one per system)");
+  HIPC(c, hipSetDevice(c->P.device));
+  for(int s = 0; s < c->n_sys; ++s) c->pulses[s] = to_params(pulses + s);
+  return upload_pulses(c);
 }
 
 int quest_pulse_evaluate(quest_ctx *c, int step, double *out)
 {
   if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
   if(!out) return fail(c, QUEST_ERR_ARG, "out is required");
+  if(c->mode == 2) return fail(c, QUEST_ERR_STATE, "context is driven by quest_run");
   HIPC(c, hipSetDevice(c->P.device));
-  HIPC(c, launch_pulse(c->k, c->d, c->pulse, step, c->stream));
+  // the device step counter belongs to the per-call sequence: set, use, restore
+  HIPC(c, launch_set_step(c->d, step, c->stream));
+  HIPC(c, launch_pulse(c->k, c->d, c->stream));
+  if(c->mode == 1 && c->cur_step >= 0) HIPC(c, launch_set_step(c->d, c->cur_step, c->stream));
   const int n = c->n;
   HIPC(c, hipMemcpyAsync(c->h_stage + n, c->d.out, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -564,19 +780,23 @@ int quest_run(quest_ctx *c, int n_steps)
   if(c->mode == 0) {
     c->mode = 2;
     c->cur_step = 0;
+    HIPC(c, launch_set_step(c->d, 0, c->stream));
   }
   if(c->bad >= 0) return fail(c, QUEST_ERR_DOMAIN, "unbounded history values at or before step " + std::to_string(c->bad));
   int rc = ensure_pc(c);
   if(rc != QUEST_OK) return rc;
   rc = ensure_phase(c, c->cur_step + n_steps + 1);
   if(rc != QUEST_OK) return rc;
-  const int W = c->P.history_window;
-  for(int step = c->cur_step; step < c->cur_step + n_steps; ++step) {
-    rc = pair_sum(c, step);
+  if(c->timing) {
+    // kernel by kernel, so that HIP events can bracket every pair-kernel launch
+    for(int i = 0; i < n_steps; ++i) {
+      rc = enqueue_step(c);
+      if(rc != QUEST_OK) return rc;
+    }
+  } else if(n_steps > 0) {
+    rc = ensure_graph(c);
     if(rc != QUEST_OK) return rc;
-    HIPC(c, launch_pc_begin(c->k, c->d, c->pulse, W, step, c->stream));
-    for(int it = 0; it <= kNumCorrectorSteps; ++it)
-      HIPC(c, launch_pc_iter(c->k, c->d, c->P.precision, W, c->fc, step, it, c->stream));
+    for(int i = 0; i < n_steps; ++i) HIPC(c, hipGraphLaunch(c->graph_exec, c->stream));
   }
   int bad = INT_MAX;
   HIPC(c, hipMemcpyAsync(&bad, c->d.bad_step, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -590,6 +810,63 @@ int quest_run(quest_ctx *c, int n_steps)
     c->bad = bad;
     return fail(c, QUEST_ERR_DOMAIN, "unbounded history values at or before step " + std::to_string(bad));
   }
+  return QUEST_OK;
+}
+
+int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
+{
+  if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
+  if(launches < 1) launches = 3;
+  HIPC(c, hipSetDevice(c->P.device));
+  if(c->k.n_work == 0) {
+    if(chunk_len) *chunk_len = c->chunk_len;
+    return QUEST_OK;
+  }
+  int nmax = 2;
+  for(int s = 0; s < c->n_sys; ++s) nmax = std::max(nmax, c->sys_begin[s + 1] - c->sys_begin[s]);
+  const int base = default_chunk(c);
+  std::vector<int> cand;
+  for(int f : {-2, -1, 0, 1, 2}) {
+    int len = f >= 0 ? base << f : base >> -f;
+    len = std::max(32, std::min(len, nmax));
+    if(std::find(cand.begin(), cand.end(), len) == cand.end()) cand.push_back(len);
+  }
+  // The pair kernel reads the ring and writes only the partial sums, which
+  // every step recomputes from scratch, so timing it between steps changes no
+  // state; the device step counter is restored below.
+  HIPC(c, launch_set_step(c->d, std::max(c->cur_step, 0), c->stream));
+  int best = c->chunk_len;
+  float best_ms = INFINITY;
+  hipEvent_t e0, e1;
+  HIPC(c, hipEventCreate(&e0));
+  HIPC(c, hipEventCreate(&e1));
+  for(int len : cand) {
+    int rc = build_work(c, len);
+    if(rc != QUEST_OK) return rc;
+    HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
+    HIPC(c, hipEventRecord(e0, c->stream));
+    for(int i = 0; i < launches; ++i) HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));
+    HIPC(c, hipEventRecord(e1, c->stream));
+    HIPC(c, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+    if(ms < best_ms) {
+      best_ms = ms;
+      best = len;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  int rc = build_work(c, best);
+  if(rc != QUEST_OK) return rc;
+  if(c->mode == 1 && c->cur_step >= 0) {
+    // per-call mode keeps the current step's pair sum in `field`: recompute it
+    rc = pair_sum(c);
+    if(rc != QUEST_OK) return rc;
+    HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if(chunk_len) *chunk_len = best;
   return QUEST_OK;
 }
 
@@ -688,10 +965,14 @@ int quest_ctx_info(const quest_ctx *c, int64_t *info)
   info[1] = c->k.ring_slots;
   info[2] = c->dmax_bound;
   info[3] = c->near_total;
-  info[4] = c->k.splits;
-  info[5] = c->k.tiles;
+  info[4] = c->k.n_work;
+  info[5] = c->k.grid;
   info[6] = c->k.state_slots;
   info[7] = c->P.precision;
+  info[8] = c->n_sys;
+  info[9] = c->chunk_len;
+  info[10] = c->k.max_chunks;
+  info[11] = c->graph_exec != nullptr;
   return QUEST_OK;
 }
 

@@ -27,9 +27,8 @@ struct Consts {
   int kstride;        // complex entries per ring slot (M + 1)
   int ring_slots;     // power of two > max delay
   int ring_mask;
-  int splits;         // source splits of the pair kernel
-  int src_per_split;
-  int tiles;          // target tiles of kBlock dots
+  int grid;           // blocks of the persistent pair kernel (a multiple of 8)
+  int tiles;          // target tiles of kBlock dots (all systems)
   int state_slots;    // state ring: power of two > max(window, order + 2)
   int state_mask;
   double c, dt, cdt, icdt, omega, theta;   // theta = omega dt
@@ -42,6 +41,9 @@ struct Consts {
   // monomial coefficients of the Lagrange basis: vinv[k][m] = [x^k] L_m(x)
   double vinv[(kMaxOrder + 1) * (kMaxOrder + 1)];
   int dyadic;         // QUEST_DYADIC_*
+  int n_systems;      // independent dot systems (pairs only within a system)
+  int n_work;         // pair-kernel work items (system, target tile, source chunk)
+  int max_chunks;     // largest number of source chunks of one system (rows of `partial`)
 };
 
 struct PulseParams {
@@ -61,7 +63,7 @@ struct DeviceState {
   double2 *cur0 = nullptr, *cur1 = nullptr;  // iterate double buffer [2][n]
   double2 *yc0 = nullptr, *yc1 = nullptr;    // corrector base [n]
   double2 *field = nullptr;            // fixed part of the field at this step [n]
-  double2 *partial = nullptr;          // [splits][n]
+  double2 *partial = nullptr;          // [max_chunks][n]
   double2 *phase = nullptr;            // e^{i theta s}, s in [0, phase_len)
   double2 *out = nullptr;              // [n] staging of per-call results
   int *near_start = nullptr;           // [n + 1]
@@ -73,22 +75,33 @@ struct DeviceState {
   int *bad_step = nullptr;             // first non-finite step (INT_MAX = none)
   int *counts = nullptr;               // scratch [n + 1]
   int *perm_dev = nullptr;             // inverse permutation (original -> sorted)
+  int *sys_of = nullptr;               // [n] system of each dot
+  int2 *sys_range = nullptr;           // [systems] dot range [begin, end)
+  int *sys_chunks = nullptr;           // [systems] source chunks (partial rows used)
+  PulseParams *pulses = nullptr;       // [systems]
+  int4 *work = nullptr;                // [n_work] (target begin, target end, source begin, source end)
+  int *work_row = nullptr;             // [n_work] partial row written by the item
+  int *step = nullptr;                 // the step being integrated (device counter; graph replays read it)
 };
 
 // ---------------------------------------------------------------- launchers
 // All launch on `s` and return the hipError_t of the launch.
 hipError_t launch_fill_ground_state(const Consts &k, DeviceState &d, hipStream_t s);
 hipError_t launch_clear_ring(const Consts &k, DeviceState &d, size_t ring_bytes, hipStream_t s);
-hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s);
-hipError_t launch_pc_begin(const Consts &k, DeviceState &d, const PulseParams &pl,
-                           int window, int step, hipStream_t s);
-hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, int window,
-                          double future_coef, int step, int iter, hipStream_t s);
+// Kernels of the step read the step number from d.step (so that one captured
+// graph serves every step); launch_set_step / launch_advance_step write it.
+hipError_t launch_set_step(DeviceState &d, int step, hipStream_t s);
+hipError_t launch_advance_step(DeviceState &d, hipStream_t s);
+hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipStream_t s);
+hipError_t launch_pc_begin(const Consts &k, DeviceState &d, int window, hipStream_t s);
+hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, double future_coef,
+                          int iter, hipStream_t s);
 hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s);
-hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, int step, hipStream_t s);
-hipError_t launch_hist_near(const Consts &k, DeviceState &d, int step, hipStream_t s);
-hipError_t launch_pulse(const Consts &k, DeviceState &d, const PulseParams &pl, int step,
-                        hipStream_t s);
+hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, hipStream_t s);
+hipError_t launch_hist_near(const Consts &k, DeviceState &d, hipStream_t s);
+hipError_t launch_pulse(const Consts &k, DeviceState &d, hipStream_t s);
+// Occupancy of the pair kernel (blocks per CU) for sizing the persistent grid.
+int pair_blocks_per_cu(const Consts &k, int precision);
 hipError_t launch_near_count(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s);

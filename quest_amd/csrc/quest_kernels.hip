@@ -244,97 +244,99 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
 
 // ------------------------------------------------------------ pair kernels
 
-// Past-only pair sum: every ordered pair (a <- b), b != a, whose integer delay
-// n >= 1 reads finished steps only, so this sum is formed once per step.  The
-// zero-delay pairs are excluded here and evaluated at every corrector
-// iteration from the near list.  Grid: tiles * splits blocks of kBlock
-// threads; block id -> (split = id % splits, tile = id / splits), so with 8
-// splits each XCD (round-robin block dispatch) walks one source range.
-#ifndef QUEST_PAIR_UNROLL
-#define QUEST_PAIR_UNROLL 1
-#endif
+// Past-only pair sum: every ordered pair (a <- b) of one system, b != a,
+// whose integer delay n >= 1 reads finished steps only, so this sum is formed
+// once per step; the zero-delay pairs are evaluated at every corrector
+// iteration from the near list.
+//
+// Persistent kernel over a work list: item w = (target tile of kBlock dots,
+// source range) of one system; block i takes items i, i + grid, ...  Items are
+// ordered tile-major with the source chunks of a tile consecutive, and `grid`
+// is a multiple of 8, so with 8 chunks per tile each XCD (round-robin block
+// dispatch) walks one source range and its L2 holds that range's ring rows.
 constexpr int kChunk = 64;   // sources summed in the pair type before the FP64 fold
 
 template <int M, bool LAB>
 __global__ __launch_bounds__(kBlock) void pair_past_f64(
     const double4 *__restrict__ pos, const double4 *__restrict__ dip,
-    const double2 *__restrict__ ring, double2 *__restrict__ partial, Consts k, int step)
+    const double2 *__restrict__ ring, const int4 *__restrict__ work, const int *__restrict__ work_row,
+    const int *__restrict__ step_ptr, double2 *__restrict__ partial, Consts k)
 {
   const PairK<double> K(k);
-  const int split = blockIdx.x % k.splits;
-  const int tile = blockIdx.x / k.splits;
-  const int tgt = tile * kBlock + threadIdx.x;
-  const int a = tgt < k.n ? tgt : k.n - 1;
-  const double4 pa = pos[a], da = dip[a];
-  const int b0 = split * k.src_per_split;
-  const int b1 = min(k.n, b0 + k.src_per_split);
+  const int step = *step_ptr;
   const size_t src_stride = static_cast<size_t>(k.ring_slots) * slot_stride<M>();
-  double accr = 0, acci = 0;
-  for(int b = b0; b < b1; ++b) {
-    const double4 pb = pos[b], db = dip[b];
-    const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-    double x, ir, r2;
-    const int n = split_f64(dx, dy, dz, K.cdt, K.icdt, x, ir, r2);
-    const bool valid = (b != a) && (n >= 1);
-    const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
-    const double2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
-    double2 cv[M + 1];
+  for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
+    const int4 wi = work[w];
+    const int tgt = wi.x + static_cast<int>(threadIdx.x);
+    const int a = tgt < wi.y ? tgt : wi.y - 1;
+    const double4 pa = pos[a], da = dip[a];
+    double accr = 0, acci = 0;
+    for(int b = wi.z; b < wi.w; ++b) {
+      const double4 pb = pos[b], db = dip[b];
+      const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+      double x, ir, r2;
+      const int n = split_f64(dx, dy, dz, K.cdt, K.icdt64, x, ir, r2);
+      const bool valid = (b != a) && (n >= 1);
+      const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
+      const double2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
+      double2 cv[M + 1];
 #pragma unroll
-    for(int i = 0; i <= M; ++i) cv[i] = c[i];
-    double q0r, q0i, q1r, q1i, q2r, q2i;
-    horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-    double tr, ti;
-    pair_term<double, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
-    accr += valid ? tr : 0.0;
-    acci += valid ? ti : 0.0;
+      for(int i = 0; i <= M; ++i) cv[i] = c[i];
+      double q0r, q0i, q1r, q1i, q2r, q2i;
+      horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
+      double tr, ti;
+      pair_term<double, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
+      accr += valid ? tr : 0.0;
+      acci += valid ? ti : 0.0;
+    }
+    if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
   }
-  if(tgt < k.n) partial[static_cast<size_t>(split) * k.n + tgt] = make_double2(accr, acci);
 }
 
 template <int M, bool LAB>
 __global__ __launch_bounds__(kBlock) void pair_past_f32(
     const float4 *__restrict__ pos, const float4 *__restrict__ dip,
     const double4 *__restrict__ pos64, const float2 *__restrict__ ring,
-    double2 *__restrict__ partial, Consts k, int step)
+    const int4 *__restrict__ work, const int *__restrict__ work_row, const int *__restrict__ step_ptr,
+    double2 *__restrict__ partial, Consts k)
 {
   const PairK<float> K(k);
-  const int split = blockIdx.x % k.splits;
-  const int tile = blockIdx.x / k.splits;
-  const int tgt = tile * kBlock + threadIdx.x;
-  const int a = tgt < k.n ? tgt : k.n - 1;
-  const float4 pa = pos[a], da = dip[a];
-  const double4 pa64 = pos64[a];
-  const int b0 = split * k.src_per_split;
-  const int b1 = min(k.n, b0 + k.src_per_split);
+  const int step = *step_ptr;
   const size_t src_stride = static_cast<size_t>(k.ring_slots) * slot_stride<M>();
-  double accr = 0, acci = 0;
-  for(int bt = b0; bt < b1; bt += kChunk) {
-    const int be = min(b1, bt + kChunk);
-    float sr = 0, si = 0;
-#pragma unroll QUEST_PAIR_UNROLL
-    for(int b = bt; b < be; ++b) {
-      const float4 pb = pos[b], db = dip[b];
-      const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-      float x, ir;
-      const int n = split_mixed(dx, dy, dz, K.icdt, K.eps_rel, K.eps_abs, pa64, pos64, b, K.cdt,
-                                K.icdt64, x, ir);
-      const bool valid = (b != a) && (n >= 1);
-      const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
-      const float2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
-      float2 cv[M + 1];
+  for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
+    const int4 wi = work[w];
+    const int tgt = wi.x + static_cast<int>(threadIdx.x);
+    const int a = tgt < wi.y ? tgt : wi.y - 1;
+    const float4 pa = pos[a], da = dip[a];
+    const double4 pa64 = pos64[a];
+    double accr = 0, acci = 0;
+    for(int bt = wi.z; bt < wi.w; bt += kChunk) {
+      const int be = min(wi.w, bt + kChunk);
+      float sr = 0, si = 0;
+      for(int b = bt; b < be; ++b) {
+        const float4 pb = pos[b], db = dip[b];
+        const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+        float x, ir;
+        const int n = split_mixed(dx, dy, dz, K.icdt, K.eps_rel, K.eps_abs, pa64, pos64, b, K.cdt,
+                                  K.icdt64, x, ir);
+        const bool valid = (b != a) && (n >= 1);
+        const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
+        const float2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
+        float2 cv[M + 1];
 #pragma unroll
-      for(int i = 0; i <= M; ++i) cv[i] = c[i];
-      float q0r, q0i, q1r, q1i, q2r, q2i;
-      horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-      float tr, ti;
-      pair_term<float, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
-      sr += valid ? tr : 0.0f;
-      si += valid ? ti : 0.0f;
+        for(int i = 0; i <= M; ++i) cv[i] = c[i];
+        float q0r, q0i, q1r, q1i, q2r, q2i;
+        horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
+        float tr, ti;
+        pair_term<float, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
+        sr += valid ? tr : 0.0f;
+        si += valid ? ti : 0.0f;
+      }
+      accr += static_cast<double>(sr);
+      acci += static_cast<double>(si);
     }
-    accr += static_cast<double>(sr);
-    acci += static_cast<double>(si);
+    if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
   }
-  if(tgt < k.n) partial[static_cast<size_t>(split) * k.n + tgt] = make_double2(accr, acci);
 }
 
 // ------------------------------------------------------- per-dot kernels
@@ -347,10 +349,12 @@ __device__ __forceinline__ double2 hist_scale(const Consts &k, const double2 *ph
   return make_double2(-k.k_green * ph.x, k.k_green * ph.y);
 }
 
-__device__ __forceinline__ double2 reduce_partials(const Consts &k, const double2 *partial, int a)
+// Sum of the pair-kernel partials of target a over its system's source chunks.
+__device__ __forceinline__ double2 reduce_partials(const Consts &k, const DeviceState &d, int a)
 {
   double2 s = make_double2(0.0, 0.0);
-  for(int j = 0; j < k.splits; ++j) s = cadd(s, partial[static_cast<size_t>(j) * k.n + a]);
+  const int rows = d.sys_chunks[d.sys_of[a]];
+  for(int j = 0; j < rows; ++j) s = cadd(s, d.partial[static_cast<size_t>(j) * k.n + a]);
   return s;
 }
 
@@ -474,15 +478,15 @@ __global__ void clear_words_kernel(uint4 *p, size_t n16)
 // Start of a step (whole-run mode): fixed field (pulse + past pair sum), the
 // predictor (integrator.cpp:142-154) and the step-invariant part of the
 // corrector (integrator.cpp:156-169).
-__global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState d, PulseParams pl,
-                                                          int window, int step)
+__global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState d, int window)
 {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if(a >= k.n) return;
+  const int step = *d.step;
   const double dt = k.dt;
   double2 hist = make_double2(0.0, 0.0);
-  if(k.n > 1) hist = cmul(reduce_partials(k, d.partial, a), hist_scale(k, d.phase, step));
-  const double2 pulse = pulse_field(pl, k.hbar, d.pos64[a], d.dip64[a], step * dt);
+  if(k.n_work > 0) hist = cmul(reduce_partials(k, d, a), hist_scale(k, d.phase, step));
+  const double2 pulse = pulse_field(d.pulses[d.sys_of[a]], k.hbar, d.pos64[a], d.dip64[a], step * dt);
   d.field[a] = cadd(pulse, hist);
 
   const double *ps0 = d.pcw, *ps1 = d.pcw + window;
@@ -514,10 +518,11 @@ __global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState 
 // writes the source's interpolant slot for the next steps' pair sums.
 template <typename R2>
 __global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d, R2 *ring,
-                                                         double future_coef, int step, int iter)
+                                                         double future_coef, int iter)
 {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if(a >= k.n) return;
+  const int step = *d.step;
   const int rd = iter & 1, wr = rd ^ 1;
   const double2 *cur0 = d.cur0 + static_cast<size_t>(rd) * k.n;
   const double2 *cur1 = d.cur1 + static_cast<size_t>(rd) * k.n;
@@ -555,32 +560,36 @@ __global__ __launch_bounds__(kBlock) void hist_commit_kernel(Consts k, DeviceSta
 }
 
 // Per-call mode: the pair sum of the finished history, scaled.
-__global__ __launch_bounds__(kBlock) void hist_reduce_kernel(Consts k, DeviceState d, int step)
+__global__ __launch_bounds__(kBlock) void hist_reduce_kernel(Consts k, DeviceState d)
 {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if(a >= k.n) return;
+  const int step = *d.step;
   double2 hist = make_double2(0.0, 0.0);
-  if(k.n > 1) hist = cmul(reduce_partials(k, d.partial, a), hist_scale(k, d.phase, step));
+  if(k.n_work > 0) hist = cmul(reduce_partials(k, d, a), hist_scale(k, d.phase, step));
   d.field[a] = hist;
 }
 
 // Per-call mode: out = past pair sum + zero-delay pairs with the current rho01.
-__global__ __launch_bounds__(kBlock) void hist_near_kernel(Consts k, DeviceState d, int step)
+__global__ __launch_bounds__(kBlock) void hist_near_kernel(Consts k, DeviceState d)
 {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if(a >= k.n) return;
+  const int step = *d.step;
   double2 om = d.field[a];
   if(d.near_start[a + 1] > d.near_start[a]) om = cadd(om, near_sum(k, d, d.cur1, a, step));
   d.out[a] = om;
 }
 
-__global__ __launch_bounds__(kBlock) void pulse_kernel(Consts k, DeviceState d, PulseParams pl,
-                                                       int step)
+__global__ __launch_bounds__(kBlock) void pulse_kernel(Consts k, DeviceState d)
 {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if(a >= k.n) return;
-  d.out[a] = pulse_field(pl, k.hbar, d.pos64[a], d.dip64[a], step * k.dt);
+  d.out[a] = pulse_field(d.pulses[d.sys_of[a]], k.hbar, d.pos64[a], d.dip64[a], *d.step * k.dt);
 }
+
+__global__ void set_step_kernel(int *step, int value) { *step = value; }
+__global__ void advance_step_kernel(int *step) { *step += 1; }
 
 // ------------------------------------------------------- zero-delay list
 
@@ -594,9 +603,10 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
   if(a >= k.n) return;
   int cnt = 0;
   int base = FILL ? d.near_start[a] : 0;
+  const int2 range = d.sys_range[d.sys_of[a]];
   if constexpr(F64) {
     const double4 pa = d.pos64[a];
-    for(int b = 0; b < k.n; ++b) {
+    for(int b = range.x; b < range.y; ++b) {
       const double4 pb = d.pos64[b];
       double x, ir, r2;
       const int n = split_f64(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, k.cdt, k.icdt, x, ir, r2);
@@ -609,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
     const float4 pa = d.pos32[a];
     const double4 pa64 = d.pos64[a];
     const PairK<float> K(k);
-    for(int b = 0; b < k.n; ++b) {
+    for(int b = range.x; b < range.y; ++b) {
       const float4 pb = d.pos32[b];
       float x, ir;
       const int n = split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.icdt, K.eps_rel,
@@ -715,26 +725,38 @@ __global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_
 inline dim3 dot_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
 template <int M>
-hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s)
+hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
-  const dim3 grid(k.tiles * k.splits), block(kBlock);
+  const dim3 grid(k.grid), block(kBlock);
   const bool lab = k.dyadic == QUEST_DYADIC_LAB;
   if(precision == QUEST_PREC_FP64) {
     if(lab)
       hipLaunchKernelGGL((pair_past_f64<M, true>), grid, block, 0, s, d.pos64, d.dip64,
-                         static_cast<const double2 *>(d.ring), d.partial, k, step);
+                         static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
     else
       hipLaunchKernelGGL((pair_past_f64<M, false>), grid, block, 0, s, d.pos64, d.dip64,
-                         static_cast<const double2 *>(d.ring), d.partial, k, step);
+                         static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
   } else {
     if(lab)
       hipLaunchKernelGGL((pair_past_f32<M, true>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
-                         static_cast<const float2 *>(d.ring), d.partial, k, step);
+                         static_cast<const float2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
     else
       hipLaunchKernelGGL((pair_past_f32<M, false>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
-                         static_cast<const float2 *>(d.ring), d.partial, k, step);
+                         static_cast<const float2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
   }
   return hipGetLastError();
+}
+
+template <int M>
+int blocks_per_cu_m(int precision)
+{
+  int nb = 0;
+  hipError_t e;
+  if(precision == QUEST_PREC_FP64)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false>, kBlock, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false>, kBlock, 0);
+  return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
 }  // namespace
@@ -760,36 +782,59 @@ hipError_t launch_clear_ring(const Consts &k, DeviceState &d, size_t ring_bytes,
   return hipGetLastError();
 }
 
-hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s)
+hipError_t launch_set_step(DeviceState &d, int step, hipStream_t s)
 {
+  hipLaunchKernelGGL(set_step_kernel, dim3(1), dim3(1), 0, s, d.step, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance_step(DeviceState &d, hipStream_t s)
+{
+  hipLaunchKernelGGL(advance_step_kernel, dim3(1), dim3(1), 0, s, d.step);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipStream_t s)
+{
+  if(k.n_work == 0) return hipSuccess;
   switch(k.order) {
-  case 1: return launch_pair_m<1>(k, d, precision, step, s);
-  case 2: return launch_pair_m<2>(k, d, precision, step, s);
-  case 3: return launch_pair_m<3>(k, d, precision, step, s);
-  case 4: return launch_pair_m<4>(k, d, precision, step, s);
-  case 5: return launch_pair_m<5>(k, d, precision, step, s);
-  case 6: return launch_pair_m<6>(k, d, precision, step, s);
+  case 1: return launch_pair_m<1>(k, d, precision, s);
+  case 2: return launch_pair_m<2>(k, d, precision, s);
+  case 3: return launch_pair_m<3>(k, d, precision, s);
+  case 4: return launch_pair_m<4>(k, d, precision, s);
+  case 5: return launch_pair_m<5>(k, d, precision, s);
+  case 6: return launch_pair_m<6>(k, d, precision, s);
   default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_pc_begin(const Consts &k, DeviceState &d, const PulseParams &pl, int window,
-                           int step, hipStream_t s)
+int pair_blocks_per_cu(const Consts &k, int precision)
 {
-  hipLaunchKernelGGL(pc_begin_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, pl, window, step);
+  switch(k.order) {
+  case 1: return blocks_per_cu_m<1>(precision);
+  case 2: return blocks_per_cu_m<2>(precision);
+  case 3: return blocks_per_cu_m<3>(precision);
+  case 4: return blocks_per_cu_m<4>(precision);
+  case 5: return blocks_per_cu_m<5>(precision);
+  default: return blocks_per_cu_m<6>(precision);
+  }
+}
+
+hipError_t launch_pc_begin(const Consts &k, DeviceState &d, int window, hipStream_t s)
+{
+  hipLaunchKernelGGL(pc_begin_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, window);
   return hipGetLastError();
 }
 
-hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, int window,
-                          double future_coef, int step, int iter, hipStream_t s)
+hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, double future_coef,
+                          int iter, hipStream_t s)
 {
-  (void)window;
   if(precision == QUEST_PREC_FP64)
     hipLaunchKernelGGL(pc_iter_kernel<double2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
-                       static_cast<double2 *>(d.ring), future_coef, step, iter);
+                       static_cast<double2 *>(d.ring), future_coef, iter);
   else
     hipLaunchKernelGGL(pc_iter_kernel<float2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
-                       static_cast<float2 *>(d.ring), future_coef, step, iter);
+                       static_cast<float2 *>(d.ring), future_coef, iter);
   return hipGetLastError();
 }
 
@@ -804,22 +849,21 @@ hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, in
   return hipGetLastError();
 }
 
-hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, int step, hipStream_t s)
+hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, hipStream_t s)
 {
-  hipLaunchKernelGGL(hist_reduce_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, step);
+  hipLaunchKernelGGL(hist_reduce_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d);
   return hipGetLastError();
 }
 
-hipError_t launch_hist_near(const Consts &k, DeviceState &d, int step, hipStream_t s)
+hipError_t launch_hist_near(const Consts &k, DeviceState &d, hipStream_t s)
 {
-  hipLaunchKernelGGL(hist_near_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, step);
+  hipLaunchKernelGGL(hist_near_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d);
   return hipGetLastError();
 }
 
-hipError_t launch_pulse(const Consts &k, DeviceState &d, const PulseParams &pl, int step,
-                        hipStream_t s)
+hipError_t launch_pulse(const Consts &k, DeviceState &d, hipStream_t s)
 {
-  hipLaunchKernelGGL(pulse_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d, pl, step);
+  hipLaunchKernelGGL(pulse_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d);
   return hipGetLastError();
 }
 

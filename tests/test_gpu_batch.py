@@ -1,0 +1,119 @@
+"""GPU: batches of independent systems in one context (quest_batch_create),
+the captured step graph, and the autotuner.
+
+Tolerances: batch vs one-context-per-system 1e-12 relative (the work lists
+differ, so summation order differs); graph replay vs kernel-by-kernel launch
+bitwise; a batch member vs the FP64 oracle 1e-10 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import dense_system, oracle_params, oracle_pulse, rel_err, short_pulse
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems():
+    sizes = [37, 300, 5, 129, 1]
+    sys_list = [dense_system(n, 0.12 + 0.002 * n, seed=100 + i, plant_near=n >= 4)
+                for i, n in enumerate(sizes)]
+    pulses = [short_pulse(area=a) for a in (math.pi / 2, math.pi, math.pi / 4, 2.0, 1.0)]
+    return sys_list, pulses
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_matches_individual_contexts(ref_weights, precision):
+    from quest_amd import Context, systems
+    consts = systems.consts()
+    sys_list, pulses = _systems()
+    steps = 250
+    with Context(sys_list, consts, precision=precision, pulse=pulses) as ctx:
+        ctx.set_pc_weights(*ref_weights)
+        info = ctx.info()
+        assert info["systems"] == len(sys_list) and info["n"] == sum(len(s["pos"]) for s in sys_list)
+        ctx.run(steps)
+        batch_rho, _ = ctx.state()
+    off = 0
+    for s, pl in zip(sys_list, pulses):
+        n = len(s["pos"])
+        with Context(s, consts, precision=precision, pulse=pl) as one:
+            one.set_pc_weights(*ref_weights)
+            one.run(steps)
+            rho, _ = one.state()
+        err = rel_err(batch_rho[off:off + n], rho)
+        print(f"system of {n}: batch vs single {err:.2e}")
+        assert err < 1e-12
+        off += n
+
+
+def test_batch_member_matches_oracle(oracle_lib, ref_weights):
+    from quest_amd import Context, systems
+    consts = systems.consts()
+    sys_list, pulses = _systems()
+    steps = 300
+    with Context(sys_list, consts, pulse=pulses) as ctx:
+        ctx.set_pc_weights(*ref_weights)
+        ctx.run(steps)
+        rho, drho = ctx.state()
+    ps, cs, fc = ref_weights
+    off = 37
+    ref = oracle_lib.solve(oracle_params(oracle_lib, consts), sys_list[1], oracle_pulse(oracle_lib, pulses[1]),
+                           ps, cs, fc, steps, mode=1)
+    err = rel_err(rho[off:off + 300], ref["y"])
+    print(f"batch member (N=300) vs oracle: {err:.2e}, max|rho01| {np.abs(ref['y'][:, 1]).max():.3f}")
+    assert np.abs(ref["y"][:, 1]).max() > 0.1
+    assert err < 1e-10 and rel_err(drho[off:off + 300], ref["yp"]) < 1e-9
+
+
+def test_graph_replays_match_kernel_by_kernel(ref_weights):
+    """The captured step graph, replayed in chunks with eager state reads and
+    synchronisations between, equals launching every kernel of every step."""
+    from quest_amd import Context, systems
+    consts = systems.consts()
+    sys_list, pulses = _systems()
+    with Context(sys_list, consts, pulse=pulses) as g, Context(sys_list, consts, pulse=pulses) as e:
+        g.set_pc_weights(*ref_weights)
+        e.set_pc_weights(*ref_weights)
+        e.set_timing(True)                      # timing mode launches kernel by kernel
+        for chunk in range(6):
+            g.run(25)
+            e.run(25)
+            assert g.info()["graph"] == 1 and e.info()["graph"] == 0
+            rg, dg = g.state()
+            re_, de = e.state()
+            assert np.array_equal(rg, re_) and np.array_equal(dg, de), f"chunk {chunk}"
+            g.sync()
+
+
+def test_autotune_changes_no_state(ref_weights):
+    from quest_amd import Context, systems
+    consts = systems.consts()
+    sys_list, pulses = _systems()
+    with Context(sys_list, consts, pulse=pulses) as a, Context(sys_list, consts, pulse=pulses) as b:
+        a.set_pc_weights(*ref_weights)
+        b.set_pc_weights(*ref_weights)
+        a.run(120)
+        b.run(120)
+        chunk = a.autotune(2)
+        assert chunk == a.info()["source_chunk"] and chunk >= 32
+        a.run(120)
+        b.run(120)
+        ra, _ = a.state()
+        rb, _ = b.state()
+    assert rel_err(ra, rb) < 1e-12
+
+
+def test_ensemble_generator_and_batch_smoke(ref_weights):
+    """The 256-system ensemble of the batched workload builds and steps."""
+    from quest_amd import Context, systems
+    ens = systems.ensemble(count=256)
+    ns = np.array([len(s["pos"]) for s, _ in ens])
+    assert ns.min() >= 200 and ns.max() <= 2000
+    with Context([s for s, _ in ens], systems.consts(), precision=1, pulse=[p for _, p in ens]) as ctx:
+        ctx.set_pc_weights(*ref_weights)
+        ctx.run(5)
+        rho, _ = ctx.state()
+        assert np.all(np.isfinite(rho))
+        assert ctx.info()["systems"] == 256

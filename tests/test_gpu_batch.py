@@ -1,8 +1,9 @@
 """GPU: batches of independent systems in one context (quest_batch_create),
 the captured step graph, and the autotuner.
 
-Tolerances: batch vs one-context-per-system 1e-12 relative (the work lists
-differ, so summation order differs); graph replay vs kernel-by-kernel launch
+Tolerances: batch vs one-context-per-system 1e-12 relative in FP64 and 1e-8
+in mixed precision (the work lists differ, so the FP32 partial sums round
+differently); graph replay vs kernel-by-kernel launch
 bitwise; a batch member vs the FP64 oracle 1e-10 relative.
 """
 import math
@@ -44,7 +45,7 @@ def test_batch_matches_individual_contexts(ref_weights, precision):
             rho, _ = one.state()
         err = rel_err(batch_rho[off:off + n], rho)
         print(f"system of {n}: batch vs single {err:.2e}")
-        assert err < 1e-12
+        assert err < (1e-12 if precision == 0 else 1e-8)
         off += n
 
 

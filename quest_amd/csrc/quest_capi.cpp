@@ -424,7 +424,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   Consts &k = c->k;
   k.n = n;
   k.order = p->interp_order;
-  k.kstride = p->interp_order + 1;  // slot_stride<M>()
+  k.kstride = p->interp_order + 1;  // coefficients per ring entry
   k.ring_slots = next_pow2(c->dmax_bound + 1);
   k.ring_mask = k.ring_slots - 1;
   k.tiles = (n + kBlock - 1) / kBlock;
@@ -445,23 +445,40 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   k.m_cdt = -p->c0 / p->dt;
   k.m_o2dt = -2 * p->laser_omega / p->dt;
   k.two_idt2 = 2 / (p->dt * p->dt);
+  k.cc_l = k.cc / (cdt * cdt * cdt);
+  k.oo_l = k.oo / cdt;
+  k.oc_l = k.oc / (cdt * cdt);
+  k.m_cdt_l = k.m_cdt / (cdt * cdt);
+  k.m_o2dt_l = k.m_o2dt / cdt;
+  k.two_idt2_l = k.two_idt2 / cdt;
+  {
+    const double t = k.theta, t2 = t * t;
+    k.sin_c[0] = t;
+    k.sin_c[1] = -t * t2 / 6.0;
+    k.sin_c[2] = t * t2 * t2 / 120.0;
+    k.cos_c[0] = -t2 / 2.0;
+    k.cos_c[1] = t2 * t2 / 24.0;
+    k.cos_c[2] = -t2 * t2 * t2 / 720.0;
+  }
   lagrange_monomials(p->interp_order, k.vinv);
   k.dyadic = p->dyadic;
   k.n_systems = n_sys;
-  // FP32 positions are stored relative to the centre of all systems' boxes;
-  // each coordinate is off by at most half an ulp of the largest centred
-  // coordinate, so a pair distance by at most sqrt(3) ulp.  The arithmetic
-  // (squares, sum, rsqrt, two products, the rounded 1/(c0 dt)) adds at most
-  // ~3.5e-7 relative.
+  // FP32 positions are stored relative to the centre of all systems' boxes,
+  // in units of c0 dt (so a pair's FP32 distance is its delay tau); each
+  // coordinate is off by at most half an ulp of the largest scaled centred
+  // coordinate, so a pair's tau by at most sqrt(3) ulp.  The arithmetic
+  // (squares, sum, rsqrt, one product) adds at most ~3e-7 relative.  eps_rel
+  // and eps_abs carry a factor two of margin.
   double ctr[3] = {0, 0, 0}, half = 0;
   if(diag > 0 || n > 0)
     for(int a = 0; a < 3; ++a) {
       ctr[a] = 0.5 * (lo[a] + hi[a]);
       half = std::max(half, 0.5 * (hi[a] - lo[a]));
     }
-  const double ulp = half > 0 ? std::ldexp(1.0, std::ilogb(half) - 23) : 0x1p-126;
+  const double half_l = half / cdt;
+  const double ulp = half_l > 0 ? std::ldexp(1.0, std::ilogb(half_l) - 23) : 0x1p-126;
   k.eps_rel = 8e-7;
-  k.eps_abs = 2.0 * std::sqrt(3.0) * ulp / cdt + 2e-6;
+  k.eps_abs = 2.0 * std::sqrt(3.0) * ulp + 2e-6;
 
   // sorted host copies
   std::vector<double4> p64(n), d64(n);
@@ -477,8 +494,8 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
     const int i = c->perm[s];
     p64[s] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
     d64[s] = make_double4(dip[3 * i], dip[3 * i + 1], dip[3 * i + 2], 0.0);
-    p32[s] = make_float4(static_cast<float>(p64[s].x - ctr[0]), static_cast<float>(p64[s].y - ctr[1]),
-                         static_cast<float>(p64[s].z - ctr[2]), 0.0f);
+    p32[s] = make_float4(static_cast<float>((p64[s].x - ctr[0]) / cdt), static_cast<float>((p64[s].y - ctr[1]) / cdt),
+                         static_cast<float>((p64[s].z - ctr[2]) / cdt), 0.0f);
     d32[s] = make_float4(static_cast<float>(d64[s].x), static_cast<float>(d64[s].y),
                          static_cast<float>(d64[s].z), 0.0f);
     if(freq) f[s] = freq[i];

@@ -36,6 +36,12 @@ struct Consts {
   double hbar, laser_omega;
   // combination constants of the pair term, see pair_term()
   double cc, oo, oc, m_cdt, m_o2dt, two_idt2;
+  // the same for the FP32 pair sum, whose positions are in units of L = c0 dt:
+  // cc / L^3, oo / L, oc / L^2, m_cdt / L^2, m_o2dt / L, two_idt2 / L
+  double cc_l, oo_l, oc_l, m_cdt_l, m_o2dt_l, two_idt2_l;
+  // FP32 sin/cos of theta x: theta, -theta^3/6, theta^5/120 and
+  // -theta^2/2, theta^4/24, -theta^6/720
+  double sin_c[3], cos_c[3];
   // error bound of the FP32 delay estimate: eps = eps_rel tau + eps_abs
   double eps_rel, eps_abs;
   // monomial coefficients of the Lagrange basis: vinv[k][m] = [x^k] L_m(x)
@@ -55,9 +61,9 @@ struct PulseParams {
 // Device buffers of one context (sorted dot order).
 struct DeviceState {
   double4 *pos64 = nullptr, *dip64 = nullptr;   // (x, y, z, 0)
-  float4 *pos32 = nullptr, *dip32 = nullptr;      // pos32 relative to the box centre
+  float4 *pos32 = nullptr, *dip32 = nullptr;      // pos32: (pos - centre) / (c0 dt)
   double *freq = nullptr, *t1 = nullptr, *t2 = nullptr;
-  void *ring = nullptr;                // R2 [n][ring_slots][kstride]
+  void *ring = nullptr;                // coefficient planes, kstride x [n][ring_slots] complex
   double2 *y0 = nullptr, *y1 = nullptr;  // state ring [state_slots][n]: rho00, rho01
   double2 *d0 = nullptr, *d1 = nullptr;  // derivative ring
   double2 *cur0 = nullptr, *cur1 = nullptr;  // iterate double buffer [2][n]

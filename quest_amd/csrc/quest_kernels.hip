@@ -2,17 +2,17 @@
 //
 // Data layout in HBM (all in the library's Morton-sorted dot order):
 //   pos/dip        (x, y, z, 0) per dot, FP64 and FP32 copies
-//   ring           R2 [n][ring_slots][M + 1]: for every source dot and every
-//                  finished step s (slot s & ring_mask) the monomial
-//                  coefficients c_k of its Lagrange interpolant of rho01
-//                  through steps s, s-1, ..., s-M, multiplied by e^{i theta s}
-//                  (rotating frame), in FP64 or FP32
+//   ring           for every source dot and every finished step s (slot
+//                  s & ring_mask) the monomial coefficients c_k of its
+//                  Lagrange interpolant of rho01 through steps s, s-1, ...,
+//                  s-M, multiplied by e^{i theta s} (rotating frame), in FP64
+//                  or FP32, in coefficient planes (see ring_index)
 //   state ring     double2 [state_slots][n] rho00, rho01 and derivatives, for
 //                  the predictor-corrector window
 //
 // The pair sum is a gather: one thread per target dot, sources walked in
 // order.  Source data are wave-uniform (scalar loads); the ring read is a
-// per-lane gather of kstride complex values.  No MFMA: the work is a
+// per-lane gather of M + 1 complex values.  No MFMA: the work is a
 // gather-and-accumulate with ~90 VALU operations per ordered pair.
 #include <hip/hip_runtime.h>
 
@@ -41,17 +41,16 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b)
   return make_double2(a.x + b.x, a.y + b.y);
 }
 
-// FP32 delay split of a pair.  Written with explicit fused operations and
-// contraction off so that every kernel that classifies a pair (the pair sum
-// and the zero-delay list builder) gets bit-identical results.
-__device__ __forceinline__ float tau_f32(float dx, float dy, float dz, float icdt,
-                                         float &r2, float &ir)
+// FP32 delay tau = r / (c0 dt) of a pair whose FP32 positions are stored in
+// units of c0 dt, and 1/r in those units.  Written with explicit fused
+// operations and contraction off so that every kernel that classifies a pair
+// (the pair sum and the zero-delay list builder) gets bit-identical results.
+__device__ __forceinline__ float tau_f32(float dx, float dy, float dz, float &r2, float &ir)
 {
 #pragma clang fp contract(off)
   r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
   ir = __builtin_amdgcn_rsqf(r2);
-  const float r = r2 * ir;
-  return r * icdt;
+  return r2 * ir;
 }
 
 // FP64 delay split, bit-identical to the reference's
@@ -94,21 +93,22 @@ __device__ __forceinline__ int split_f64(double dx, double dy, double dz, double
 
 // Delay split of the mixed-precision path: the FP32 estimate decides the
 // integer part unless its fraction lies within the estimate's error bound
-// eps = eps_rel tau + eps_abs (position rounding plus six roundings, with a
-// factor two of margin) of an integer; then the exact FP64 split decides.  The
-// integer delays are therefore the reference's in both precisions, and the
-// pair kernel and the zero-delay list builder classify every pair alike.
-__device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float icdt, float eps_rel,
-                                           float eps_abs, const double4 &pa64,
+// eps = eps_rel tau + eps_abs (position rounding plus five roundings, with a
+// factor two of margin) of an integer -- tested as |fr - 1/2| > 1/2 - eps --
+// and then the exact FP64 split decides.  The integer delays are therefore the
+// reference's in both precisions, and the pair kernel and the zero-delay list
+// builder classify every pair alike.  half_m_eps_abs = 1/2 - eps_abs.
+__device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float eps_rel,
+                                           float half_m_eps_abs, const double4 &pa64,
                                            const double4 *__restrict__ pos64, int b, double cdt,
                                            double icdt64, float &x, float &ir)
 {
   float r2;
-  const float tau = tau_f32(dx, dy, dz, icdt, r2, ir);
+  const float tau = tau_f32(dx, dy, dz, r2, ir);
   const float fl = __builtin_floorf(tau);
   const float fr = tau - fl;
-  const float eps = __builtin_fmaf(eps_rel, tau, eps_abs);
-  if(__builtin_expect(fr < eps || fr > 1.0f - eps, 0)) {
+  const float lim = __builtin_fmaf(-eps_rel, tau, half_m_eps_abs);
+  if(__builtin_expect(__builtin_fabsf(fr - 0.5f) > lim, 0)) {
     const double4 pb = pos64[b];
     double xd, ird, r2d;
     const int n = split_f64(pa64.x - pb.x, pa64.y - pb.y, pa64.z - pb.z, cdt, icdt64, xd, ird, r2d);
@@ -120,14 +120,19 @@ __device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float i
   return static_cast<int>(__builtin_fmaxf(fl, 0.0f));
 }
 
-// sin and cos of u in [0, 0.23): Taylor series (Horner in u^2), truncated
-// where the next term is below the format's rounding error.
-__device__ __forceinline__ void sincos_small(float u, float &s, float &c)
+// sin and cos of theta x for x in [0, 1), theta x < 0.23: Taylor series in
+// x^2 with theta folded into the coefficients (sn[] = theta, -theta^3/6,
+// theta^5/120; cs[] = -theta^2/2, theta^4/24, -theta^6/720), truncated where
+// the next term is below FP32 rounding.
+__device__ __forceinline__ void sincos_theta(const float *sn, const float *cs, float x, float &s,
+                                             float &c)
 {
-  const float u2 = u * u;
-  s = u * __builtin_fmaf(u2, __builtin_fmaf(u2, 1.0f / 120.0f, -1.0f / 6.0f), 1.0f);
-  c = __builtin_fmaf(u2, __builtin_fmaf(u2, __builtin_fmaf(u2, -1.0f / 720.0f, 1.0f / 24.0f), -0.5f), 1.0f);
+  const float x2 = x * x;
+  s = x * __builtin_fmaf(x2, __builtin_fmaf(x2, sn[2], sn[1]), sn[0]);
+  c = __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, cs[2], cs[1]), cs[0]), 1.0f);
 }
+// sin and cos of u in [0, 0.23): Taylor series (Horner in u^2), truncated
+// where the next term is below FP64 rounding.
 __device__ __forceinline__ void sincos_small(double u, double &s, double &c)
 {
   const double u2 = u * u;
@@ -150,10 +155,36 @@ __device__ __forceinline__ void sincos_small(double u, double &s, double &c)
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// Complex entries per ring slot: M + 1, unpadded -- the pair sum is bound by
-// how many cache lines the concurrently walked ring windows span, not by
-// load alignment.
-template <int M> constexpr int slot_stride() { return M + 1; }
+// Interpolant ring.  Entry e = a * ring_slots + slot holds the M + 1
+// coefficients of source a for one slot; E = n * ring_slots entries.
+//   FP64: one record per entry                    double2 [E][M + 1]
+//   FP32: (M + 1) / 2 planes of two coefficients  float4  [(M + 1) / 2][E],
+//         then, for even M, one plane of c_M      float2  [E]
+// A wave's lanes read one source at the ~30 consecutive slots its targets'
+// delays span.  In the FP32 planes each load instruction touches a short
+// contiguous run of 16-B aligned entries (measured 3.5% faster than 40-B
+// records at N = 65536); the FP64 planes were no faster than records and
+// need five more SGPRs, which costs residency.
+// Index of coefficient kk of entry e, in units of the complex type.
+__device__ __forceinline__ size_t ring_index(const double2 *, int order, size_t /*E*/, size_t e, int kk)
+{
+  return e * static_cast<size_t>(order + 1) + kk;
+}
+__device__ __forceinline__ size_t ring_index(const float2 *, int order, size_t E, size_t e, int kk)
+{
+  const int pairs = (order + 1) / 2;
+  if(kk < 2 * pairs) return (static_cast<size_t>(kk >> 1) * E + e) * 2 + (kk & 1);
+  return static_cast<size_t>(2 * pairs) * E + e;
+}
+
+// Load at a wave-uniform base plus a 32-bit per-lane byte offset (the
+// base + zero-extended offset form of the global loads: no per-lane 64-bit
+// address arithmetic).
+template <typename T>
+__device__ __forceinline__ T load_off(const T *__restrict__ base, unsigned byte_off)
+{
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
 
 // Value q0 = p(x), slope q1 = p'(x) and half curvature q2 = p''(x)/2 of the
 // degree-M polynomial sum_k c_k x^k (simultaneous Horner).
@@ -192,14 +223,22 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
   }
 }
 
-// Pair constants converted to the pair arithmetic's type.
+// Pair constants converted to the pair arithmetic's type.  FP64 geometry is
+// in um; FP32 positions are stored in units of L = c0 dt, so there 1/r, g3
+// and g1 are L, L^2 and L times their um values and the constants that
+// multiply them carry the inverse factors (the *_l constants).
 template <typename R> struct PairK {
-  R icdt, cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, eps_abs;
+  static constexpr bool kF32 = sizeof(R) == sizeof(float);
+  R cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, half_m_eps_abs;
+  R sn[3], cs[3];  // sincos_theta coefficients (FP32)
   double cdt, icdt64;
   __host__ __device__ explicit PairK(const Consts &k)
-      : icdt((R)k.icdt), cc((R)k.cc), oo((R)k.oo), oc((R)k.oc), m_cdt((R)k.m_cdt),
-        m_o2dt((R)k.m_o2dt), two_idt2((R)k.two_idt2), theta((R)k.theta),
-        eps_rel((R)k.eps_rel), eps_abs((R)k.eps_abs), cdt(k.cdt), icdt64(k.icdt) {}
+      : cc((R)(kF32 ? k.cc_l : k.cc)), oo((R)(kF32 ? k.oo_l : k.oo)), oc((R)(kF32 ? k.oc_l : k.oc)),
+        m_cdt((R)(kF32 ? k.m_cdt_l : k.m_cdt)), m_o2dt((R)(kF32 ? k.m_o2dt_l : k.m_o2dt)),
+        two_idt2((R)(kF32 ? k.two_idt2_l : k.two_idt2)), theta((R)k.theta), eps_rel((R)k.eps_rel),
+        half_m_eps_abs((R)(0.5 - k.eps_abs)),
+        sn{(R)k.sin_c[0], (R)k.sin_c[1], (R)k.sin_c[2]}, cs{(R)k.cos_c[0], (R)k.cos_c[1], (R)k.cos_c[2]},
+        cdt(k.cdt), icdt64(k.icdt) {}
 };
 
 // The field at a target from one source, given the geometry and the source's
@@ -236,7 +275,10 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
     const R cr = a0 * q0r - b0 * q0i + a1 * q1r - b1 * q1i + a2 * q2r;
     const R ci = a0 * q0i + b0 * q0r + a1 * q1i + b1 * q1r + a2 * q2i;
     R sn, cs;
-    sincos_small(K.theta * x, sn, cs);
+    if constexpr(sizeof(R) == sizeof(float))
+      sincos_theta(K.sn, K.cs, x, sn, cs);
+    else
+      sincos_small(K.theta * x, sn, cs);
     outr = cr * cs + ci * sn;
     outi = ci * cs - cr * sn;
   }
@@ -254,17 +296,19 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
 // ordered tile-major with the source chunks of a tile consecutive, and `grid`
 // is a multiple of 8, so with 8 chunks per tile each XCD (round-robin block
 // dispatch) walks one source range and its L2 holds that range's ring rows.
-constexpr int kChunk = 64;   // sources summed in the pair type before the FP64 fold
+constexpr int kChunk = 64;
+#ifndef QUEST_F64_ATTR
+#define QUEST_F64_ATTR
+#endif   // sources summed in the pair type before the FP64 fold
 
 template <int M, bool LAB>
-__global__ __launch_bounds__(kBlock) void pair_past_f64(
+__global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
     const double4 *__restrict__ pos, const double4 *__restrict__ dip,
     const double2 *__restrict__ ring, const int4 *__restrict__ work, const int *__restrict__ work_row,
     const int *__restrict__ step_ptr, double2 *__restrict__ partial, Consts k)
 {
   const PairK<double> K(k);
   const int step = *step_ptr;
-  const size_t src_stride = static_cast<size_t>(k.ring_slots) * slot_stride<M>();
   for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x);
@@ -276,12 +320,13 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
       const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
       double x, ir, r2;
       const int n = split_f64(dx, dy, dz, K.cdt, K.icdt64, x, ir, r2);
-      const bool valid = (b != a) && (n >= 1);
+      const bool valid = n >= 1;  // the self pair has r = 0, hence n = 0
       const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
-      const double2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
+      const double2 *c = ring + static_cast<size_t>(b) * k.ring_slots * (M + 1);
+      const unsigned off = slot * static_cast<unsigned>((M + 1) * sizeof(double2));
       double2 cv[M + 1];
 #pragma unroll
-      for(int i = 0; i <= M; ++i) cv[i] = c[i];
+      for(int i = 0; i <= M; ++i) cv[i] = load_off(c + i, off);
       double q0r, q0i, q1r, q1i, q2r, q2i;
       horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
       double tr, ti;
@@ -308,7 +353,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
 {
   const PairK<float> K(k);
   const int step = *step_ptr;
-  const size_t src_stride = static_cast<size_t>(k.ring_slots) * slot_stride<M>();
+  const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
+  constexpr int kPairs = (M + 1) / 2;
+  const float4 *ring2 = reinterpret_cast<const float4 *>(ring);
+  const float2 *ring1 = ring + 2 * kPairs * E;
   for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x);
@@ -323,14 +371,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
         const float4 pb = pos[b], db = dip[b];
         const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float x, ir;
-        const int n = split_mixed(dx, dy, dz, K.icdt, K.eps_rel, K.eps_abs, pa64, pos64, b, K.cdt,
+        const int n = split_mixed(dx, dy, dz, K.eps_rel, K.half_m_eps_abs, pa64, pos64, b, K.cdt,
                                   K.icdt64, x, ir);
-        const bool valid = (b != a) && (n >= 1);
+        const bool valid = n >= 1;  // the self pair has r = 0, hence n = 0
         const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
-        const float2 *c = ring + static_cast<size_t>(b) * src_stride + slot * slot_stride<M>();
+        const size_t row = static_cast<size_t>(b) * k.ring_slots;
         float2 cv[M + 1];
 #pragma unroll
-        for(int i = 0; i <= M; ++i) cv[i] = c[i];
+        for(int i = 0; i < kPairs; ++i) {
+          const float4 v = load_off(ring2 + i * E + row, slot * static_cast<unsigned>(sizeof(float4)));
+          cv[2 * i] = make_float2(v.x, v.y);
+          cv[2 * i + 1] = make_float2(v.z, v.w);
+        }
+        if constexpr(2 * kPairs == M) cv[M] = load_off(ring1 + row, slot * static_cast<unsigned>(sizeof(float2)));
         float q0r, q0i, q1r, q1i, q2r, q2i;
         horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
         float tr, ti;
@@ -445,7 +498,8 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
   if(lab)
     for(int m = 0; m <= k.order; ++m) smp[m] = make_double2(2.0 * smp[m].x, 0.0);
   const double2 ph = lab ? make_double2(1.0, 0.0) : phase[s];
-  R2 *dst = ring + (static_cast<size_t>(a) * k.ring_slots + (s & k.ring_mask)) * k.kstride;
+  const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
+  const size_t e = static_cast<size_t>(a) * k.ring_slots + (s & k.ring_mask);
   for(int kk = 0; kk <= k.order; ++kk) {
     double2 c = make_double2(0.0, 0.0);
     for(int m = 0; m <= k.order; ++m) {
@@ -454,8 +508,9 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
       c.y += v * smp[m].y;
     }
     c = cmul(c, ph);
-    dst[kk].x = c.x;
-    dst[kk].y = c.y;
+    R2 &dst = ring[ring_index(ring, k.order, E, e, kk)];
+    dst.x = c.x;
+    dst.y = c.y;
   }
 }
 
@@ -628,8 +683,8 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
     for(int b = range.x; b < range.y; ++b) {
       const float4 pb = d.pos32[b];
       float x, ir;
-      const int n = split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.icdt, K.eps_rel,
-                                K.eps_abs, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
+      const int n = split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.eps_rel,
+                                K.half_m_eps_abs, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
       if(b != a && n < 1) {
         if(FILL) d.near_src[base + cnt] = b;
         ++cnt;

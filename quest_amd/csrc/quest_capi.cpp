@@ -445,12 +445,12 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   k.m_cdt = -p->c0 / p->dt;
   k.m_o2dt = -2 * p->laser_omega / p->dt;
   k.two_idt2 = 2 / (p->dt * p->dt);
-  k.cc_l = k.cc / (cdt * cdt * cdt);
-  k.oo_l = k.oo / cdt;
-  k.oc_l = k.oc / (cdt * cdt);
-  k.m_cdt_l = k.m_cdt / (cdt * cdt);
-  k.m_o2dt_l = k.m_o2dt / cdt;
-  k.two_idt2_l = k.two_idt2 / cdt;
+  k.f32_scale = k.two_idt2 / cdt;
+  k.cc_l = k.cc / (cdt * cdt * cdt) / k.f32_scale;
+  k.oo_l = k.oo / cdt / k.f32_scale;
+  k.oc_l = k.oc / (cdt * cdt) / k.f32_scale;
+  k.m_cdt_l = k.m_cdt / (cdt * cdt) / k.f32_scale;
+  k.m_o2dt_l = k.m_o2dt / cdt / k.f32_scale;
   {
     const double t = k.theta, t2 = t * t;
     k.sin_c[0] = t;

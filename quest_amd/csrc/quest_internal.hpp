@@ -36,9 +36,10 @@ struct Consts {
   double hbar, laser_omega;
   // combination constants of the pair term, see pair_term()
   double cc, oo, oc, m_cdt, m_o2dt, two_idt2;
-  // the same for the FP32 pair sum, whose positions are in units of L = c0 dt:
-  // cc / L^3, oo / L, oc / L^2, m_cdt / L^2, m_o2dt / L, two_idt2 / L
-  double cc_l, oo_l, oc_l, m_cdt_l, m_o2dt_l, two_idt2_l;
+  // the same for the FP32 pair sum, whose positions are in units of L = c0 dt,
+  // divided by f32_scale = two_idt2 / L (which the FP32 kernel applies once
+  // per target, in FP64): cc / L^3, oo / L, oc / L^2, m_cdt / L^2, m_o2dt / L
+  double cc_l, oo_l, oc_l, m_cdt_l, m_o2dt_l, f32_scale;
   // FP32 sin/cos of theta x: theta, -theta^3/6, theta^5/120 and
   // -theta^2/2, theta^4/24, -theta^6/720
   double sin_c[3], cos_c[3];

@@ -45,10 +45,12 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b)
 // units of c0 dt, and 1/r in those units.  Written with explicit fused
 // operations and contraction off so that every kernel that classifies a pair
 // (the pair sum and the zero-delay list builder) gets bit-identical results.
+// r2 is clamped to 1e-30 (physical pairs are ~0.01 and above in these units)
+// so that tau is finite for coincident dots too.
 __device__ __forceinline__ float tau_f32(float dx, float dy, float dz, float &r2, float &ir)
 {
 #pragma clang fp contract(off)
-  r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+  r2 = __builtin_fmaxf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), 1e-30f);
   ir = __builtin_amdgcn_rsqf(r2);
   return r2 * ir;
 }
@@ -98,7 +100,8 @@ __device__ __forceinline__ int split_f64(double dx, double dy, double dz, double
 // and then the exact FP64 split decides.  The integer delays are therefore the
 // reference's in both precisions, and the pair kernel and the zero-delay list
 // builder classify every pair alike.  half_m_eps_abs = 1/2 - eps_abs.
-__device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float eps_rel,
+// Returns n as an integer-valued float.
+__device__ __forceinline__ float split_mixed(float dx, float dy, float dz, float eps_rel,
                                            float half_m_eps_abs, const double4 &pa64,
                                            const double4 *__restrict__ pos64, int b, double cdt,
                                            double icdt64, float &x, float &ir)
@@ -112,12 +115,11 @@ __device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float e
     const double4 pb = pos64[b];
     double xd, ird, r2d;
     const int n = split_f64(pa64.x - pb.x, pa64.y - pb.y, pa64.z - pb.z, cdt, icdt64, xd, ird, r2d);
-    x = static_cast<float>(xd);
-    return n;
+    x = xd == xd ? static_cast<float>(xd) : 0.0f;  // coincident dots: n = 0, finite x
+    return static_cast<float>(n);
   }
   x = fr;
-  // NaN (coincident dots) -> 0; keeps the conversion defined
-  return static_cast<int>(__builtin_fmaxf(fl, 0.0f));
+  return fl;  // tau is finite and >= 0
 }
 
 // sin and cos of theta x for x in [0, 1), theta x < 0.23: Taylor series in
@@ -226,7 +228,8 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
 // Pair constants converted to the pair arithmetic's type.  FP64 geometry is
 // in um; FP32 positions are stored in units of L = c0 dt, so there 1/r, g3
 // and g1 are L, L^2 and L times their um values and the constants that
-// multiply them carry the inverse factors (the *_l constants).
+// multiply them carry the inverse factors (the *_l constants), all divided by
+// f32_scale = two_idt2 / L so that the g1 q2 term needs no multiply.
 template <typename R> struct PairK {
   static constexpr bool kF32 = sizeof(R) == sizeof(float);
   R cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, half_m_eps_abs;
@@ -235,7 +238,7 @@ template <typename R> struct PairK {
   __host__ __device__ explicit PairK(const Consts &k)
       : cc((R)(kF32 ? k.cc_l : k.cc)), oo((R)(kF32 ? k.oo_l : k.oo)), oc((R)(kF32 ? k.oc_l : k.oc)),
         m_cdt((R)(kF32 ? k.m_cdt_l : k.m_cdt)), m_o2dt((R)(kF32 ? k.m_o2dt_l : k.m_o2dt)),
-        two_idt2((R)(kF32 ? k.two_idt2_l : k.two_idt2)), theta((R)k.theta), eps_rel((R)k.eps_rel),
+        two_idt2((R)(kF32 ? 1.0 : k.two_idt2)), theta((R)k.theta), eps_rel((R)k.eps_rel),
         half_m_eps_abs((R)(0.5 - k.eps_abs)),
         sn{(R)k.sin_c[0], (R)k.sin_c[1], (R)k.sin_c[2]}, cs{(R)k.cos_c[0], (R)k.cos_c[1], (R)k.cos_c[2]},
         cdt(k.cdt), icdt64(k.icdt) {}
@@ -263,7 +266,7 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
   const R g3 = (aab - 3 * bq) * ir2;
   const R g1 = (aab - bq) * ir;
   const R a1 = K.m_cdt * g3;
-  const R a2 = K.two_idt2 * g1;
+  const R a2 = PairK<R>::kF32 ? g1 : K.two_idt2 * g1;  // FP32 constants are divided by it
   if constexpr(LAB) {
     const R a0 = K.cc * ir * g3;
     outr = a0 * q0r + a1 * q1r + a2 * q2r;
@@ -355,6 +358,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
   const int step = *step_ptr;
   const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
   constexpr int kPairs = (M + 1) / 2;
+  const unsigned mask8 = static_cast<unsigned>(k.ring_mask) << 3;
   const float4 *ring2 = reinterpret_cast<const float4 *>(ring);
   const float2 *ring1 = ring + 2 * kPairs * E;
   for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
@@ -371,30 +375,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
         const float4 pb = pos[b], db = dip[b];
         const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float x, ir;
-        const int n = split_mixed(dx, dy, dz, K.eps_rel, K.half_m_eps_abs, pa64, pos64, b, K.cdt,
-                                  K.icdt64, x, ir);
-        const bool valid = n >= 1;  // the self pair has r = 0, hence n = 0
-        const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
+        const float fn = split_mixed(dx, dy, dz, K.eps_rel, K.half_m_eps_abs, pa64, pos64, b, K.cdt,
+                                     K.icdt64, x, ir);
+        const int mn = static_cast<int>(-fn);  // -n
+        // n >= 1 pairs only (the self pair has r = 0, hence n = 0): 1/r = 0
+        // zeroes every coefficient of the term, all other factors being finite
+        ir = mn <= -1 ? ir : 0.0f;
+        // byte offsets of the slot (step - n) & mask in the 8-B and 16-B planes
+        const unsigned off8 = (static_cast<unsigned>(step + mn) << 3) & mask8;
+        const unsigned off16 = off8 << 1;
         const size_t row = static_cast<size_t>(b) * k.ring_slots;
         float2 cv[M + 1];
 #pragma unroll
         for(int i = 0; i < kPairs; ++i) {
-          const float4 v = load_off(ring2 + i * E + row, slot * static_cast<unsigned>(sizeof(float4)));
+          const float4 v = load_off(ring2 + i * E + row, off16);
           cv[2 * i] = make_float2(v.x, v.y);
           cv[2 * i + 1] = make_float2(v.z, v.w);
         }
-        if constexpr(2 * kPairs == M) cv[M] = load_off(ring1 + row, slot * static_cast<unsigned>(sizeof(float2)));
+        if constexpr(2 * kPairs == M) cv[M] = load_off(ring1 + row, off8);
         float q0r, q0i, q1r, q1i, q2r, q2i;
         horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
         float tr, ti;
         pair_term<float, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
-        sr += valid ? tr : 0.0f;
-        si += valid ? ti : 0.0f;
+        sr += tr;
+        si += ti;
       }
       accr += static_cast<double>(sr);
       acci += static_cast<double>(si);
     }
-    if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
+    if(tgt < wi.y)
+      partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.f32_scale, acci * k.f32_scale);
   }
 }
 
@@ -683,8 +693,8 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
     for(int b = range.x; b < range.y; ++b) {
       const float4 pb = d.pos32[b];
       float x, ir;
-      const int n = split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.eps_rel,
-                                K.half_m_eps_abs, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
+      const int n = static_cast<int>(split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.eps_rel,
+                                                 K.half_m_eps_abs, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir));
       if(b != a && n < 1) {
         if(FILL) d.near_src[base + cnt] = b;
         ++cnt;

@@ -130,6 +130,16 @@ int quest_set_pulse(quest_ctx *ctx, const quest_pulse *pulse);
 /* One pulse per system of a batch context (pulses: [n_systems]). */
 int quest_set_pulses(quest_ctx *ctx, const quest_pulse *pulses);
 
+/* Several pulses acting on one system (0 <= system < n_systems; 0 <=
+ * n_pulses <= QUEST_MAX_PULSES; n_pulses = 0 clears): the pulse field is the
+ * sum of the pulses' fields, added in list order, as the reference's
+ * integrator sums its list of interactions (integrator.cpp:173-179) when
+ * main.cpp's one PulseInteraction (main.cpp:43-46) is replaced by several;
+ * README.md:64 speaks of "pulse(s)" while read_pulse_config reads one
+ * (pulse.cpp:37-46). */
+#define QUEST_MAX_PULSES 8
+int quest_set_pulse_list(quest_ctx *ctx, int system, int n_pulses, const quest_pulse *pulses);
+
 /* Replaces PulseInteraction::evaluate(step) (pulse_interaction.cpp:9-20):
  * out[n] (complex, imaginary part 0). */
 int quest_pulse_evaluate(quest_ctx *ctx, int step, double *out);

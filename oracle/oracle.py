@@ -65,6 +65,10 @@ def lib():
                                C.POINTER(QoPulse), dp, dp, C.c_double, C.c_int, C.c_int,
                                C.c_int, dp, dp, dp, C.POINTER(C.c_int), dp]
         L.qo_solve.restype = C.c_int
+        L.qo_solve_pulses.argtypes = [C.POINTER(QoParams), C.c_int, C.c_int, dp, dp, dp, dp, dp,
+                                      C.POINTER(QoPulse), C.c_int, dp, dp, C.c_double, C.c_int,
+                                      C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int), dp]
+        L.qo_solve_pulses.restype = C.c_int
         L.qo_time_reference_steps.argtypes = [C.POINTER(QoParams), C.c_int, dp, dp, dp, dp,
                                               dp, C.POINTER(QoPulse), dp, dp, C.c_double,
                                               C.c_int, dp, dp]
@@ -176,9 +180,11 @@ def isfinite(sol) -> bool:
     return bool(lib().qo_isfinite(_dp(s.view(np.float64))))
 
 
-def solve(p: QoParams, system, pl: QoPulse, ps, cs, fc, n_steps: int, mode: int = 1,
-          record_every: int = 0, kind: int = 0):
+def solve(p: QoParams, system, pl, ps, cs, fc, n_steps: int, mode: int = 1,
+          record_every: int = 0, kind: int = 0, use_list: bool = False):
     """Run the whole PC integration.  system: dict with pos, dip, freq, t1, t2.
+    pl: one QoPulse, or a list of them (their fields add, in list order).
+    use_list routes a single pulse through qo_solve_pulses as a list of one.
 
     Returns dict(rec=[n_rec][n][2] complex or None, y=[n][2], yp=[n][2],
     status, bad_step, seconds).
@@ -193,12 +199,17 @@ def solve(p: QoParams, system, pl: QoPulse, ps, cs, fc, n_steps: int, mode: int 
     fyp = np.zeros((n, 2), dtype=np.complex128)
     bad = C.c_int(-1)
     secs = C.c_double(0)
-    status = lib().qo_solve(C.byref(p), kind, n, _dp(pos), _dp(dip), _dp(freq), _dp(t1),
-                            _dp(t2), C.byref(pl), _dp(ps), _dp(cs), float(fc), n_steps,
-                            mode, record_every,
-                            _dp(rec.view(np.float64)) if n_rec else None,
-                            _dp(fy.view(np.float64)), _dp(fyp.view(np.float64)),
-                            C.byref(bad), C.byref(secs))
+    tail = (_dp(ps), _dp(cs), float(fc), n_steps, mode, record_every,
+            _dp(rec.view(np.float64)) if n_rec else None,
+            _dp(fy.view(np.float64)), _dp(fyp.view(np.float64)), C.byref(bad), C.byref(secs))
+    if isinstance(pl, (list, tuple)) or use_list:
+        pls = list(pl) if isinstance(pl, (list, tuple)) else [pl]
+        arr = (QoPulse * max(len(pls), 1))(*pls)
+        status = lib().qo_solve_pulses(C.byref(p), kind, n, _dp(pos), _dp(dip), _dp(freq), _dp(t1),
+                                       _dp(t2), arr, len(pls), *tail)
+    else:
+        status = lib().qo_solve(C.byref(p), kind, n, _dp(pos), _dp(dip), _dp(freq), _dp(t1),
+                                _dp(t2), C.byref(pl), *tail)
     if status < 0:
         raise MemoryError("oracle allocation failed")
     return {"rec": rec[:n_rec] if n_rec else None, "y": fy, "yp": fyp,

@@ -23,6 +23,10 @@
  *   qo_isfinite           the reference's test/history_test.cpp:13-53
  *   qo_solve              N = 1 analytic pulse-area limit (rho00 =
  *                         cos^2(theta/2)), free decay, mode 0 == mode 1
+ *   qo_solve_pulses       one-pulse list == qo_solve bitwise; two equal
+ *                         pulses == one of twice the amplitude; N = 1 with
+ *                         pulse areas t1, t2: rho00 = cos^2((t1 + t2)/2);
+ *                         empty list keeps the ground state
  */
 #include "quest_oracle.h"
 
@@ -321,7 +325,8 @@ typedef struct {
   const qo_params *p;
   int kind, n, n_slots, n_steps;
   const double *pos, *dip, *freq, *t1, *t2;
-  const qo_pulse *pulse;
+  const qo_pulse *pulse;     /* [n_pulses] */
+  int n_pulses;
   const double *ps, *cs;
   double fc;
   double *y, *yp;            /* [n][n_slots][2] complex */
@@ -384,13 +389,23 @@ static void apply_rhs(qo_run *R, int step, const cplx *field)
   }
 }
 
+/* The pulse interactions of dot i at time t, accumulated from zero in list
+ * order (integrator.cpp:173-179 sums its interactions with std::accumulate;
+ * main.cpp:43-46 lists one pulse interaction, several add the same way). */
+static cplx pulses_at(const qo_run *R, int i, double t)
+{
+  cplx f = 0;
+  for(int q = 0; q < R->n_pulses; ++q)
+    f = f + pulse_one(R->pulse + q, R->p->hbar, R->pos + 3 * i, R->dip + 3 * i, t);
+  return f;
+}
+
 /* mode 0: Integrator::evaluator (integrator.cpp:171-185) with the pulse
  * interaction first and the history interaction second (main.cpp:43-46). */
 static void evaluator_reference(qo_run *R, int step, cplx *field)
 {
   const double t = step * R->p->dt;
-  for(int i = 0; i < R->n; ++i)
-    R->pulse_now[i] = pulse_one(R->pulse, R->p->hbar, R->pos + 3 * i, R->dip + 3 * i, t);
+  for(int i = 0; i < R->n; ++i) R->pulse_now[i] = pulses_at(R, i, t);
   qo_evaluate_history(R->p, R->kind, R->n, R->n_slots, R->delays, R->coef, R->y,
                       step, (double *)R->hist);
   for(int i = 0; i < R->n; ++i) field[i] = (0 + R->pulse_now[i]) + R->hist[i];
@@ -482,7 +497,8 @@ static int all_finite(qo_run *R, int step)
 static int run_alloc(qo_run *R, const qo_params *p, int kind, int n,
                      const double *pos, const double *dip, const double *freq,
                      const double *t1, const double *t2, const qo_pulse *pulse,
-                     const double *ps, const double *cs, double fc, int n_steps)
+                     int n_pulses, const double *ps, const double *cs, double fc,
+                     int n_steps)
 {
   memset(R, 0, sizeof(*R));
   R->p = p;
@@ -496,6 +512,7 @@ static int run_alloc(qo_run *R, const qo_params *p, int kind, int n,
   R->t1 = t1;
   R->t2 = t2;
   R->pulse = pulse;
+  R->n_pulses = pulse ? n_pulses : 0;
   R->ps = ps;
   R->cs = cs;
   R->fc = fc;
@@ -536,8 +553,21 @@ int qo_solve(const qo_params *p, int kind, int n, const double *pos,
              int record_every, double *rec, double *final_y, double *final_yp,
              int *bad_step, double *seconds)
 {
+  return qo_solve_pulses(p, kind, n, pos, dip, freq, t1, t2, pulse, pulse ? 1 : 0,
+                         ps, cs, future_coef, n_steps, mode, record_every, rec,
+                         final_y, final_yp, bad_step, seconds);
+}
+
+int qo_solve_pulses(const qo_params *p, int kind, int n, const double *pos,
+                    const double *dip, const double *freq, const double *t1,
+                    const double *t2, const qo_pulse *pulses, int n_pulses,
+                    const double *ps, const double *cs, double future_coef,
+                    int n_steps, int mode, int record_every, double *rec,
+                    double *final_y, double *final_yp, int *bad_step,
+                    double *seconds)
+{
   qo_run R;
-  if(run_alloc(&R, p, kind, n, pos, dip, freq, t1, t2, pulse, ps, cs,
+  if(run_alloc(&R, p, kind, n, pos, dip, freq, t1, t2, pulses, n_pulses, ps, cs,
                future_coef, n_steps) != 0) {
     run_free(&R);
     return -1;
@@ -554,8 +584,7 @@ int qo_solve(const qo_params *p, int kind, int n, const double *pos,
     if(mode == 1) {
       past_sums(&R, step);
       const double t = step * p->dt;
-      for(int i = 0; i < n; ++i)
-        R.pulse_now[i] = pulse_one(pulse, p->hbar, pos + 3 * i, dip + 3 * i, t);
+      for(int i = 0; i < n; ++i) R.pulse_now[i] = pulses_at(&R, i, t);
     }
     predictor(&R, step);
     if(mode == 0) evaluator_reference(&R, step, field);
@@ -608,8 +637,8 @@ double qo_time_reference_steps(const qo_params *p, int n, const double *pos,
                                double *step_seconds)
 {
   qo_run R;
-  if(run_alloc(&R, p, 0, n, pos, dip, freq, t1, t2, pulse, ps, cs,
-               future_coef, n_steps) != 0) {
+  if(run_alloc(&R, p, 0, n, pos, dip, freq, t1, t2, pulse, pulse ? 1 : 0, ps,
+               cs, future_coef, n_steps) != 0) {
     run_free(&R);
     return NAN;
   }

@@ -108,6 +108,16 @@ int qo_solve(const qo_params *p, int kind, int n, const double *pos,
              int record_every, double *rec, double *final_y, double *final_yp,
              int *bad_step, double *seconds);
 
+/* qo_solve with a list of pulse interactions (n_pulses >= 0), summed in list
+ * order ahead of the history interaction (integrator.cpp:173-179). */
+int qo_solve_pulses(const qo_params *p, int kind, int n, const double *pos,
+                    const double *dip, const double *freq, const double *t1,
+                    const double *t2, const qo_pulse *pulses, int n_pulses,
+                    const double *ps, const double *cs, double future_coef,
+                    int n_steps, int mode, int record_every, double *rec,
+                    double *final_y, double *final_yp, int *bad_step,
+                    double *seconds);
+
 /* The reference method's step loop only, for timing: builds the table (its
  * time goes to *setup_seconds), then runs n_steps of predictor + 11 scatter
  * sums on a ground-state system with the pulse on.  Returns a checksum. */

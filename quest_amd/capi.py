@@ -39,7 +39,9 @@ EXPORTS = [
     "quest_run", "quest_current_step", "quest_bad_step", "quest_get_state",
     "quest_pair_delays", "quest_set_timing", "quest_pair_kernel_time", "quest_ctx_info",
     "quest_sync", "quest_batch_create", "quest_set_pulses", "quest_autotune",
+    "quest_set_pulse_list",
 ]
+MAX_PULSES = 8  # QUEST_MAX_PULSES
 
 
 class QuestError(RuntimeError):
@@ -103,6 +105,7 @@ def load():
                                      dp, C.POINTER(vp)]
     L.quest_set_pulses.argtypes = [vp, C.POINTER(Pulse)]
     L.quest_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.quest_set_pulse_list.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Pulse)]
     for name in EXPORTS:
         if name not in ("quest_ctx_destroy", "quest_last_error"):
             getattr(L, name).restype = C.c_int
@@ -214,6 +217,12 @@ class Context:
         """One pulse per system of a batch."""
         arr = (Pulse * len(pulses))(*[_pulse_struct(pl) for pl in pulses])
         self._check(self._L.quest_set_pulses(self.ptr, arr))
+
+    def set_pulse_list(self, pulses, system: int = 0):
+        """Several pulses on one system (their fields add, in list order); an
+        empty list clears the system's pulses."""
+        arr = (Pulse * max(len(pulses), 1))(*[_pulse_struct(pl) for pl in pulses])
+        self._check(self._L.quest_set_pulse_list(self.ptr, int(system), len(pulses), arr))
 
     def autotune(self, launches: int = 3) -> int:
         """quest_autotune: the fastest source-chunk length (kept)."""

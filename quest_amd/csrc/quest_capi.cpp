@@ -276,7 +276,7 @@ int ensure_pc(quest_ctx *c)
 
 int upload_pulses(quest_ctx *c)
 {
-  HIPC(c, hipMemcpyAsync(c->d.pulses, c->pulses.data(), sizeof(PulseParams) * c->n_sys,
+  HIPC(c, hipMemcpyAsync(c->d.pulses, c->pulses.data(), sizeof(PulseParams) * c->pulses.size(),
                          hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   return QUEST_OK;
@@ -540,7 +540,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_of), sizeof(int) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_range), sizeof(int2) * n_sys));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_chunks), sizeof(int) * n_sys));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.pulses), sizeof(PulseParams) * n_sys));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.pulses), sizeof(PulseParams) * n_sys * QUEST_MAX_PULSES));
   HIPC(c, hipHostMalloc(reinterpret_cast<void **>(&c->h_stage), sizeof(double2) * nn * 2));
 
   HIPC(c, hipMemcpyAsync(d.pos64, p64.data(), sizeof(double4) * nn, hipMemcpyHostToDevice, c->stream));
@@ -553,7 +553,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, hipMemcpyAsync(d.perm_dev, c->inv.data(), sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.sys_of, sys_of.data(), sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.sys_range, ranges.data(), sizeof(int2) * n_sys, hipMemcpyHostToDevice, c->stream));
-  c->pulses.assign(n_sys, PulseParams{});
+  c->pulses.assign(static_cast<size_t>(n_sys) * QUEST_MAX_PULSES, PulseParams{});
   int rc = upload_pulses(c);
   if(rc != QUEST_OK) return rc;
 
@@ -617,6 +617,13 @@ PulseParams to_params(const quest_pulse *pl)
     q.pol[a] = pl->polarization[a];
   }
   return q;
+}
+
+// Pulse list of system s on the host copy: the given pulses first, the rest off.
+void put_pulses(quest_ctx *c, int s, int n, const quest_pulse *pl)
+{
+  PulseParams *row = c->pulses.data() + static_cast<size_t>(s) * QUEST_MAX_PULSES;
+  for(int p = 0; p < QUEST_MAX_PULSES; ++p) row[p] = p < n ? to_params(pl + p) : PulseParams{};
 }
 
 int create_checked(const quest_params *params, int n_sys, const int *sizes, const double *pos,
@@ -730,7 +737,7 @@ int quest_set_pulse(quest_ctx *c, const quest_pulse *pl)
 {
   if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
   HIPC(c, hipSetDevice(c->P.device));
-  c->pulses.assign(c->n_sys, to_params(pl));
+  for(int s = 0; s < c->n_sys; ++s) put_pulses(c, s, pl ? 1 : 0, pl);
   return upload_pulses(c);
 }
 
@@ -739,7 +746,20 @@ int quest_set_pulses(quest_ctx *c, const quest_pulse *pulses)
   if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
   if(!pulses) return fail(c, QUEST_ERR_ARG, "pulses is required (one per system)");
   HIPC(c, hipSetDevice(c->P.device));
-  for(int s = 0; s < c->n_sys; ++s) c->pulses[s] = to_params(pulses + s);
+  for(int s = 0; s < c->n_sys; ++s) put_pulses(c, s, 1, pulses + s);
+  return upload_pulses(c);
+}
+
+int quest_set_pulse_list(quest_ctx *c, int system, int n_pulses, const quest_pulse *pulses)
+{
+  if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
+  if(system < 0 || system >= c->n_sys)
+    return fail(c, QUEST_ERR_ARG, "system must be in [0, " + std::to_string(c->n_sys) + ")");
+  if(n_pulses < 0 || n_pulses > QUEST_MAX_PULSES)
+    return fail(c, QUEST_ERR_ARG, "n_pulses must be in [0, " + std::to_string(QUEST_MAX_PULSES) + "]");
+  if(n_pulses > 0 && !pulses) return fail(c, QUEST_ERR_ARG, "pulses is required when n_pulses > 0");
+  HIPC(c, hipSetDevice(c->P.device));
+  put_pulses(c, system, n_pulses, pulses);
   return upload_pulses(c);
 }
 

@@ -85,7 +85,7 @@ struct DeviceState {
   int *sys_of = nullptr;               // [n] system of each dot
   int2 *sys_range = nullptr;           // [systems] dot range [begin, end)
   int *sys_chunks = nullptr;           // [systems] source chunks (partial rows used)
-  PulseParams *pulses = nullptr;       // [systems]
+  PulseParams *pulses = nullptr;       // [systems][QUEST_MAX_PULSES], used ones first
   int4 *work = nullptr;                // [n_work] (target begin, target end, source begin, source end)
   int *work_row = nullptr;             // [n_work] partial row written by the item
   int *step = nullptr;                 // the step being integrated (device counter; graph replays read it)

@@ -432,7 +432,6 @@ __device__ __forceinline__ double2 reduce_partials(const Consts &k, const Device
 __device__ __forceinline__ double2 pulse_field(const PulseParams &pl, double hbar, double4 r,
                                                double4 d, double t)
 {
-  if(!pl.on) return make_double2(0.0, 0.0);
   const double kr = (pl.k[0] * r.x + pl.k[1] * r.y) + pl.k[2] * r.z;
   const double arg = kr - pl.freq * (t - pl.delay);
   const double u = arg / pl.width;
@@ -441,6 +440,17 @@ __device__ __forceinline__ double2 pulse_field(const PulseParams &pl, double hba
   const double e1 = (pl.amplitude / 2 * pl.pol[1]) * g;
   const double e2 = (pl.amplitude / 2 * pl.pol[2]) * g;
   return make_double2(((e0 * d.x + e1 * d.y) + e2 * d.z) / hbar, 0.0);
+}
+
+// The pulses of dot a's system, summed in list order from 0 (the reference
+// integrator accumulates its interactions from zero, integrator.cpp:173-179).
+__device__ __forceinline__ double2 pulse_sum(const Consts &k, const DeviceState &d, int a, double t)
+{
+  const PulseParams *pl = d.pulses + static_cast<size_t>(d.sys_of[a]) * QUEST_MAX_PULSES;
+  double2 f = make_double2(0.0, 0.0);
+  for(int p = 0; p < QUEST_MAX_PULSES && pl[p].on; ++p)
+    f = cadd(f, pulse_field(pl[p], k.hbar, d.pos64[a], d.dip64[a], t));
+  return f;
 }
 
 // QuantumDot::liouville_rhs (quantum_dot.cpp:10-19).
@@ -557,7 +567,7 @@ __global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState 
   const double dt = k.dt;
   double2 hist = make_double2(0.0, 0.0);
   if(k.n_work > 0) hist = cmul(reduce_partials(k, d, a), hist_scale(k, d.phase, step));
-  const double2 pulse = pulse_field(d.pulses[d.sys_of[a]], k.hbar, d.pos64[a], d.dip64[a], step * dt);
+  const double2 pulse = pulse_sum(k, d, a, step * dt);
   d.field[a] = cadd(pulse, hist);
 
   const double *ps0 = d.pcw, *ps1 = d.pcw + window;
@@ -656,7 +666,7 @@ __global__ __launch_bounds__(kBlock) void pulse_kernel(Consts k, DeviceState d)
 {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if(a >= k.n) return;
-  d.out[a] = pulse_field(d.pulses[d.sys_of[a]], k.hbar, d.pos64[a], d.dip64[a], *d.step * k.dt);
+  d.out[a] = pulse_sum(k, d, a, *d.step * k.dt);
 }
 
 __global__ void set_step_kernel(int *step, int value) { *step = value; }

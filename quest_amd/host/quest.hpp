@@ -30,6 +30,7 @@
 #include <iterator>
 #include <memory>
 #include <numeric>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -205,6 +206,24 @@ inline Pulse read_pulse_config(const std::string &fname)
   Pulse p;
   ifs >> p;
   return p;
+}
+
+// Every row of a pulse file, in order (an extension: the reference reads the
+// first row only, pulse.cpp:37-46, though README.md:64 speaks of "pulse(s)").
+// At most QUEST_MAX_PULSES rows feed the device path.
+inline std::vector<Pulse> read_pulse_configs(const std::string &fname)
+{
+  std::ifstream ifs(fname);
+  if(!ifs) throw std::runtime_error("Could not open " + fname);
+  std::vector<Pulse> out;
+  std::string line;
+  while(std::getline(ifs, line)) {
+    if(line.find_first_not_of(" \t\r") == std::string::npos) continue;
+    std::istringstream is(line);
+    Pulse p;
+    if(is >> p) out.push_back(p);
+  }
+  return out;
 }
 
 // ---------------------------------------------------------------- history.h
@@ -568,6 +587,20 @@ class DeviceIntegrator {
       const quest_pulse q = pulse->to_c();
       quest_detail::check(quest_set_pulse(ctx.get(), &q), ctx.get());
     }
+  }
+  // Several pulse interactions (their fields add, in list order, ahead of the
+  // history interaction -- the reference integrator's sum over interactions,
+  // integrator.cpp:173-179).
+  DeviceIntegrator(const std::shared_ptr<const DotVector> &dots,
+                   const std::vector<std::shared_ptr<const Pulse>> &pulses,
+                   const std::shared_ptr<GreenFunction::Dyadic> &dyadic, const int interp_order,
+                   const std::shared_ptr<History::HistoryArray> &history)
+      : DeviceIntegrator(dots, std::shared_ptr<const Pulse>(), dyadic, interp_order, history)
+  {
+    std::vector<quest_pulse> q;
+    for(const auto &p : pulses) q.push_back(p->to_c());
+    quest_detail::check(quest_set_pulse_list(ctx.get(), 0, static_cast<int>(q.size()), q.data()),
+                        ctx.get());
   }
   void set_weights(const Weights &w)
   {

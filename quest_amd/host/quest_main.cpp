@@ -3,7 +3,8 @@
 // in the reference's format.
 //
 //   quest_hip [--config input.cfg] [--precision fp64|mixed] [--host-loop]
-//             [--output output.dat] [--device N] [--help] [--version]
+//             [--all-pulses] [--output output.dat] [--device N] [--help]
+//             [--version]
 //
 // input.cfg is the reference's INI file (configuration.cpp:17-42):
 //   [files]      qd_path (dots.cfg), pulse_path (pulse.cfg)
@@ -14,6 +15,9 @@
 // By default the whole predictor-corrector loop runs on the device;
 // --host-loop keeps the reference's host Integrator and evaluates the
 // interactions on the device at every predictor/corrector evaluation.
+// --all-pulses drives the dots with every row of pulse.cfg (one pulse
+// interaction each, fields added in file order) instead of the first row
+// only, which is what the reference reads (pulse.cpp:37-46).
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
@@ -78,6 +82,7 @@ struct Options {
   std::string config_path = "input.cfg";
   std::string output = "output.dat";
   bool host_loop = false;
+  bool all_pulses = false;
 };
 
 // configuration.cpp:7-81
@@ -97,6 +102,7 @@ Options parse_configs(int argc, char *argv[])
                    "  -c [ --config ] arg    path to configuration file (default input.cfg)\n"
                    "  --precision arg        fp64 (default) or mixed\n"
                    "  --host-loop            host predictor-corrector, device interactions\n"
+                   "  --all-pulses           every row of pulse.cfg (default: the first)\n"
                    "  --output arg           output file (default output.dat)\n"
                    "  --device arg           HIP device ordinal (default 0)\n";
       throw CommandLineException();
@@ -112,6 +118,8 @@ Options parse_configs(int argc, char *argv[])
       else throw std::runtime_error("--precision must be fp64 or mixed");
     } else if(a == "--host-loop") {
       o.host_loop = true;
+    } else if(a == "--all-pulses") {
+      o.all_pulses = true;
     } else if(a == "--output") {
       o.output = next();
     } else if(a == "--device") {
@@ -159,20 +167,28 @@ int main(int argc, char *argv[])
       for(int sol_idx = 0; sol_idx < config.num_particles; ++sol_idx)
         (*history)[sol_idx][t][0] = {cmplx(1, 0), cmplx(0, 0)};  // ground state
 
-    auto pulse1 = std::make_shared<Pulse>(read_pulse_config(config.pulse_path));
+    std::vector<std::shared_ptr<const Pulse>> pulses;
+    if(opt.all_pulses) {
+      for(const Pulse &p : read_pulse_configs(config.pulse_path)) pulses.push_back(std::make_shared<Pulse>(p));
+      if(pulses.size() > QUEST_MAX_PULSES)
+        throw std::runtime_error("pulse file has more than " + std::to_string(QUEST_MAX_PULSES) + " rows");
+    } else {
+      pulses.push_back(std::make_shared<Pulse>(read_pulse_config(config.pulse_path)));
+    }
     auto rotating_dyadic =
         std::make_shared<GreenFunction::RotatingDyadic>(config.mu0, config.c0, config.hbar, config.omega);
 
     std::cout << "Solving..." << std::endl;
     if(opt.host_loop) {
-      std::vector<std::shared_ptr<Interaction>> interactions{
-          std::make_shared<PulseInteraction>(qds, pulse1),
-          std::make_shared<HistoryInteraction>(qds, history, rotating_dyadic, config.interpolation_order)};
+      std::vector<std::shared_ptr<Interaction>> interactions;
+      for(const auto &p : pulses) interactions.push_back(std::make_shared<PulseInteraction>(qds, p));
+      interactions.push_back(
+          std::make_shared<HistoryInteraction>(qds, history, rotating_dyadic, config.interpolation_order));
       PredictorCorrector::Integrator integrator(config.dt, 18, 22, 3.15, history, rhs_funs,
                                                 std::move(interactions));
       integrator.solve();
     } else {
-      PredictorCorrector::DeviceIntegrator integrator(qds, pulse1, rotating_dyadic,
+      PredictorCorrector::DeviceIntegrator integrator(qds, pulses, rotating_dyadic,
                                                       config.interpolation_order, history);
       integrator.set_weights(PredictorCorrector::Weights(18, 22, 3.15));
       integrator.solve(1);

@@ -108,6 +108,16 @@ static void file_formats()
   const Vec3 e = p({0, 0, 0.5}, 5.0);
   const double arg = 7.6015964 * 0.5;
   CHECK(std::fabs(e.x - 15589.2260227 / 2 * std::exp(-std::pow(arg / 227.89013, 2) / 2)) < 1e-9);
+  {
+    std::ofstream f("mirror_test_pulse.cfg", std::ios::app);
+    f << "\n1000 0.02 0.5 2278.9 7.6 0 0 0 0.6 0.8\n";
+  }
+  // every row, in order (the reference's reader takes the first only)
+  const std::vector<Pulse> all = read_pulse_configs("mirror_test_pulse.cfg");
+  CHECK(all.size() == 2);
+  CHECK(all.size() == 2 && all[0].to_c().amplitude == 15589.2260227 && all[1].to_c().amplitude == 1000);
+  CHECK(all.size() == 2 && all[1].to_c().polarization[2] == 0.8);
+  CHECK(read_pulse_config("mirror_test_pulse.cfg").to_c().amplitude == 15589.2260227);
   std::remove("mirror_test_dots.cfg");
   std::remove("mirror_test_pulse.cfg");
 }

@@ -88,6 +88,35 @@ def test_graph_replays_match_kernel_by_kernel(ref_weights):
             g.sync()
 
 
+def test_batch_pulse_lists_match_individual_contexts(ref_weights):
+    """Per-system pulse lists in a batch (two pulses, none, one) equal the
+    same lists on one context per system."""
+    from quest_amd import Context, systems
+    consts = systems.consts()
+    sys_list, pulses = _systems()
+    sys_list = sys_list[:3]
+    lists = [[pulses[0], short_pulse(area=0.5, delay=0.03, sigma_t=0.004)], [], [pulses[2]]]
+    steps = 200
+    with Context(sys_list, consts) as ctx:
+        ctx.set_pc_weights(*ref_weights)
+        for s, pl in enumerate(lists):
+            ctx.set_pulse_list(pl, system=s)
+        ctx.run(steps)
+        batch_rho, _ = ctx.state()
+    off = 0
+    for s, pl in zip(sys_list, lists):
+        n = len(s["pos"])
+        with Context(s, consts) as one:
+            one.set_pc_weights(*ref_weights)
+            one.set_pulse_list(pl)
+            one.run(steps)
+            rho, _ = one.state()
+        assert rel_err(batch_rho[off:off + n], rho) < 1e-12
+        if not pl:
+            assert np.all(rho[:, 1] == 0)
+        off += n
+
+
 def test_autotune_changes_no_state(ref_weights):
     from quest_amd import Context, systems
     consts = systems.consts()

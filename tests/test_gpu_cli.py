@@ -102,3 +102,44 @@ def test_dense_system(oracle_lib, tmp_path, loop, precision):
     err = rel_err(got, want)
     print(f"{loop} loop {precision}: rel err {err:.3e}, max|rho01| {np.abs(want[:, :, 1]).max():.3f}")
     assert err < (1e-10 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("loop", ["device", "host"])
+def test_all_pulses(oracle_lib, tmp_path, loop):
+    """--all-pulses: every row of pulse.cfg drives the dots (the pulse
+    interactions' fields add); without it only the first row does, as in the
+    reference.  Both against the oracle's list of pulse interactions."""
+    from quest_amd import pc_weights
+    n, steps, omega = 30, 400, 2278.9
+    sysd = dense_system(n, 0.18, seed=78)
+    with open(tmp_path / "dots.cfg", "w") as f:
+        for i in range(n):
+            f.write(" ".join(repr(float(v)) for v in (*sysd["pos"][i], sysd["freq"][i], sysd["t1"][i],
+                                                       sysd["t2"][i], *sysd["dip"][i])) + "\n")
+    pls = []
+    for area, delay, sigma, pol in ((math.pi / 3, 0.012, 0.003, (0.0, 0.0, 1.0)),
+                                    (math.pi / 4, 0.028, 0.004, (0.0, 0.6, 0.8))):
+        amp = area * 0.65821193 / (5.2917721e-4 * sigma * math.sqrt(2 * math.pi))
+        pls.append({"amplitude": amp, "delay": delay, "width": omega * sigma, "freq": omega,
+                    "wavevector": [omega / 299.792458, 0.0, 0.0], "polarization": list(pol)})
+    with open(tmp_path / "pulse.cfg", "w") as f:
+        for pl in pls:
+            f.write(" ".join(repr(float(v)) for v in (pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                                                       *pl["wavevector"], *pl["polarization"])) + "\n")
+    with open(tmp_path / "input.cfg", "w") as f:
+        f.write(f"[parameters]\n num_particles = {n}\n total_time = {steps * 1e-4}\n timestep = 1e-4\n"
+                f" interpolation_order = 4\n[constants]\n c0 = 299.792458\n hbar = 0.65821193\n"
+                f" mu0 = 2.0133545e-04\n laser_frequency = {omega}\n")
+    consts = {"c0": 299.792458, "hbar": 0.65821193, "mu0": 2.0133545e-04, "laser_omega": omega, "dt": 1e-4}
+    ps, cs, fc = pc_weights(18, 22, 3.15)
+    p = oracle_params(oracle_lib, consts)
+    loop_args = ["--host-loop"] if loop == "host" else []
+    for args, used in ((["--all-pulses"], pls), ([], pls[:1])):
+        run_cli(tmp_path, loop_args + args)
+        got = read_output(tmp_path / "output.dat", n)
+        want = oracle_lib.solve(p, sysd, [oracle_pulse(oracle_lib, pl) for pl in used], ps, cs, fc, steps,
+                                mode=1, record_every=1)["rec"]
+        err = rel_err(got, want)
+        print(f"{loop} loop, {len(used)} pulse(s): rel err {err:.3e}")
+        assert np.abs(want[:, :, 1]).max() > 0.1
+        assert err < 1e-10

@@ -340,6 +340,62 @@ def test_solve_modes_agree(oracle_lib, ref_weights):
     assert rel_err(r1["yp"], r0["yp"]) < 1e-12
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_pulse_list_of_one_equals_single_pulse(oracle_lib, ref_weights, mode):
+    """qo_solve_pulses with a one-pulse list is qo_solve, bit for bit."""
+    from quest_amd import systems
+    consts = systems.consts()
+    sysd = dense_system(24, 0.15, seed=8)
+    pl = oracle_pulse(oracle_lib, short_pulse())
+    ps, cs, fc = ref_weights
+    p = oracle_params(oracle_lib, consts)
+    r0 = oracle_lib.solve(p, sysd, pl, ps, cs, fc, 250, mode=mode)
+    r1 = oracle_lib.solve(p, sysd, pl, ps, cs, fc, 250, mode=mode, use_list=True)
+    assert np.array_equal(r0["y"], r1["y"]) and np.array_equal(r0["yp"], r1["yp"])
+
+
+def test_two_equal_pulses_equal_one_of_double_amplitude(oracle_lib, ref_weights):
+    """The pulse fields add: [p, p] drives exactly as p with twice the amplitude
+    (doubling is exact in binary floating point)."""
+    from quest_amd import systems
+    consts = systems.consts()
+    sysd = dense_system(24, 0.15, seed=9)
+    base = short_pulse(area=math.pi / 4)
+    double = dict(base, amplitude=2 * base["amplitude"])
+    ps, cs, fc = ref_weights
+    p = oracle_params(oracle_lib, consts)
+    r2 = oracle_lib.solve(p, sysd, [oracle_pulse(oracle_lib, base)] * 2, ps, cs, fc, 250, mode=1)
+    rd = oracle_lib.solve(p, sysd, oracle_pulse(oracle_lib, double), ps, cs, fc, 250, mode=1)
+    assert np.abs(rd["y"][:, 1]).max() > 0.1
+    assert np.array_equal(r2["y"], rd["y"])
+
+
+@pytest.mark.parametrize("areas", [(math.pi / 4, math.pi / 4), (math.pi / 3, math.pi / 2)])
+def test_two_pulses_add_areas(oracle_lib, ref_weights, areas):
+    """N = 1, resonant, no damping: two separated pulses rotate about the same
+    axis, so rho00 = cos^2((theta1 + theta2) / 2) after both."""
+    from quest_amd import systems
+    consts = systems.consts()
+    p1 = short_pulse(area=areas[0], delay=0.015, sigma_t=0.003)
+    p2 = short_pulse(area=areas[1], delay=0.035, sigma_t=0.003)
+    ps, cs, fc = ref_weights
+    r = oracle_lib.solve(oracle_params(oracle_lib, consts), _single_dot(),
+                         [oracle_pulse(oracle_lib, p1), oracle_pulse(oracle_lib, p2)],
+                         ps, cs, fc, 550, mode=0)
+    total = areas[0] + areas[1]
+    assert abs(r["y"][0, 0].real - math.cos(total / 2) ** 2) < 1e-6
+
+
+def test_empty_pulse_list_keeps_ground_state(oracle_lib, ref_weights):
+    from quest_amd import systems
+    consts = systems.consts()
+    sysd = dense_system(12, 0.1, seed=10)
+    ps, cs, fc = ref_weights
+    r = oracle_lib.solve(oracle_params(oracle_lib, consts), sysd, [], ps, cs, fc, 60, mode=1)
+    # rho00 moves only by the rounding of the PC weights (they sum to 1 to ~1e-13)
+    assert np.abs(r["y"][:, 0] - 1).max() < 1e-12 and np.all(r["y"][:, 1] == 0)
+
+
 def test_solve_reports_nonfinite(oracle_lib, ref_weights):
     from quest_amd import systems
     consts = systems.consts()

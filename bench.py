@@ -35,10 +35,12 @@ FP32_PEAK = 157.3      # TFLOP/s, MI355X vector FP32 (datasheet)
 FP64_PEAK = 78.6       # TFLOP/s, MI355X vector FP64
 
 
-def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights):
+def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0):
     from quest_amd import Context
     ps, cs, fc = weights
     with Context(sysd, consts, precision=precision, pulse=pulse) as ctx:
+        if method:
+            ctx.set_pair_method(method)
         ctx.set_pc_weights(ps, cs, fc)
         info = ctx.info()
         if warmup:
@@ -123,6 +125,41 @@ def ensemble_main(args):
     return 0
 
 
+def table_main(args):
+    """FP64 coefficient-table method (QUEST_PAIR_TABLE: the reference's
+    precomputed per-pair coefficients streamed every step) beside the FP64
+    on-the-fly path on the same workload.  The table kernel is HBM-bound: its
+    roofline is the table bytes it streams per launch, (M + 1) x 16 B + 2 B per
+    ordered pair, over the kernel's mean duration, against 8 TB/s."""
+    from quest_amd import PAIR_TABLE, pc_weights, systems
+    sysd, full_steps = systems.workload(args.workload)
+    consts = systems.consts()
+    pulse = systems.pulse(math.pi / 2)
+    weights = pc_weights(18, 22, 3.15)
+    n = sysd["pos"].shape[0]
+    tab = run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights, method=PAIR_TABLE)
+    fly = run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights)
+    pairs = float(n) * n
+    value = pairs * args.steps / tab["seconds"]
+    table_bytes = pairs * (5 * 16 + 2)
+    achieved = table_bytes / (tab["pair_ms"] * 1e-3) / 1e9
+    out = {
+        "metric": "pair-interactions/s (N²·steps/s), FP64 coefficient-table method",
+        "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * tab["seconds"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: N={n} dots, table of {table_bytes / 1e9:.2f} GB, "
+                               f"{full_steps}-step run", "n_dots": n},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None, "kernel": "pair_table_f64<4,false>",
+                     "kernel_ms": tab["pair_ms"], "bytes_per_pair": 82},
+        "on_the_fly_fp64": {"value": pairs * args.steps / fly["seconds"], "pair_kernel_ms": fly["pair_ms"]},
+        "rel_rho_table_vs_on_the_fly": float(np.abs(tab["rho"] - fly["rho"]).max() / np.abs(fly["rho"]).max()),
+    }
+    print(json.dumps(out))
+    return 0
+
+
 def rho01_rel_err(mixed_rho, fp64_rho) -> float:
     """max |rho01(mixed) - rho01(fp64)| / max |rho01(fp64)| over all dots."""
     scale = float(np.abs(fp64_rho[:, 1]).max())
@@ -187,6 +224,33 @@ def cpu_baseline(steps=5):
                        f"excluded); N=65536 needs a 180 GB table")}
 
 
+def cpu_best(steps=20):
+    """The oracle's split gather (one pair sum per step plus the zero-delay
+    pairs per evaluation, the arithmetic the GPU path uses) on up to 16 host
+    threads (the box's CPU share per GPU): the best CPU figure beside the
+    serial reference method."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import pc_weights
+    from quest_amd import systems
+    oracle.build()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    oracle.set_threads(threads)
+    sysd, _ = systems.workload("n4096")
+    c = systems.consts()
+    pl = systems.pulse(math.pi / 2)
+    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
+    ps, cs, fc = pc_weights.weights()
+    r = oracle.solve(p, sysd, oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                                           pl["wavevector"], pl["polarization"]), ps, cs, fc, steps, mode=1)
+    oracle.set_threads(1)
+    n = sysd["pos"].shape[0]
+    return {"value": n * n * steps / r["seconds"], "unit": "pair-interactions/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"FP64 oracle, split gather (OpenMP), N={n} system of the same generator, "
+                       f"{steps} steps in {r['seconds']:.2f} s on {threads} threads (setup excluded)")}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +259,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="n65536")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--pair-method", choices=("on-the-fly", "table"), default="on-the-fly",
+                    help="table: the FP64 coefficient-table method beside FP64 on the fly")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -204,6 +270,8 @@ def main(argv=None) -> int:
 
     if args.workload == "ensemble":
         return ensemble_main(args)
+    if args.pair_method == "table":
+        return table_main(args)
     from quest_amd import pc_weights, systems
     sysd, full_steps = systems.workload(args.workload)
     consts = systems.consts()
@@ -253,6 +321,7 @@ def main(argv=None) -> int:
         out["rel_rho01_err_vs_fp64"] = rho01_rel_err(mixed["rho"], fp64["rho"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+        out["cpu_best"] = cpu_best()
     print(json.dumps(out))
     return 0
 

@@ -121,6 +121,25 @@ const char *quest_last_error(const quest_ctx *ctx);
 int quest_hist_evaluate(quest_ctx *ctx, int step, const double *rho01,
                         double *out);
 
+/* How the past pair sum is formed.
+ * ON_THE_FLY: Green's function and interpolant evaluated per pair per step
+ *             (default; FP64 or mixed).
+ * TABLE:      the reference's own method -- a per-pair coefficient table
+ *             built once (HistoryInteraction::build_coefficient_table,
+ *             history_interaction.cpp:21-45) and streamed every step against a
+ *             ring of rho01 samples (evaluate, :47-70).  FP64 contexts only;
+ *             memory (order + 1) x 16 B + 2 B per ordered pair of each system
+ *             (1.4 GB at N = 4096, 44 GB at N = 23170), checked against free
+ *             device memory (QUEST_ERR_NOMEM).  Bound by HBM bandwidth. */
+#define QUEST_PAIR_ON_THE_FLY 0
+#define QUEST_PAIR_TABLE 1
+
+/* Selects the pair method; call before the first quest_run or
+ * quest_hist_evaluate (QUEST_ERR_STATE afterwards).  Replaces the choice the
+ * reference makes once and for all in HistoryInteraction's constructor
+ * (history_interaction.cpp:5-19 builds the table). */
+int quest_set_pair_method(quest_ctx *ctx, int method);
+
 /* --------------------------------------------- PulseInteraction::evaluate */
 
 /* Sets (or, with NULL, clears) the pulse of every system: Pulse

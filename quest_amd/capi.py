@@ -39,8 +39,9 @@ EXPORTS = [
     "quest_run", "quest_current_step", "quest_bad_step", "quest_get_state",
     "quest_pair_delays", "quest_set_timing", "quest_pair_kernel_time", "quest_ctx_info",
     "quest_sync", "quest_batch_create", "quest_set_pulses", "quest_autotune",
-    "quest_set_pulse_list",
+    "quest_set_pulse_list", "quest_set_pair_method",
 ]
+PAIR_ON_THE_FLY, PAIR_TABLE = 0, 1  # QUEST_PAIR_*
 MAX_PULSES = 8  # QUEST_MAX_PULSES
 
 
@@ -106,6 +107,7 @@ def load():
     L.quest_set_pulses.argtypes = [vp, C.POINTER(Pulse)]
     L.quest_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.quest_set_pulse_list.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Pulse)]
+    L.quest_set_pair_method.argtypes = [vp, C.c_int]
     for name in EXPORTS:
         if name not in ("quest_ctx_destroy", "quest_last_error"):
             getattr(L, name).restype = C.c_int
@@ -223,6 +225,11 @@ class Context:
         empty list clears the system's pulses."""
         arr = (Pulse * max(len(pulses), 1))(*[_pulse_struct(pl) for pl in pulses])
         self._check(self._L.quest_set_pulse_list(self.ptr, int(system), len(pulses), arr))
+
+    def set_pair_method(self, method: int):
+        """PAIR_ON_THE_FLY (default) or PAIR_TABLE (the reference's per-pair
+        coefficient table, FP64 only); before the first step."""
+        self._check(self._L.quest_set_pair_method(self.ptr, int(method)))
 
     def autotune(self, launches: int = 3) -> int:
         """quest_autotune: the fastest source-chunk length (kept)."""

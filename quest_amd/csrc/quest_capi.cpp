@@ -55,6 +55,7 @@ struct quest_ctx {
   int partial_rows = 0;        // rows allocated in d.partial
   int cu_count = 0;
   int blocks_per_cu = 1;
+  int pair_method = QUEST_PAIR_ON_THE_FLY;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
 };
@@ -363,7 +364,7 @@ void free_device(quest_ctx *c)
                   d.d0, d.d1, d.cur0, d.cur1, d.yc0, d.yc1, d.field, d.partial, d.phase, d.out,
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.sys_of, d.sys_range, d.sys_chunks,
-                  d.pulses, d.work, d.work_row, d.step};
+                  d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff};
   for(void *p : ptrs)
     if(p) (void)hipFree(p);
   d = DeviceState{};
@@ -619,6 +620,68 @@ PulseParams to_params(const quest_pulse *pl)
   return q;
 }
 
+void free_table(quest_ctx *c)
+{
+  DeviceState &d = c->d;
+  for(void *p : {static_cast<void *>(d.samp), static_cast<void *>(d.table), static_cast<void *>(d.table_n),
+                 static_cast<void *>(d.sys_toff)})
+    if(p) (void)hipFree(p);
+  d.samp = nullptr;
+  d.table = nullptr;
+  d.table_n = nullptr;
+  d.sys_toff = nullptr;
+  c->k.pair_table = 0;
+  c->k.table_plane = 0;
+}
+
+// Coefficient-table mode: the reference's per-pair coefficient table
+// (history_interaction.cpp:21-45) built on the device for every system, and
+// a ring of rho01 samples for the table's gather.
+int enable_table(quest_ctx *c)
+{
+  const int M = c->P.interp_order;
+  if(c->P.precision != QUEST_PREC_FP64)
+    return fail(c, QUEST_ERR_ARG, "the coefficient table is FP64: create the context with QUEST_PREC_FP64");
+  if(c->dmax_bound + M + 2 > 32767)
+    return fail(c, QUEST_ERR_ARG, "delays exceed the table's 16-bit delay field");
+  std::vector<long long> toff(c->n_sys);
+  long long plane = 0;
+  for(int s = 0; s < c->n_sys; ++s) {
+    const long long ns = c->sys_begin[s + 1] - c->sys_begin[s];
+    toff[s] = plane;
+    plane += ns * ns;
+  }
+  const size_t coef_bytes = static_cast<size_t>(plane) * (M + 1) * sizeof(double2);
+  const size_t n_bytes = (static_cast<size_t>(plane) * sizeof(int16_t) + 15) / 16 * 16;
+  Consts &k = c->k;
+  k.samp_slots = 1;
+  while(k.samp_slots < c->dmax_bound + M + 2) k.samp_slots <<= 1;
+  k.samp_mask = k.samp_slots - 1;
+  const size_t samp_bytes = static_cast<size_t>(c->n) * k.samp_slots * sizeof(double2);
+  size_t free_b = 0, total_b = 0;
+  HIPC(c, hipMemGetInfo(&free_b, &total_b));
+  const size_t need = coef_bytes + n_bytes + samp_bytes;
+  if(need + (size_t(1) << 30) > free_b)
+    return fail(c, QUEST_ERR_NOMEM, "coefficient table needs " + std::to_string(need >> 20) + " MiB, " +
+                                        std::to_string(free_b >> 20) + " MiB free");
+  DeviceState &d = c->d;
+  if(hipMalloc(&d.table, coef_bytes) != hipSuccess || hipMalloc(&d.table_n, n_bytes) != hipSuccess ||
+     hipMalloc(&d.samp, samp_bytes) != hipSuccess ||
+     hipMalloc(&d.sys_toff, sizeof(long long) * c->n_sys) != hipSuccess) {
+    free_table(c);
+    return fail(c, QUEST_ERR_NOMEM, "coefficient table allocation failed");
+  }
+  HIPC(c, hipMemcpyAsync(d.sys_toff, toff.data(), sizeof(long long) * c->n_sys, hipMemcpyHostToDevice,
+                         c->stream));
+  k.table_plane = plane;
+  for(int s = 0; s < c->n_sys; ++s)
+    HIPC(c, launch_table_build(k, d, make_int2(c->sys_begin[s], c->sys_begin[s + 1]), toff[s], c->stream));
+  HIPC(c, launch_clear(d.samp, samp_bytes, c->stream));  // rho01 = 0 before step 0 (main.cpp:32-36)
+  HIPC(c, hipStreamSynchronize(c->stream));
+  k.pair_table = 1;
+  return QUEST_OK;
+}
+
 // Pulse list of system s on the host copy: the given pulses first, the rest off.
 void put_pulses(quest_ctx *c, int s, int n, const quest_pulse *pl)
 {
@@ -761,6 +824,26 @@ int quest_set_pulse_list(quest_ctx *c, int system, int n_pulses, const quest_pul
   HIPC(c, hipSetDevice(c->P.device));
   put_pulses(c, system, n_pulses, pulses);
   return upload_pulses(c);
+}
+
+int quest_set_pair_method(quest_ctx *c, int method)
+{
+  if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
+  if(method != QUEST_PAIR_ON_THE_FLY && method != QUEST_PAIR_TABLE)
+    return fail(c, QUEST_ERR_ARG, "method must be QUEST_PAIR_ON_THE_FLY or QUEST_PAIR_TABLE");
+  if(method == c->pair_method) return QUEST_OK;
+  if(c->mode != 0) return fail(c, QUEST_ERR_STATE, "choose the pair method before the first step");
+  HIPC(c, hipSetDevice(c->P.device));
+  drop_graph(c);
+  if(method == QUEST_PAIR_TABLE) {
+    const int rc = enable_table(c);
+    if(rc != QUEST_OK) return rc;
+  } else {
+    free_table(c);
+  }
+  c->pair_method = method;
+  c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
+  return build_work(c, c->chunk_len);
 }
 
 int quest_pulse_evaluate(quest_ctx *c, int step, double *out)

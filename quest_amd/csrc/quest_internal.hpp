@@ -51,6 +51,11 @@ struct Consts {
   int n_systems;      // independent dot systems (pairs only within a system)
   int n_work;         // pair-kernel work items (system, target tile, source chunk)
   int max_chunks;     // largest number of source chunks of one system (rows of `partial`)
+  // coefficient-table mode (the reference's precomputed per-pair coefficients)
+  int pair_table;     // 1: the pair sum streams `table` instead of evaluating on the fly
+  int samp_slots;     // rho01 sample ring: power of two > max delay + order + 1
+  int samp_mask;
+  long long table_plane;  // table entries per coefficient plane (sum over systems of n_s^2)
 };
 
 struct PulseParams {
@@ -89,12 +94,21 @@ struct DeviceState {
   int4 *work = nullptr;                // [n_work] (target begin, target end, source begin, source end)
   int *work_row = nullptr;             // [n_work] partial row written by the item
   int *step = nullptr;                 // the step being integrated (device counter; graph replays read it)
+  // coefficient-table mode only
+  double2 *samp = nullptr;             // rho01 sample ring [n][samp_slots] (finished steps)
+  double2 *table = nullptr;            // [order + 1][table_plane]: system s, source b, target a at
+                                       // sys_toff[s] + (b - begin_s) n_s + (a - begin_s)
+  int16_t *table_n = nullptr;          // [table_plane] integer delay (0 for excluded pairs)
+  long long *sys_toff = nullptr;       // [systems] first table entry of each system
 };
 
 // ---------------------------------------------------------------- launchers
 // All launch on `s` and return the hipError_t of the launch.
 hipError_t launch_fill_ground_state(const Consts &k, DeviceState &d, hipStream_t s);
 hipError_t launch_clear_ring(const Consts &k, DeviceState &d, size_t ring_bytes, hipStream_t s);
+hipError_t launch_clear(void *p, size_t bytes, hipStream_t s);  // bytes: a multiple of 16
+// Coefficient table of one system (dots [range.x, range.y)), entries from toff.
+hipError_t launch_table_build(const Consts &k, DeviceState &d, int2 range, long long toff, hipStream_t s);
 // Kernels of the step read the step number from d.step (so that one captured
 // graph serves every step); launch_set_step / launch_advance_step write it.
 hipError_t launch_set_step(DeviceState &d, int step, hipStream_t s);

@@ -408,6 +408,49 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
   }
 }
 
+// Past-only pair sum from the coefficient table (the reference's evaluate,
+// history_interaction.cpp:47-70, as a gather: per target, sum over sources of
+// coef[m] rho01_b(step - n - m)).  Same work list, grid and partial layout as
+// the on-the-fly kernels; the coefficients are complete, so pair_field applies
+// no further scale.  Bound by HBM: per ordered pair the kernel streams
+// (M + 1) x 16 B of coefficients and a 2-B delay; the sample gather (M + 1)
+// x 16 B per lane hits L1/L2 (a source's few hundred samples are shared by
+// all targets).
+template <int M, bool LAB>
+__global__ __launch_bounds__(kBlock) void pair_table_f64(
+    const double2 *__restrict__ table, const int16_t *__restrict__ table_n,
+    const double2 *__restrict__ samp, const int4 *__restrict__ work, const int *__restrict__ work_row,
+    const int *__restrict__ step_ptr, const int *__restrict__ sys_of, const int2 *__restrict__ sys_range,
+    const long long *__restrict__ sys_toff, double2 *__restrict__ partial, Consts k)
+{
+  const int step = *step_ptr;
+  const size_t P = static_cast<size_t>(k.table_plane);
+  for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
+    const int4 wi = work[w];
+    const int s = sys_of[wi.x];
+    const int2 rg = sys_range[s];
+    const size_t ns = static_cast<size_t>(rg.y - rg.x);
+    const int tgt = wi.x + static_cast<int>(threadIdx.x);
+    const int a = tgt < wi.y ? tgt : wi.y - 1;
+    const size_t base = static_cast<size_t>(sys_toff[s]) + static_cast<size_t>(a - rg.x);
+    double accr = 0, acci = 0;
+    for(int b = wi.z; b < wi.w; ++b) {
+      const size_t e = base + static_cast<size_t>(b - rg.x) * ns;
+      const int n = table_n[e];
+      const double2 *sb = samp + static_cast<size_t>(b) * k.samp_slots;
+#pragma unroll
+      for(int m = 0; m <= M; ++m) {
+        const double2 c = table[m * P + e];
+        double2 r = sb[(step - n - m) & k.samp_mask];
+        if constexpr(LAB) r = make_double2(2.0 * r.x, 0.0);
+        accr = fma(c.x, r.x, fma(-c.y, r.y, accr));
+        acci = fma(c.x, r.y, fma(c.y, r.x, acci));
+      }
+    }
+    if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
+  }
+}
+
 // ------------------------------------------------------- per-dot kernels
 
 // Field scale of the pair sum at `step`: -K e^{-i theta step} (rotating) or K (lab).
@@ -425,6 +468,14 @@ __device__ __forceinline__ double2 reduce_partials(const Consts &k, const Device
   const int rows = d.sys_chunks[d.sys_of[a]];
   for(int j = 0; j < rows; ++j) s = cadd(s, d.partial[static_cast<size_t>(j) * k.n + a]);
   return s;
+}
+
+// Field of the past pair sum at target a: the partials times -K e^{-i theta
+// step} (on-the-fly) or as they are (the table holds complete coefficients).
+__device__ __forceinline__ double2 pair_field(const Consts &k, const DeviceState &d, int a, int step)
+{
+  const double2 sum = reduce_partials(k, d, a);
+  return k.pair_table ? sum : cmul(sum, hist_scale(k, d.phase, step));
 }
 
 // Pulse::operator() projected on the dipole (pulse.cpp:15-20,
@@ -509,8 +560,10 @@ __device__ __forceinline__ double2 near_sum(const Consts &k, const DeviceState &
 // 2 Re rho01 without the phase (lab).
 template <typename R2>
 __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase, R2 *ring,
-                                           const double2 *y1, double2 rho_s, int a, int s)
+                                           double2 *samp, const double2 *y1, double2 rho_s, int a,
+                                           int s)
 {
+  if(samp) samp[static_cast<size_t>(a) * k.samp_slots + (s & k.samp_mask)] = rho_s;
   double2 smp[kMaxOrder + 1];
   smp[0] = rho_s;
   for(int m = 1; m <= k.order; ++m) smp[m] = y1[static_cast<size_t>((s - m) & k.state_mask) * k.n + a];
@@ -566,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState 
   const int step = *d.step;
   const double dt = k.dt;
   double2 hist = make_double2(0.0, 0.0);
-  if(k.n_work > 0) hist = cmul(reduce_partials(k, d, a), hist_scale(k, d.phase, step));
+  if(k.n_work > 0) hist = pair_field(k, d, a, step);
   const double2 pulse = pulse_sum(k, d, a, step * dt);
   d.field[a] = cadd(pulse, hist);
 
@@ -624,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d
   d.d0[o] = m0;
   d.d1[o] = m1;
   if(!(finite2(y0) && finite2(y1) && finite2(m0) && finite2(m1))) atomicMin(d.bad_step, step);
-  build_slot(k, d.phase, ring, d.y1, y1, a, step);
+  build_slot(k, d.phase, ring, d.samp, d.y1, y1, a, step);
 }
 
 // Per-call mode: the values passed for `step` become final -- state ring
@@ -637,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void hist_commit_kernel(Consts k, DeviceSta
   if(a >= k.n) return;
   const double2 y1 = d.cur1[a];
   d.y1[static_cast<size_t>(step & k.state_mask) * k.n + a] = y1;
-  build_slot(k, d.phase, ring, d.y1, y1, a, step);
+  build_slot(k, d.phase, ring, d.samp, d.y1, y1, a, step);
 }
 
 // Per-call mode: the pair sum of the finished history, scaled.
@@ -647,7 +700,7 @@ __global__ __launch_bounds__(kBlock) void hist_reduce_kernel(Consts k, DeviceSta
   if(a >= k.n) return;
   const int step = *d.step;
   double2 hist = make_double2(0.0, 0.0);
-  if(k.n_work > 0) hist = cmul(reduce_partials(k, d, a), hist_scale(k, d.phase, step));
+  if(k.n_work > 0) hist = pair_field(k, d, a, step);
   d.field[a] = hist;
 }
 
@@ -738,20 +791,17 @@ __device__ void lagrange_products(double x, int M, double *l0, double *l1, doubl
   }
 }
 
-// FP64 per-pair coefficients of the zero-delay list: the reference's
-// coefficients[pair][m] (history_interaction.cpp:29-43 with
-// lagrange_set.cpp:17-41 and rotating_green_function.cpp:3-24 /
-// green_function.cpp:3-20), target a <- source b.
-__global__ __launch_bounds__(kBlock) void near_coef_kernel(Consts k, DeviceState d, int total)
+// The reference's FP64 coefficients[pair][m] of target a <- source b
+// (history_interaction.cpp:29-43 with lagrange_set.cpp:17-41 and
+// rotating_green_function.cpp:3-24 / green_function.cpp:3-20), already
+// multiplied by the prefactor -K e^{-i w r / c} (rotating) or K (lab); r > 0.
+// Returns the integer delay.
+__device__ int ref_pair_coefs(const Consts &k, double4 pa, double4 pb, double4 da, double4 db,
+                              double2 *out)
 {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if(e >= total) return;
-  const int a = d.near_tgt[e], b = d.near_src[e];
-  const double4 pa = d.pos64[a], pb = d.pos64[b], da = d.dip64[a], db = d.dip64[b];
   const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
   double x, ir, r2;
   const int n = split_f64(dx, dy, dz, k.cdt, k.icdt, x, ir, r2);
-  d.near_n[e] = n;
   const int M = k.order;
   double l0[kMaxOrder + 1], l1[kMaxOrder + 1], l2[kMaxOrder + 1];
   lagrange_products(x, M, l0, l1, l2);
@@ -772,7 +822,6 @@ __global__ __launch_bounds__(kBlock) void near_coef_kernel(Consts k, DeviceState
   } else {
     ph = make_double2(k.k_green, 0.0);
   }
-  double2 *out = d.near_coef + static_cast<size_t>(e) * (M + 1);
   for(int m = 0; m <= M; ++m) {
     // time-derivative weights: w1 = -L'(x)/dt, w2 = L''(x)/dt^2
     const double w0 = l0[m], w1 = -l1[m] / k.dt, w2 = l2[m] / (k.dt * k.dt);
@@ -781,6 +830,50 @@ __global__ __launch_bounds__(kBlock) void near_coef_kernel(Consts k, DeviceState
     else
       g = make_double2(T0 * w0 + T1 * w1 + T2 * (w2 - w * w * w0), T1 * w * w0 + T2 * 2.0 * w * w1);
     out[m] = cmul(ph, g);
+  }
+  return n;
+}
+
+// FP64 per-pair coefficients of the zero-delay list.
+__global__ __launch_bounds__(kBlock) void near_coef_kernel(Consts k, DeviceState d, int total)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= total) return;
+  const int a = d.near_tgt[e], b = d.near_src[e];
+  double2 cf[kMaxOrder + 1];
+  d.near_n[e] = ref_pair_coefs(k, d.pos64[a], d.pos64[b], d.dip64[a], d.dip64[b], cf);
+  double2 *out = d.near_coef + static_cast<size_t>(e) * (k.order + 1);
+  for(int m = 0; m <= k.order; ++m) out[m] = cf[m];
+}
+
+// Coefficient table of one system (the reference's build_coefficient_table,
+// history_interaction.cpp:21-45, for every ordered pair): pairs with integer
+// delay >= 1 get their coefficients and delay; the self pair and the
+// zero-delay pairs (evaluated from the near list) get zeros.
+__global__ __launch_bounds__(kBlock) void table_build_kernel(Consts k, DeviceState d, int2 range,
+                                                             long long toff)
+{
+  const long long ns = range.y - range.x;
+  const long long total = ns * ns;
+  const size_t P = static_cast<size_t>(k.table_plane);
+  for(long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < total;
+      i += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const int b = range.x + static_cast<int>(i / ns), a = range.x + static_cast<int>(i % ns);
+    const size_t e = static_cast<size_t>(toff + i);
+    double2 cf[kMaxOrder + 1];
+    int n = 0;
+    if(a != b) {
+      const double4 pa = d.pos64[a], pb = d.pos64[b];
+      double x, ir, r2;
+      n = split_f64(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, k.cdt, k.icdt, x, ir, r2);
+      if(n >= 1) n = ref_pair_coefs(k, pa, pb, d.dip64[a], d.dip64[b], cf);
+    }
+    if(n < 1) {
+      n = 0;
+      for(int m = 0; m <= k.order; ++m) cf[m] = make_double2(0.0, 0.0);
+    }
+    for(int m = 0; m <= k.order; ++m) d.table[m * P + e] = cf[m];
+    d.table_n[e] = static_cast<int16_t>(n);
   }
 }
 
@@ -829,11 +922,26 @@ hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, hipStre
 }
 
 template <int M>
-int blocks_per_cu_m(int precision)
+hipError_t launch_table_m(const Consts &k, DeviceState &d, hipStream_t s)
+{
+  const dim3 grid(k.grid), block(kBlock);
+  if(k.dyadic == QUEST_DYADIC_LAB)
+    hipLaunchKernelGGL((pair_table_f64<M, true>), grid, block, 0, s, d.table, d.table_n, d.samp, d.work,
+                       d.work_row, d.step, d.sys_of, d.sys_range, d.sys_toff, d.partial, k);
+  else
+    hipLaunchKernelGGL((pair_table_f64<M, false>), grid, block, 0, s, d.table, d.table_n, d.samp, d.work,
+                       d.work_row, d.step, d.sys_of, d.sys_range, d.sys_toff, d.partial, k);
+  return hipGetLastError();
+}
+
+template <int M>
+int blocks_per_cu_m(int precision, bool table)
 {
   int nb = 0;
   hipError_t e;
-  if(precision == QUEST_PREC_FP64)
+  if(table)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_table_f64<M, false>, kBlock, 0);
+  else if(precision == QUEST_PREC_FP64)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false>, kBlock, 0);
   else
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false>, kBlock, 0);
@@ -863,6 +971,25 @@ hipError_t launch_clear_ring(const Consts &k, DeviceState &d, size_t ring_bytes,
   return hipGetLastError();
 }
 
+hipError_t launch_clear(void *p, size_t bytes, hipStream_t s)
+{
+  const size_t n16 = bytes / 16;
+  const int blocks = static_cast<int>(std::min<size_t>((n16 + kBlock - 1) / kBlock, 8192));
+  hipLaunchKernelGGL(clear_words_kernel, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s,
+                     static_cast<uint4 *>(p), n16);
+  return hipGetLastError();
+}
+
+hipError_t launch_table_build(const Consts &k, DeviceState &d, int2 range, long long toff, hipStream_t s)
+{
+  const long long ns = range.y - range.x;
+  const long long total = ns * ns;
+  if(total == 0) return hipSuccess;
+  const int blocks = static_cast<int>(std::min<long long>((total + kBlock - 1) / kBlock, 65536));
+  hipLaunchKernelGGL(table_build_kernel, dim3(blocks), dim3(kBlock), 0, s, k, d, range, toff);
+  return hipGetLastError();
+}
+
 hipError_t launch_set_step(DeviceState &d, int step, hipStream_t s)
 {
   hipLaunchKernelGGL(set_step_kernel, dim3(1), dim3(1), 0, s, d.step, step);
@@ -878,6 +1005,17 @@ hipError_t launch_advance_step(DeviceState &d, hipStream_t s)
 hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
   if(k.n_work == 0) return hipSuccess;
+  if(k.pair_table) {
+    switch(k.order) {
+    case 1: return launch_table_m<1>(k, d, s);
+    case 2: return launch_table_m<2>(k, d, s);
+    case 3: return launch_table_m<3>(k, d, s);
+    case 4: return launch_table_m<4>(k, d, s);
+    case 5: return launch_table_m<5>(k, d, s);
+    case 6: return launch_table_m<6>(k, d, s);
+    default: return hipErrorInvalidValue;
+    }
+  }
   switch(k.order) {
   case 1: return launch_pair_m<1>(k, d, precision, s);
   case 2: return launch_pair_m<2>(k, d, precision, s);
@@ -891,13 +1029,14 @@ hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipS
 
 int pair_blocks_per_cu(const Consts &k, int precision)
 {
+  const bool t = k.pair_table != 0;
   switch(k.order) {
-  case 1: return blocks_per_cu_m<1>(precision);
-  case 2: return blocks_per_cu_m<2>(precision);
-  case 3: return blocks_per_cu_m<3>(precision);
-  case 4: return blocks_per_cu_m<4>(precision);
-  case 5: return blocks_per_cu_m<5>(precision);
-  default: return blocks_per_cu_m<6>(precision);
+  case 1: return blocks_per_cu_m<1>(precision, t);
+  case 2: return blocks_per_cu_m<2>(precision, t);
+  case 3: return blocks_per_cu_m<3>(precision, t);
+  case 4: return blocks_per_cu_m<4>(precision, t);
+  case 5: return blocks_per_cu_m<5>(precision, t);
+  default: return blocks_per_cu_m<6>(precision, t);
   }
 }
 

@@ -75,6 +75,9 @@ WORKLOADS = {
     "n4096": dict(n=4096, steps=5000, random_dipoles=False, detuning_sigma=0.0, seed=2),
     "n65536": dict(n=65536, steps=2000, random_dipoles=True, detuning_sigma=2.0, seed=3),
     "n131072": dict(n=131072, steps=1000, random_dipoles=True, detuning_sigma=2.0, seed=5),
+    # coefficient-table measurements: the largest size whose FP64 table (22 GB) is
+    # far from the memory limit while streaming for milliseconds per step
+    "n16384": dict(n=16384, steps=2000, random_dipoles=True, detuning_sigma=2.0, seed=6),
 }
 
 

@@ -137,6 +137,8 @@ def table_main(args):
     pulse = systems.pulse(math.pi / 2)
     weights = pc_weights(18, 22, 3.15)
     n = sysd["pos"].shape[0]
+    if args.copy_probe:
+        copy_probe(args.copy_probe << 20)
     tab = run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights, method=PAIR_TABLE)
     fly = run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights)
     pairs = float(n) * n

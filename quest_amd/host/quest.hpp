@@ -50,6 +50,7 @@ struct Configuration {
   int interpolation_order = 0;
   std::string qd_path = "dots.cfg", pulse_path = "pulse.cfg";
   int precision = QUEST_PREC_FP64;  // arithmetic of the pair sum on the device
+  int pair_method = QUEST_PAIR_ON_THE_FLY;  // or QUEST_PAIR_TABLE (the reference's coefficient table)
   int device = 0;
 };
 inline Configuration config;
@@ -431,7 +432,11 @@ class HistoryInteraction : public Interaction {
                      const std::shared_ptr<GreenFunction::Dyadic> &dyadic, const int interp_order)
       : Interaction(dots), history(history), dyadic(dyadic),
         ctx(*dots, quest_detail::params_from_config(dyadic->kind(), dyadic->omega(), interp_order)),
-        rho(2 * dots->size()), out(2 * dots->size()) {}
+        rho(2 * dots->size()), out(2 * dots->size())
+  {
+    if(config.pair_method != QUEST_PAIR_ON_THE_FLY)
+      quest_detail::check(quest_set_pair_method(ctx.get(), config.pair_method), ctx.get());
+  }
 
   const ResultArray &evaluate(const int time_idx) override
   {
@@ -587,6 +592,8 @@ class DeviceIntegrator {
       const quest_pulse q = pulse->to_c();
       quest_detail::check(quest_set_pulse(ctx.get(), &q), ctx.get());
     }
+    if(config.pair_method != QUEST_PAIR_ON_THE_FLY)
+      quest_detail::check(quest_set_pair_method(ctx.get(), config.pair_method), ctx.get());
   }
   // Several pulse interactions (their fields add, in list order, ahead of the
   // history interaction -- the reference integrator's sum over interactions,

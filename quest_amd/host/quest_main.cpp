@@ -3,8 +3,8 @@
 // in the reference's format.
 //
 //   quest_hip [--config input.cfg] [--precision fp64|mixed] [--host-loop]
-//             [--all-pulses] [--output output.dat] [--device N] [--help]
-//             [--version]
+//             [--all-pulses] [--pair-method on-the-fly|table]
+//             [--output output.dat] [--device N] [--help] [--version]
 //
 // input.cfg is the reference's INI file (configuration.cpp:17-42):
 //   [files]      qd_path (dots.cfg), pulse_path (pulse.cfg)
@@ -18,6 +18,8 @@
 // --all-pulses drives the dots with every row of pulse.cfg (one pulse
 // interaction each, fields added in file order) instead of the first row
 // only, which is what the reference reads (pulse.cpp:37-46).
+// --pair-method table streams the reference's per-pair coefficient table
+// (FP64 only) instead of evaluating the Green's function per pair.
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
@@ -103,6 +105,7 @@ Options parse_configs(int argc, char *argv[])
                    "  --precision arg        fp64 (default) or mixed\n"
                    "  --host-loop            host predictor-corrector, device interactions\n"
                    "  --all-pulses           every row of pulse.cfg (default: the first)\n"
+                   "  --pair-method arg      on-the-fly (default) or table (FP64 only)\n"
                    "  --output arg           output file (default output.dat)\n"
                    "  --device arg           HIP device ordinal (default 0)\n";
       throw CommandLineException();
@@ -120,6 +123,11 @@ Options parse_configs(int argc, char *argv[])
       o.host_loop = true;
     } else if(a == "--all-pulses") {
       o.all_pulses = true;
+    } else if(a == "--pair-method") {
+      const std::string v = next();
+      if(v == "on-the-fly") config.pair_method = QUEST_PAIR_ON_THE_FLY;
+      else if(v == "table") config.pair_method = QUEST_PAIR_TABLE;
+      else throw std::runtime_error("--pair-method must be on-the-fly or table");
     } else if(a == "--output") {
       o.output = next();
     } else if(a == "--device") {

@@ -144,6 +144,23 @@ void lagrange_monomials(int M, double *vinv)
   }
 }
 
+// vinv_c[j][m] = sum_{k >= j} C(k, j) (1/2)^(k - j) vinv[k][m]: the Lagrange
+// basis as polynomials in (x - 1/2).
+void centre_basis(int M, const double *vinv, double *vinv_c)
+{
+  for(int i = 0; i < (kMaxOrder + 1) * (kMaxOrder + 1); ++i) vinv_c[i] = 0;
+  for(int j = 0; j <= M; ++j)
+    for(int m = 0; m <= M; ++m) {
+      double acc = 0;
+      for(int kk = j; kk <= M; ++kk) {
+        double binom = 1;
+        for(int t = 0; t < j; ++t) binom = binom * (kk - t) / (t + 1);
+        acc += binom * std::ldexp(1.0, -(kk - j)) * vinv[kk * (kMaxOrder + 1) + m];
+      }
+      vinv_c[j * (kMaxOrder + 1) + m] = acc;
+    }
+}
+
 void drop_graph(quest_ctx *c)
 {
   if(c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
@@ -459,9 +476,11 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
     k.sin_c[2] = t * t2 * t2 / 120.0;
     k.cos_c[0] = -t2 / 2.0;
     k.cos_c[1] = t2 * t2 / 24.0;
-    k.cos_c[2] = -t2 * t2 * t2 / 720.0;
   }
   lagrange_monomials(p->interp_order, k.vinv);
+  centre_basis(p->interp_order, k.vinv, k.vinv_c);
+  k.half_phase_re = std::cos(-0.5 * k.theta);
+  k.half_phase_im = std::sin(-0.5 * k.theta);
   k.dyadic = p->dyadic;
   k.n_systems = n_sys;
   // FP32 positions are stored relative to the centre of all systems' boxes,

@@ -40,13 +40,16 @@ struct Consts {
   // divided by f32_scale = two_idt2 / L (which the FP32 kernel applies once
   // per target, in FP64): cc / L^3, oo / L, oc / L^2, m_cdt / L^2, m_o2dt / L
   double cc_l, oo_l, oc_l, m_cdt_l, m_o2dt_l, f32_scale;
-  // FP32 sin/cos of theta x: theta, -theta^3/6, theta^5/120 and
-  // -theta^2/2, theta^4/24, -theta^6/720
-  double sin_c[3], cos_c[3];
+  // FP32 sin/cos of theta x, |x| <= 1/2: theta, -theta^3/6, theta^5/120 and
+  // -theta^2/2, theta^4/24
+  double sin_c[3], cos_c[2];
   // error bound of the FP32 delay estimate: eps = eps_rel tau + eps_abs
   double eps_rel, eps_abs;
   // monomial coefficients of the Lagrange basis: vinv[k][m] = [x^k] L_m(x)
   double vinv[(kMaxOrder + 1) * (kMaxOrder + 1)];
+  // the same about x = 1/2 (FP32 ring): vinv_c[j][m] = [(x - 1/2)^j] L_m(x)
+  double vinv_c[(kMaxOrder + 1) * (kMaxOrder + 1)];
+  double half_phase_re, half_phase_im;  // e^{-i theta / 2} (FP32 ring, rotating frame)
   int dyadic;         // QUEST_DYADIC_*
   int n_systems;      // independent dot systems (pairs only within a system)
   int n_work;         // pair-kernel work items (system, target tile, source chunk)

@@ -100,38 +100,39 @@ __device__ __forceinline__ int split_f64(double dx, double dy, double dz, double
 // and then the exact FP64 split decides.  The integer delays are therefore the
 // reference's in both precisions, and the pair kernel and the zero-delay list
 // builder classify every pair alike.  half_m_eps_abs = 1/2 - eps_abs.
-// Returns n as an integer-valued float.
+// Returns n as an integer-valued float and the fraction centred on 1/2,
+// xc = x - 1/2 in [-1/2, 1/2) (the FP32 interpolant's variable).
 __device__ __forceinline__ float split_mixed(float dx, float dy, float dz, float eps_rel,
                                            float half_m_eps_abs, const double4 &pa64,
                                            const double4 *__restrict__ pos64, int b, double cdt,
-                                           double icdt64, float &x, float &ir)
+                                           double icdt64, float &xc, float &ir)
 {
   float r2;
   const float tau = tau_f32(dx, dy, dz, r2, ir);
   const float fl = __builtin_floorf(tau);
   const float fr = tau - fl;
   const float lim = __builtin_fmaf(-eps_rel, tau, half_m_eps_abs);
-  if(__builtin_expect(__builtin_fabsf(fr - 0.5f) > lim, 0)) {
+  xc = fr - 0.5f;  // exact (Sterbenz, or fr a multiple of tau's ulp below 1/2)
+  if(__builtin_expect(__builtin_fabsf(xc) > lim, 0)) {
     const double4 pb = pos64[b];
     double xd, ird, r2d;
     const int n = split_f64(pa64.x - pb.x, pa64.y - pb.y, pa64.z - pb.z, cdt, icdt64, xd, ird, r2d);
-    x = xd == xd ? static_cast<float>(xd) : 0.0f;  // coincident dots: n = 0, finite x
+    xc = xd == xd ? static_cast<float>(xd - 0.5) : 0.0f;  // coincident dots: n = 0, finite xc
     return static_cast<float>(n);
   }
-  x = fr;
   return fl;  // tau is finite and >= 0
 }
 
-// sin and cos of theta x for x in [0, 1), theta x < 0.23: Taylor series in
-// x^2 with theta folded into the coefficients (sn[] = theta, -theta^3/6,
-// theta^5/120; cs[] = -theta^2/2, theta^4/24, -theta^6/720), truncated where
-// the next term is below FP32 rounding.
+// sin and cos of theta x for x in [-1/2, 1/2), |theta x| <= 0.114: Taylor
+// series in x^2 with theta folded into the coefficients (sn[] = theta,
+// -theta^3/6, theta^5/120; cs[] = -theta^2/2, theta^4/24), truncated where
+// the next term (u^7/5040 <= 4e-11, u^6/720 <= 3e-9) is below FP32 rounding.
 __device__ __forceinline__ void sincos_theta(const float *sn, const float *cs, float x, float &s,
                                              float &c)
 {
   const float x2 = x * x;
   s = x * __builtin_fmaf(x2, __builtin_fmaf(x2, sn[2], sn[1]), sn[0]);
-  c = __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, cs[2], cs[1]), cs[0]), 1.0f);
+  c = __builtin_fmaf(x2, __builtin_fmaf(x2, cs[1], cs[0]), 1.0f);
 }
 // sin and cos of u in [0, 0.23): Taylor series (Horner in u^2), truncated
 // where the next term is below FP64 rounding.
@@ -159,9 +160,10 @@ __device__ __forceinline__ double fmaT(double a, double b, double c) { return __
 
 // Interpolant ring.  Entry e = a * ring_slots + slot holds the M + 1
 // coefficients of source a for one slot; E = n * ring_slots entries.
-//   FP64: one record per entry                    double2 [E][M + 1]
-//   FP32: (M + 1) / 2 planes of two coefficients  float4  [(M + 1) / 2][E],
-//         then, for even M, one plane of c_M      float2  [E]
+//   FP64: one record per entry, monomials in x     double2 [E][M + 1]
+//   FP32: monomials in xc = x - 1/2, in (M + 1) / 2 planes of two coefficients
+//         float4 [(M + 1) / 2][E], then, for even M, one plane of c_M
+//         float2 [E]
 // A wave's lanes read one source at the ~30 consecutive slots its targets'
 // delays span.  In the FP32 planes each load instruction touches a short
 // contiguous run of 16-B aligned entries (measured 3.5% faster than 40-B
@@ -233,14 +235,14 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
 template <typename R> struct PairK {
   static constexpr bool kF32 = sizeof(R) == sizeof(float);
   R cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, half_m_eps_abs;
-  R sn[3], cs[3];  // sincos_theta coefficients (FP32)
+  R sn[3], cs[2];  // sincos_theta coefficients (FP32)
   double cdt, icdt64;
   __host__ __device__ explicit PairK(const Consts &k)
       : cc((R)(kF32 ? k.cc_l : k.cc)), oo((R)(kF32 ? k.oo_l : k.oo)), oc((R)(kF32 ? k.oc_l : k.oc)),
         m_cdt((R)(kF32 ? k.m_cdt_l : k.m_cdt)), m_o2dt((R)(kF32 ? k.m_o2dt_l : k.m_o2dt)),
         two_idt2((R)(kF32 ? 1.0 : k.two_idt2)), theta((R)k.theta), eps_rel((R)k.eps_rel),
         half_m_eps_abs((R)(0.5 - k.eps_abs)),
-        sn{(R)k.sin_c[0], (R)k.sin_c[1], (R)k.sin_c[2]}, cs{(R)k.cos_c[0], (R)k.cos_c[1], (R)k.cos_c[2]},
+        sn{(R)k.sin_c[0], (R)k.sin_c[1], (R)k.sin_c[2]}, cs{(R)k.cos_c[0], (R)k.cos_c[1]},
         cdt(k.cdt), icdt64(k.icdt) {}
 };
 
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
       for(int b = bt; b < be; ++b) {
         const float4 pb = pos[b], db = dip[b];
         const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float x, ir;
+        float x, ir;  // x: the fraction centred on 1/2 (the FP32 ring's variable)
         const float fn = split_mixed(dx, dy, dz, K.eps_rel, K.half_m_eps_abs, pa64, pos64, b, K.cdt,
                                      K.icdt64, x, ir);
         const int mn = static_cast<int>(-fn);  // -n
@@ -570,13 +572,18 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
   const bool lab = k.dyadic == QUEST_DYADIC_LAB;
   if(lab)
     for(int m = 0; m <= k.order; ++m) smp[m] = make_double2(2.0 * smp[m].x, 0.0);
-  const double2 ph = lab ? make_double2(1.0, 0.0) : phase[s];
+  // FP32 ring: coefficients about x = 1/2, so the rotation e^{-i theta x} of
+  // the pair kernel splits into e^{-i theta / 2} (here) and e^{-i theta xc}
+  constexpr bool f32 = sizeof(R2) == sizeof(float2);
+  const double *basis = f32 ? k.vinv_c : k.vinv;
+  const double2 ph = lab ? make_double2(1.0, 0.0)
+                         : (f32 ? cmul(phase[s], make_double2(k.half_phase_re, k.half_phase_im)) : phase[s]);
   const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
   const size_t e = static_cast<size_t>(a) * k.ring_slots + (s & k.ring_mask);
   for(int kk = 0; kk <= k.order; ++kk) {
     double2 c = make_double2(0.0, 0.0);
     for(int m = 0; m <= k.order; ++m) {
-      const double v = k.vinv[kk * (kMaxOrder + 1) + m];
+      const double v = basis[kk * (kMaxOrder + 1) + m];
       c.x += v * smp[m].x;
       c.y += v * smp[m].y;
     }

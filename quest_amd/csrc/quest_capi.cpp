@@ -169,22 +169,28 @@ void drop_graph(quest_ctx *c)
   c->graph = nullptr;
 }
 
+// Work items (one per wave) of the pair kernel for a given chunk length.
 int count_work(const quest_ctx *c, int chunk_len)
 {
   int64_t items = 0;
   for(int s = 0; s < c->n_sys; ++s) {
     const int ns = c->sys_begin[s + 1] - c->sys_begin[s];
     if(ns < 2) continue;
-    items += static_cast<int64_t>((ns + kBlock - 1) / kBlock) * ((ns + chunk_len - 1) / chunk_len);
+    items += static_cast<int64_t>((ns + kWave - 1) / kWave) * ((ns + chunk_len - 1) / chunk_len);
   }
   return static_cast<int>(std::min<int64_t>(items, INT_MAX));
 }
 
 // Work list of the pair kernel for a given source-chunk length: for every
-// system, every target tile of kBlock dots, every chunk of chunk_len sources
-// (chunks of a tile consecutive, see pair_past_*).
+// system, every target tile of one wave (kWave dots) and every chunk of
+// chunk_len sources.  Items are ordered so that the kBlock / kWave items a
+// block takes together are consecutive wave tiles with the same chunk, and
+// the chunks of a group of tiles consecutive (see pair_past_*): for systems
+// of at least kBlock dots a block is a kBlock-target tile, as with one item
+// per block, while small systems leave at most a wave's worth of lanes idle.
 int build_work(quest_ctx *c, int chunk_len)
 {
+  constexpr int kPer = kBlock / kWave;
   std::vector<int4> work;
   std::vector<int> row;
   std::vector<int> chunks(c->n_sys, 0);
@@ -192,15 +198,16 @@ int build_work(quest_ctx *c, int chunk_len)
   for(int s = 0; s < c->n_sys; ++s) {
     const int b = c->sys_begin[s], e = c->sys_begin[s + 1], ns = e - b;
     if(ns < 2) continue;
-    const int tiles = (ns + kBlock - 1) / kBlock, nch = (ns + chunk_len - 1) / chunk_len;
+    const int wtiles = (ns + kWave - 1) / kWave, nch = (ns + chunk_len - 1) / chunk_len;
     chunks[s] = nch;
     max_chunks = std::max(max_chunks, nch);
-    for(int t = 0; t < tiles; ++t)
-      for(int q = 0; q < nch; ++q) {
-        work.push_back(make_int4(b + t * kBlock, std::min(e, b + (t + 1) * kBlock), b + q * chunk_len,
-                                 std::min(e, b + (q + 1) * chunk_len)));
-        row.push_back(q);
-      }
+    for(int g = 0; g < wtiles; g += kPer)
+      for(int q = 0; q < nch; ++q)
+        for(int t = g; t < std::min(wtiles, g + kPer); ++t) {
+          work.push_back(make_int4(b + t * kWave, std::min(e, b + (t + 1) * kWave), b + q * chunk_len,
+                                   std::min(e, b + (q + 1) * chunk_len)));
+          row.push_back(q);
+        }
   }
   DeviceState &d = c->d;
   drop_graph(c);  // the graph's pair kernel holds the old list
@@ -227,10 +234,10 @@ int build_work(quest_ctx *c, int chunk_len)
   c->k.n_work = static_cast<int>(work.size());
   c->k.max_chunks = max_chunks;
   // persistent grid: one full wave of resident blocks (a multiple of 8, see
-  // pair_past_*), at most the item count rounded up to 8
+  // pair_past_*), at most the blocks the items need, rounded up to 8
   const int resident = std::max(8, c->cu_count * c->blocks_per_cu / 8 * 8);
-  const int items8 = (c->k.n_work + 7) / 8 * 8;
-  c->k.grid = std::max(8, std::min(resident, items8));
+  const int blocks8 = ((c->k.n_work + kPer - 1) / kPer + 7) / 8 * 8;
+  c->k.grid = std::max(8, std::min(resident, blocks8));
   return QUEST_OK;
 }
 
@@ -241,7 +248,7 @@ int default_chunk(const quest_ctx *c)
   int nmax = 2;
   for(int s = 0; s < c->n_sys; ++s) nmax = std::max(nmax, c->sys_begin[s + 1] - c->sys_begin[s]);
   int len = std::max(64, (nmax + 7) / 8);
-  const int resident = c->cu_count * c->blocks_per_cu;
+  const int resident = c->cu_count * c->blocks_per_cu * (kBlock / kWave);  // waves
   while(len > 64 && count_work(c, len) < resident) len = std::max(64, (len + 1) / 2);
   return len;
 }

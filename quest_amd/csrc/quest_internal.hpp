@@ -15,6 +15,7 @@ namespace quest {
 constexpr int kMaxOrder = 6;       // interpolation orders 1..6 are instantiated
 constexpr int kMaxWindow = 64;     // PC history window limit
 constexpr int kBlock = 256;        // threads per block of the per-dot and pair kernels
+constexpr int kWave = 64;          // wavefront size; a pair-kernel work item is one wave of targets
 constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
 
 int pc_weights(int n_lambda, int n_time, double radius, double *ps, double *cs,

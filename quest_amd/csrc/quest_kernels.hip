@@ -296,11 +296,20 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
 // once per step; the zero-delay pairs are evaluated at every corrector
 // iteration from the near list.
 //
-// Persistent kernel over a work list: item w = (target tile of kBlock dots,
-// source range) of one system; block i takes items i, i + grid, ...  Items are
-// ordered tile-major with the source chunks of a tile consecutive, and `grid`
-// is a multiple of 8, so with 8 chunks per tile each XCD (round-robin block
-// dispatch) walks one source range and its L2 holds that range's ring rows.
+// Persistent kernel over a work list: item w = (target tile of one wave of
+// dots, source range) of one system; wave j of block i takes items
+// 4 i + j, 4 (i + grid) + j, ...  A block's four consecutive items share a
+// source chunk (for systems of >= 256 dots: one 256-target tile), the chunks
+// of a group of tiles are consecutive, and `grid` is a multiple of 8, so with
+// 8 chunks per tile group each XCD (round-robin block dispatch) walks one
+// source range and its L2 holds that range's ring rows.
+__device__ __forceinline__ int first_item()
+{
+  return static_cast<int>(blockIdx.x) * (kBlock / kWave) +
+         __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+}
+__device__ __forceinline__ int items_stride() { return static_cast<int>(gridDim.x) * (kBlock / kWave); }
+
 constexpr int kChunk = 64;
 #ifndef QUEST_F64_ATTR
 #define QUEST_F64_ATTR
@@ -314,9 +323,9 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
 {
   const PairK<double> K(k);
   const int step = *step_ptr;
-  for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
+  for(int w = first_item(); w < k.n_work; w += items_stride()) {
     const int4 wi = work[w];
-    const int tgt = wi.x + static_cast<int>(threadIdx.x);
+    const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
     const double4 pa = pos[a], da = dip[a];
     double accr = 0, acci = 0;
@@ -363,9 +372,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
   const unsigned mask8 = static_cast<unsigned>(k.ring_mask) << 3;
   const float4 *ring2 = reinterpret_cast<const float4 *>(ring);
   const float2 *ring1 = ring + 2 * kPairs * E;
-  for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
+  for(int w = first_item(); w < k.n_work; w += items_stride()) {
     const int4 wi = work[w];
-    const int tgt = wi.x + static_cast<int>(threadIdx.x);
+    const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
     const float4 pa = pos[a], da = dip[a];
     const double4 pa64 = pos64[a];
@@ -427,12 +436,12 @@ __global__ __launch_bounds__(kBlock) void pair_table_f64(
 {
   const int step = *step_ptr;
   const size_t P = static_cast<size_t>(k.table_plane);
-  for(int w = blockIdx.x; w < k.n_work; w += gridDim.x) {
+  for(int w = first_item(); w < k.n_work; w += items_stride()) {
     const int4 wi = work[w];
     const int s = sys_of[wi.x];
     const int2 rg = sys_range[s];
     const size_t ns = static_cast<size_t>(rg.y - rg.x);
-    const int tgt = wi.x + static_cast<int>(threadIdx.x);
+    const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
     const size_t base = static_cast<size_t>(sys_toff[s]) + static_cast<size_t>(a - rg.x);
     double accr = 0, acci = 0;

@@ -220,10 +220,12 @@ int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
  * reference, which has no tunable kernel. */
 int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);
 
-/* Integer facts about the context.  info: [12] int64 = n_dots, ring slots,
+/* Integer facts about the context.  info: [14] int64 = n_dots, ring slots,
  * delay bound, zero-delay pairs, work items, persistent grid, state slots,
  * precision, systems, source-chunk length, chunks of the largest system,
- * step graph captured (0/1). */
+ * step graph captured (0/1), block size of the one-launch predictor-corrector
+ * step (0: one launch per iteration, used when a connected group of
+ * zero-delay partners exceeds 256 dots), size of the largest such group. */
 int quest_ctx_info(const quest_ctx *ctx, int64_t *info);
 
 /* Blocks until all work queued on the context's stream is done. */

@@ -286,10 +286,11 @@ class Context:
         return ms.value, cnt.value
 
     def info(self) -> dict:
-        a = (C.c_int64 * 12)()
+        a = (C.c_int64 * 14)()
         self._check(self._L.quest_ctx_info(self.ptr, a))
         keys = ["n", "ring_slots", "max_delay_bound", "near_pairs", "work_items", "grid",
-                "state_slots", "precision", "systems", "source_chunk", "splits", "graph"]
+                "state_slots", "precision", "systems", "source_chunk", "splits", "graph",
+                "step_block", "max_group"]
         return dict(zip(keys, list(a)))
 
     def sync(self) -> None:

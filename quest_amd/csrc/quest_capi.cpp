@@ -56,6 +56,9 @@ struct quest_ctx {
   int cu_count = 0;
   int blocks_per_cu = 1;
   int pair_method = QUEST_PAIR_ON_THE_FLY;
+  int step_block = 0;          // block size of the fused step kernel (0: per-iteration kernels)
+  int step_blocks = 0;         // its grid
+  int max_group = 0;           // largest connected group of zero-delay partners
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
 };
@@ -351,9 +354,14 @@ int enqueue_step(quest_ctx *c)
 {
   int rc = pair_sum(c);
   if(rc != QUEST_OK) return rc;
-  HIPC(c, launch_pc_begin(c->k, c->d, c->P.history_window, c->stream));
-  for(int it = 0; it <= kNumCorrectorSteps; ++it)
-    HIPC(c, launch_pc_iter(c->k, c->d, c->P.precision, c->fc, it, c->stream));
+  if(c->step_block > 0) {
+    HIPC(c, launch_pc_step(c->k, c->d, c->P.precision, c->fc, c->P.history_window, c->step_block,
+                           c->step_blocks, c->stream));
+  } else {
+    HIPC(c, launch_pc_begin(c->k, c->d, c->P.history_window, c->stream));
+    for(int it = 0; it <= kNumCorrectorSteps; ++it)
+      HIPC(c, launch_pc_iter(c->k, c->d, c->P.precision, c->fc, it, c->stream));
+  }
   HIPC(c, launch_advance_step(c->d, c->stream));
   return QUEST_OK;
 }
@@ -388,7 +396,8 @@ void free_device(quest_ctx *c)
                   d.d0, d.d1, d.cur0, d.cur1, d.yc0, d.yc1, d.field, d.partial, d.phase, d.out,
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.sys_of, d.sys_range, d.sys_chunks,
-                  d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff};
+                  d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff,
+                  d.step_map};
   for(void *p : ptrs)
     if(p) (void)hipFree(p);
   d = DeviceState{};
@@ -398,6 +407,67 @@ void free_device(quest_ctx *c)
   c->ev_pool.clear();
   if(c->stream) (void)hipStreamDestroy(c->stream);
   c->stream = nullptr;
+}
+
+// Packs the dots into blocks of the fused step kernel so that every
+// connected group of zero-delay partners (the only coupling inside a step)
+// lies in one block.  Groups are placed whole, in order of their first dot,
+// members in ascending order (so the map stays close to the identity and the
+// per-dot loads stay coalesced); a group that does not fit the rest of a
+// block starts the next one.  The block is the smallest of 64, 128, 256
+// threads that holds the largest group; a larger group leaves the
+// per-iteration kernels in charge (step_block = 0).
+int pack_step_blocks(quest_ctx *c, const std::vector<int> &start, const std::vector<int> &near_src)
+{
+  const int n = c->n;
+  std::vector<int> root(n);
+  for(int i = 0; i < n; ++i) root[i] = i;
+  auto find = [&root](int x) {
+    while(root[x] != x) {
+      root[x] = root[root[x]];
+      x = root[x];
+    }
+    return x;
+  };
+  for(int a = 0; a < n; ++a)
+    for(int e = start[a]; e < start[a + 1]; ++e) {
+      const int b = near_src[e];
+      if(b < 0 || b >= n) return fail(c, QUEST_ERR_HIP, "zero-delay list holds an invalid source index");
+      const int ra = find(a), rb = find(b);
+      if(ra != rb) root[std::max(ra, rb)] = std::min(ra, rb);  // the group's root is its first dot
+    }
+  std::vector<int> size(n, 0), first(n + 1, 0), members(n);
+  for(int i = 0; i < n; ++i) ++size[find(i)];
+  for(int i = 0; i < n; ++i) first[i + 1] = first[i] + size[i];
+  std::vector<int> fill(first.begin(), first.end() - 1);
+  for(int i = 0; i < n; ++i) members[fill[find(i)]++] = i;
+  c->max_group = n > 0 ? *std::max_element(size.begin(), size.end()) : 0;
+  c->step_block = 0;
+  c->step_blocks = 0;
+  int block = 0;
+  for(int b : {64, 128, kBlock})
+    if(c->max_group <= b) {
+      block = b;
+      break;
+    }
+  if(block == 0 || n == 0) return QUEST_OK;
+  std::vector<int> map;
+  map.reserve(static_cast<size_t>(n) + n / 4 + block);
+  for(int r = 0; r < n; ++r) {
+    if(size[r] == 0) continue;
+    const size_t used = map.size() % block;
+    if(used != 0 && used + size[r] > static_cast<size_t>(block)) map.resize(map.size() + block - used, -1);
+    map.insert(map.end(), members.begin() + first[r], members.begin() + first[r] + size[r]);
+  }
+  if(map.size() % block) map.resize(map.size() + block - map.size() % block, -1);
+  const size_t blocks = map.size() / block;
+  DeviceState &d = c->d;
+  HIPC(c, hipMalloc(reinterpret_cast<void **>(&d.step_map), sizeof(int) * map.size()));
+  HIPC(c, hipMemcpyAsync(d.step_map, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->step_block = block;
+  c->step_blocks = static_cast<int>(blocks);
+  return QUEST_OK;
 }
 
 int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, const double *pos,
@@ -605,7 +675,11 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, hipMemcpyAsync(d.near_start, start.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice, c->stream));
   HIPC(c, launch_near_fill(k, d, p->precision, c->stream));
   HIPC(c, launch_near_coef(k, d, c->near_total, c->stream));
+  std::vector<int> near_src(ne);
+  HIPC(c, hipMemcpyAsync(near_src.data(), d.near_src, sizeof(int) * ne, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
+  rc = pack_step_blocks(c, start, near_src);
+  if(rc != QUEST_OK) return rc;
   rc = ensure_phase(c, 1024);
   if(rc != QUEST_OK) return rc;
   return start_state(c);
@@ -1119,6 +1193,8 @@ int quest_ctx_info(const quest_ctx *c, int64_t *info)
   info[9] = c->chunk_len;
   info[10] = c->k.max_chunks;
   info[11] = c->graph_exec != nullptr;
+  info[12] = c->step_block;
+  info[13] = c->max_group;
   return QUEST_OK;
 }
 

@@ -104,6 +104,8 @@ struct DeviceState {
                                        // sys_toff[s] + (b - begin_s) n_s + (a - begin_s)
   int16_t *table_n = nullptr;          // [table_plane] integer delay (0 for excluded pairs)
   long long *sys_toff = nullptr;       // [systems] first table entry of each system
+  // fused predictor-corrector step: thread -> dot (-1 idle), zero-delay groups within a block
+  int *step_map = nullptr;
 };
 
 // ---------------------------------------------------------------- launchers
@@ -119,6 +121,10 @@ hipError_t launch_set_step(DeviceState &d, int step, hipStream_t s);
 hipError_t launch_advance_step(DeviceState &d, hipStream_t s);
 hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_pc_begin(const Consts &k, DeviceState &d, int window, hipStream_t s);
+// The whole step of every dot (predictor + 11 iterations) in one launch of
+// `blocks` blocks of `block` (<= kBlock) threads over d.step_map.
+hipError_t launch_pc_step(const Consts &k, DeviceState &d, int precision, double future_coef, int window,
+                          int block, int blocks, hipStream_t s);
 hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, double future_coef,
                           int iter, hipStream_t s);
 hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s);

@@ -628,10 +628,8 @@ __global__ void clear_words_kernel(uint4 *p, size_t n16)
 // Start of a step (whole-run mode): fixed field (pulse + past pair sum), the
 // predictor (integrator.cpp:142-154) and the step-invariant part of the
 // corrector (integrator.cpp:156-169).
-__global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState d, int window)
+__device__ __forceinline__ void pc_begin_dot(const Consts &k, const DeviceState &d, int window, int a)
 {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if(a >= k.n) return;
   const int step = *d.step;
   const double dt = k.dt;
   double2 hist = make_double2(0.0, 0.0);
@@ -662,16 +660,20 @@ __global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState 
   d.yc1[a] = c1;
 }
 
+__global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState d, int window)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a < k.n) pc_begin_dot(k, d, window, a);
+}
+
 // One evaluation of the step (integrator.cpp:171-185) and the next corrector
 // update (integrator.cpp:156-169) for iterations 0..9; iteration 10 commits
 // the step to the state ring, checks finiteness (integrator.cpp:187-207) and
 // writes the source's interpolant slot for the next steps' pair sums.
 template <typename R2>
-__global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d, R2 *ring,
-                                                         double future_coef, int iter)
+__device__ __forceinline__ void pc_iter_dot(const Consts &k, const DeviceState &d, R2 *ring,
+                                            double future_coef, int iter, int a)
 {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if(a >= k.n) return;
   const int step = *d.step;
   const int rd = iter & 1, wr = rd ^ 1;
   const double2 *cur0 = d.cur0 + static_cast<size_t>(rd) * k.n;
@@ -694,6 +696,33 @@ __global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d
   d.d1[o] = m1;
   if(!(finite2(y0) && finite2(y1) && finite2(m0) && finite2(m1))) atomicMin(d.bad_step, step);
   build_slot(k, d.phase, ring, d.samp, d.y1, y1, a, step);
+}
+
+template <typename R2>
+__global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d, R2 *ring,
+                                                         double future_coef, int iter)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a < k.n) pc_iter_dot(k, d, ring, future_coef, iter, a);
+}
+
+// The whole predictor-corrector step of every dot in one launch: predictor,
+// then 11 x (evaluate, correct or commit).  Between iterations a dot reads
+// only its own values and, through the zero-delay list, its partners'
+// iterates; the host packs dots (step_map: thread -> dot, -1 idle) so that
+// every connected group of zero-delay partners sits in one block, and the
+// block barrier (a workgroup-scope fence on gfx950) orders the iterations.
+template <typename R2>
+__global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d, R2 *ring,
+                                                         double future_coef, int window)
+{
+  const int a = d.step_map[blockIdx.x * blockDim.x + threadIdx.x];
+  if(a >= 0) pc_begin_dot(k, d, window, a);
+  __syncthreads();
+  for(int iter = 0; iter <= kNumCorrectorSteps; ++iter) {
+    if(a >= 0) pc_iter_dot(k, d, ring, future_coef, iter, a);
+    __syncthreads();
+  }
 }
 
 // Per-call mode: the values passed for `step` become final -- state ring
@@ -1054,6 +1083,18 @@ int pair_blocks_per_cu(const Consts &k, int precision)
   case 5: return blocks_per_cu_m<5>(precision, t);
   default: return blocks_per_cu_m<6>(precision, t);
   }
+}
+
+hipError_t launch_pc_step(const Consts &k, DeviceState &d, int precision, double future_coef, int window,
+                          int block, int blocks, hipStream_t s)
+{
+  if(precision == QUEST_PREC_FP64)
+    hipLaunchKernelGGL(pc_step_kernel<double2>, dim3(blocks), dim3(block), 0, s, k, d,
+                       static_cast<double2 *>(d.ring), future_coef, window);
+  else
+    hipLaunchKernelGGL(pc_step_kernel<float2>, dim3(blocks), dim3(block), 0, s, k, d,
+                       static_cast<float2 *>(d.ring), future_coef, window);
+  return hipGetLastError();
 }
 
 hipError_t launch_pc_begin(const Consts &k, DeviceState &d, int window, hipStream_t s)

@@ -58,6 +58,7 @@ struct quest_ctx {
   int pair_method = QUEST_PAIR_ON_THE_FLY;
   int step_block = 0;          // block size of the fused step kernel (0: per-iteration kernels)
   int step_blocks = 0;         // its grid
+  int step_sync_blocks = 0;    // leading blocks holding the zero-delay groups
   int max_group = 0;           // largest connected group of zero-delay partners
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -356,7 +357,7 @@ int enqueue_step(quest_ctx *c)
   if(rc != QUEST_OK) return rc;
   if(c->step_block > 0) {
     HIPC(c, launch_pc_step(c->k, c->d, c->P.precision, c->fc, c->P.history_window, c->step_block,
-                           c->step_blocks, c->stream));
+                           c->step_blocks, c->step_sync_blocks, c->stream));
   } else {
     HIPC(c, launch_pc_begin(c->k, c->d, c->P.history_window, c->stream));
     for(int it = 0; it <= kNumCorrectorSteps; ++it)
@@ -409,14 +410,16 @@ void free_device(quest_ctx *c)
   c->stream = nullptr;
 }
 
-// Packs the dots into blocks of the fused step kernel so that every
+// Packs the dots into blocks of the one-launch step kernel so that every
 // connected group of zero-delay partners (the only coupling inside a step)
-// lies in one block.  Groups are placed whole, in order of their first dot,
-// members in ascending order (so the map stays close to the identity and the
-// per-dot loads stay coalesced); a group that does not fit the rest of a
-// block starts the next one.  The block is the smallest of 64, 128, 256
-// threads that holds the largest group; a larger group leaves the
-// per-iteration kernels in charge (step_block = 0).
+// lies in one block.  Groups of two or more come first, whole, in order of
+// their first dot, members ascending; a group that does not fit the rest of a
+// block starts the next one.  Those blocks (step_sync_blocks) synchronise
+// between iterations.  The dots without partners follow in ascending order
+// (so the map stays close to the identity and the per-dot loads coalesce).
+// The block is the smallest of 64, 128, 256 threads that holds the largest
+// group; a larger group leaves the per-iteration kernels in charge
+// (step_block = 0).
 int pack_step_blocks(quest_ctx *c, const std::vector<int> &start, const std::vector<int> &near_src)
 {
   const int n = c->n;
@@ -454,11 +457,14 @@ int pack_step_blocks(quest_ctx *c, const std::vector<int> &start, const std::vec
   std::vector<int> map;
   map.reserve(static_cast<size_t>(n) + n / 4 + block);
   for(int r = 0; r < n; ++r) {
-    if(size[r] == 0) continue;
+    if(size[r] < 2) continue;
     const size_t used = map.size() % block;
     if(used != 0 && used + size[r] > static_cast<size_t>(block)) map.resize(map.size() + block - used, -1);
     map.insert(map.end(), members.begin() + first[r], members.begin() + first[r] + size[r]);
   }
+  const int sync_blocks = static_cast<int>((map.size() + block - 1) / block);
+  for(int r = 0; r < n; ++r)
+    if(size[r] == 1) map.push_back(r);
   if(map.size() % block) map.resize(map.size() + block - map.size() % block, -1);
   const size_t blocks = map.size() / block;
   DeviceState &d = c->d;
@@ -467,6 +473,7 @@ int pack_step_blocks(quest_ctx *c, const std::vector<int> &start, const std::vec
   HIPC(c, hipStreamSynchronize(c->stream));
   c->step_block = block;
   c->step_blocks = static_cast<int>(blocks);
+  c->step_sync_blocks = sync_blocks;
   return QUEST_OK;
 }
 

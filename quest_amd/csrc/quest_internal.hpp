@@ -104,7 +104,8 @@ struct DeviceState {
                                        // sys_toff[s] + (b - begin_s) n_s + (a - begin_s)
   int16_t *table_n = nullptr;          // [table_plane] integer delay (0 for excluded pairs)
   long long *sys_toff = nullptr;       // [systems] first table entry of each system
-  // fused predictor-corrector step: thread -> dot (-1 idle), zero-delay groups within a block
+  // one-launch predictor-corrector step: thread -> dot (-1 idle), each group of
+  // zero-delay partners within one block, all groups in the leading blocks
   int *step_map = nullptr;
 };
 
@@ -122,9 +123,10 @@ hipError_t launch_advance_step(DeviceState &d, hipStream_t s);
 hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_pc_begin(const Consts &k, DeviceState &d, int window, hipStream_t s);
 // The whole step of every dot (predictor + 11 iterations) in one launch of
-// `blocks` blocks of `block` (<= kBlock) threads over d.step_map.
+// `blocks` blocks of `block` (<= kBlock) threads over d.step_map; the first
+// `sync_blocks` blocks hold every dot with zero-delay partners.
 hipError_t launch_pc_step(const Consts &k, DeviceState &d, int precision, double future_coef, int window,
-                          int block, int blocks, hipStream_t s);
+                          int block, int blocks, int sync_blocks, hipStream_t s);
 hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, double future_coef,
                           int iter, hipStream_t s);
 hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s);

@@ -575,12 +575,16 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
                                            int s)
 {
   if(samp) samp[static_cast<size_t>(a) * k.samp_slots + (s & k.samp_mask)] = rho_s;
+  // loops unrolled to the largest order (guards on k.order) keep smp in registers
   double2 smp[kMaxOrder + 1];
   smp[0] = rho_s;
-  for(int m = 1; m <= k.order; ++m) smp[m] = y1[static_cast<size_t>((s - m) & k.state_mask) * k.n + a];
+#pragma unroll
+  for(int m = 1; m <= kMaxOrder; ++m)
+    smp[m] = m <= k.order ? y1[static_cast<size_t>((s - m) & k.state_mask) * k.n + a] : make_double2(0.0, 0.0);
   const bool lab = k.dyadic == QUEST_DYADIC_LAB;
   if(lab)
-    for(int m = 0; m <= k.order; ++m) smp[m] = make_double2(2.0 * smp[m].x, 0.0);
+#pragma unroll
+    for(int m = 0; m <= kMaxOrder; ++m) smp[m] = make_double2(2.0 * smp[m].x, 0.0);
   // FP32 ring: coefficients about x = 1/2, so the rotation e^{-i theta x} of
   // the pair kernel splits into e^{-i theta / 2} (here) and e^{-i theta xc}
   constexpr bool f32 = sizeof(R2) == sizeof(float2);
@@ -591,7 +595,9 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
   const size_t e = static_cast<size_t>(a) * k.ring_slots + (s & k.ring_mask);
   for(int kk = 0; kk <= k.order; ++kk) {
     double2 c = make_double2(0.0, 0.0);
-    for(int m = 0; m <= k.order; ++m) {
+#pragma unroll
+    for(int m = 0; m <= kMaxOrder; ++m) {
+      if(m > k.order) break;
       const double v = basis[kk * (kMaxOrder + 1) + m];
       c.x += v * smp[m].x;
       c.y += v * smp[m].y;
@@ -628,19 +634,23 @@ __global__ void clear_words_kernel(uint4 *p, size_t n16)
 // Start of a step (whole-run mode): fixed field (pulse + past pair sum), the
 // predictor (integrator.cpp:142-154) and the step-invariant part of the
 // corrector (integrator.cpp:156-169).
-__device__ __forceinline__ void pc_begin_dot(const Consts &k, const DeviceState &d, int window, int a)
+__device__ __forceinline__ void pc_predict(const Consts &k, const DeviceState &d, int window, int a, int step,
+                                           double2 &field, double2 &p0, double2 &p1, double2 &c0, double2 &c1)
 {
-  const int step = *d.step;
   const double dt = k.dt;
   double2 hist = make_double2(0.0, 0.0);
   if(k.n_work > 0) hist = pair_field(k, d, a, step);
   const double2 pulse = pulse_sum(k, d, a, step * dt);
-  d.field[a] = cadd(pulse, hist);
+  field = cadd(pulse, hist);
 
   const double *ps0 = d.pcw, *ps1 = d.pcw + window;
   const double *cs0 = d.pcw + 2 * window, *cs1 = d.pcw + 3 * window;
-  double2 p0 = make_double2(0.0, 0.0), p1 = p0, c0 = p0, c1 = p0;
+  p0 = make_double2(0.0, 0.0);
+  p1 = p0;
+  c0 = p0;
+  c1 = p0;
   const int start = step - window;
+#pragma unroll 4
   for(int h = 0; h < window; ++h) {
     const size_t o = static_cast<size_t>((start + h) & k.state_mask) * k.n + a;
     const double2 y0 = d.y0[o], y1 = d.y1[o], g0 = d.d0[o], g1 = d.d1[o];
@@ -654,6 +664,13 @@ __device__ __forceinline__ void pc_begin_dot(const Consts &k, const DeviceState 
     c1.x += y1.x * cw + g1.x * cd;
     c1.y += y1.y * cw + g1.y * cd;
   }
+}
+
+__device__ __forceinline__ void pc_begin_dot(const Consts &k, const DeviceState &d, int window, int a)
+{
+  double2 field, p0, p1, c0, c1;
+  pc_predict(k, d, window, a, *d.step, field, p0, p1, c0, c1);
+  d.field[a] = field;
   d.cur0[a] = p0;
   d.cur1[a] = p1;
   d.yc0[a] = c0;
@@ -670,6 +687,21 @@ __global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState 
 // update (integrator.cpp:156-169) for iterations 0..9; iteration 10 commits
 // the step to the state ring, checks finiteness (integrator.cpp:187-207) and
 // writes the source's interpolant slot for the next steps' pair sums.
+// The step's final iterate and derivative into the state ring; finiteness
+// check (integrator.cpp:187-207); the source's interpolant slot.
+template <typename R2>
+__device__ __forceinline__ void pc_commit(const Consts &k, const DeviceState &d, R2 *ring, int a, int step,
+                                          double2 y0, double2 y1, double2 m0, double2 m1)
+{
+  const size_t o = static_cast<size_t>(step & k.state_mask) * k.n + a;
+  d.y0[o] = y0;
+  d.y1[o] = y1;
+  d.d0[o] = m0;
+  d.d1[o] = m1;
+  if(!(finite2(y0) && finite2(y1) && finite2(m0) && finite2(m1))) atomicMin(d.bad_step, step);
+  build_slot(k, d.phase, ring, d.samp, d.y1, y1, a, step);
+}
+
 template <typename R2>
 __device__ __forceinline__ void pc_iter_dot(const Consts &k, const DeviceState &d, R2 *ring,
                                             double future_coef, int iter, int a)
@@ -689,13 +721,7 @@ __device__ __forceinline__ void pc_iter_dot(const Consts &k, const DeviceState &
     d.cur1[static_cast<size_t>(wr) * k.n + a] = make_double2(f * m1.x + d.yc1[a].x, f * m1.y + d.yc1[a].y);
     return;
   }
-  const size_t o = static_cast<size_t>(step & k.state_mask) * k.n + a;
-  d.y0[o] = y0;
-  d.y1[o] = y1;
-  d.d0[o] = m0;
-  d.d1[o] = m1;
-  if(!(finite2(y0) && finite2(y1) && finite2(m0) && finite2(m1))) atomicMin(d.bad_step, step);
-  build_slot(k, d.phase, ring, d.samp, d.y1, y1, a, step);
+  pc_commit(k, d, ring, a, step, y0, y1, m0, m1);
 }
 
 template <typename R2>
@@ -707,22 +733,50 @@ __global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d
 }
 
 // The whole predictor-corrector step of every dot in one launch: predictor,
-// then 11 x (evaluate, correct or commit).  Between iterations a dot reads
-// only its own values and, through the zero-delay list, its partners'
-// iterates; the host packs dots (step_map: thread -> dot, -1 idle) so that
-// every connected group of zero-delay partners sits in one block, and the
-// block barrier (a workgroup-scope fence on gfx950) orders the iterations.
+// then 11 x (evaluate, correct or commit), the same arithmetic as
+// pc_begin_kernel + pc_iter_kernel.  A dot's iterate, corrector base, fixed
+// field and parameters stay in registers.  The only coupling inside a step is
+// a zero-delay partner's current iterate: the host packs dots (step_map:
+// thread -> dot, -1 idle) so that every connected group of zero-delay
+// partners sits in one of the first `sync_blocks` blocks; there a dot with
+// partners publishes each iterate to the cur1 double buffer, and a block
+// barrier (a workgroup-scope fence on gfx950) separates the iterations.
+// Blocks of dots without partners run barrier-free.
 template <typename R2>
 __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d, R2 *ring,
-                                                         double future_coef, int window)
+                                                         double future_coef, int window, int sync_blocks)
 {
   const int a = d.step_map[blockIdx.x * blockDim.x + threadIdx.x];
-  if(a >= 0) pc_begin_dot(k, d, window, a);
-  __syncthreads();
-  for(int iter = 0; iter <= kNumCorrectorSteps; ++iter) {
-    if(a >= 0) pc_iter_dot(k, d, ring, future_coef, iter, a);
-    __syncthreads();
+  const bool sync = static_cast<int>(blockIdx.x) < sync_blocks;
+  const int step = *d.step;
+  double2 field, y0, y1, c0, c1;
+  double freq = 0, t1 = 1, t2 = 1;
+  bool partners = false;
+  if(a >= 0) {
+    pc_predict(k, d, window, a, step, field, y0, y1, c0, c1);
+    freq = d.freq[a];
+    t1 = d.t1[a];
+    t2 = d.t2[a];
+    partners = sync && d.near_start[a + 1] > d.near_start[a];
+    if(partners) d.cur1[a] = y1;
   }
+  if(sync) __syncthreads();
+  const double f = future_coef * k.dt;
+  double2 m0, m1;
+  for(int iter = 0; iter <= kNumCorrectorSteps; ++iter) {
+    if(a >= 0) {
+      double2 om = field;
+      if(partners) om = cadd(om, near_sum(k, d, d.cur1 + static_cast<size_t>(iter & 1) * k.n, a, step));
+      bloch_rhs(y0, y1, om, k.laser_omega, freq, t1, t2, m0, m1);
+      if(iter < kNumCorrectorSteps) {
+        y0 = make_double2(f * m0.x + c0.x, f * m0.y + c0.y);
+        y1 = make_double2(f * m1.x + c1.x, f * m1.y + c1.y);
+        if(partners) d.cur1[static_cast<size_t>((iter & 1) ^ 1) * k.n + a] = y1;
+      }
+    }
+    if(sync && iter < kNumCorrectorSteps) __syncthreads();
+  }
+  if(a >= 0) pc_commit(k, d, ring, a, step, y0, y1, m0, m1);
 }
 
 // Per-call mode: the values passed for `step` become final -- state ring
@@ -1086,14 +1140,14 @@ int pair_blocks_per_cu(const Consts &k, int precision)
 }
 
 hipError_t launch_pc_step(const Consts &k, DeviceState &d, int precision, double future_coef, int window,
-                          int block, int blocks, hipStream_t s)
+                          int block, int blocks, int sync_blocks, hipStream_t s)
 {
   if(precision == QUEST_PREC_FP64)
     hipLaunchKernelGGL(pc_step_kernel<double2>, dim3(blocks), dim3(block), 0, s, k, d,
-                       static_cast<double2 *>(d.ring), future_coef, window);
+                       static_cast<double2 *>(d.ring), future_coef, window, sync_blocks);
   else
     hipLaunchKernelGGL(pc_step_kernel<float2>, dim3(blocks), dim3(block), 0, s, k, d,
-                       static_cast<float2 *>(d.ring), future_coef, window);
+                       static_cast<float2 *>(d.ring), future_coef, window, sync_blocks);
   return hipGetLastError();
 }
 

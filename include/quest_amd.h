@@ -67,7 +67,17 @@ typedef struct {
   int source_chunk;    /* sources per pair-kernel work item; 0 = default
                           (8 chunks of the largest system, halved until the
                           work list fills the GPU); see quest_autotune      */
+  int item_claim;      /* how the pair kernel's waves take work items:
+                          QUEST_CLAIM_AUTO (dynamic for batches, static for
+                          one system; quest_autotune times both),
+                          QUEST_CLAIM_STATIC (a fixed stride, XCD-aware) or
+                          QUEST_CLAIM_DYNAMIC (a device counter; balances
+                          items of uneven size)                              */
 } quest_params;
+
+#define QUEST_CLAIM_AUTO 0
+#define QUEST_CLAIM_STATIC 1
+#define QUEST_CLAIM_DYNAMIC 2
 
 /* Pulse (pulse.h:10-24): amplitude, delay, width (dimensionless), freq,
  * wavevector, polarization.  The polarization is used as given, as on the
@@ -215,17 +225,19 @@ int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
                            int reset);
 
 /* Times the pair kernel (`launches` launches each, after a warm-up) for
- * source-chunk lengths around the default and keeps the fastest; returns it
- * in *chunk_len.  Changes no simulation state.  No counterpart in the
+ * source-chunk lengths around the default, each with static and dynamic item
+ * claiming (only the one item_claim names, unless QUEST_CLAIM_AUTO), and
+ * keeps the fastest; returns its chunk length in *chunk_len.  Changes no simulation state.  No counterpart in the
  * reference, which has no tunable kernel. */
 int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);
 
-/* Integer facts about the context.  info: [14] int64 = n_dots, ring slots,
+/* Integer facts about the context.  info: [15] int64 = n_dots, ring slots,
  * delay bound, zero-delay pairs, work items, persistent grid, state slots,
  * precision, systems, source-chunk length, chunks of the largest system,
  * step graph captured (0/1), block size of the one-launch predictor-corrector
  * step (0: one launch per iteration, used when a connected group of
- * zero-delay partners exceeds 256 dots), size of the largest such group. */
+ * zero-delay partners exceeds 256 dots), size of the largest such group,
+ * pair-kernel items claimed dynamically (1) or by a static stride (0). */
 int quest_ctx_info(const quest_ctx *ctx, int64_t *info);
 
 /* Blocks until all work queued on the context's stream is done. */

@@ -42,6 +42,7 @@ EXPORTS = [
     "quest_set_pulse_list", "quest_set_pair_method",
 ]
 PAIR_ON_THE_FLY, PAIR_TABLE = 0, 1  # QUEST_PAIR_*
+CLAIM_AUTO, CLAIM_STATIC, CLAIM_DYNAMIC = 0, 1, 2  # QUEST_CLAIM_*
 MAX_PULSES = 8  # QUEST_MAX_PULSES
 
 
@@ -61,7 +62,8 @@ class Params(C.Structure):
     _fields_ = [("c0", C.c_double), ("hbar", C.c_double), ("mu0", C.c_double),
                 ("laser_omega", C.c_double), ("dt", C.c_double), ("interp_order", C.c_int),
                 ("history_window", C.c_int), ("precision", C.c_int), ("dyadic", C.c_int),
-                ("device", C.c_int), ("source_chunk", C.c_int)]
+                ("device", C.c_int), ("source_chunk", C.c_int),
+                ("item_claim", C.c_int)]
 
 
 class Pulse(C.Structure):
@@ -142,7 +144,7 @@ class Context:
     of a batch are the systems' arrays concatenated in order."""
 
     def __init__(self, system, consts, precision=PREC_FP64, dyadic=DYADIC_ROTATING,
-                 order=4, window=22, device=0, source_chunk=0, pulse=None):
+                 order=4, window=22, device=0, source_chunk=0, pulse=None, item_claim=0):
         L = load()
         self._L = L
         self.ptr = None
@@ -163,7 +165,7 @@ class Context:
         self.n = int(sum(sizes))
         self.sizes = sizes
         p = Params(consts["c0"], consts["hbar"], consts["mu0"], consts["laser_omega"],
-                   consts["dt"], order, window, precision, dyadic, device, source_chunk)
+                   consts["dt"], order, window, precision, dyadic, device, source_chunk, item_claim)
         out = C.c_void_p()
         if batch:
             arr = (C.c_int * len(sizes))(*sizes)
@@ -286,11 +288,11 @@ class Context:
         return ms.value, cnt.value
 
     def info(self) -> dict:
-        a = (C.c_int64 * 14)()
+        a = (C.c_int64 * 15)()
         self._check(self._L.quest_ctx_info(self.ptr, a))
         keys = ["n", "ring_slots", "max_delay_bound", "near_pairs", "work_items", "grid",
                 "state_slots", "precision", "systems", "source_chunk", "splits", "graph",
-                "step_block", "max_group"]
+                "step_block", "max_group", "dynamic_items"]
         return dict(zip(keys, list(a)))
 
     def sync(self) -> None:

@@ -639,7 +639,8 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, alloc(reinterpret_cast<void **>(&d.counts), sizeof(int) * (nn + 1)));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.pcw), sizeof(double) * 4 * kMaxWindow));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.bad_step), sizeof(int)));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.step), sizeof(int)));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.step), 4 * sizeof(int)));
+  HIPC(c, launch_clear(d.step, 4 * sizeof(int), c->stream));  // step and the pair kernels' item counters
   HIPC(c, alloc(reinterpret_cast<void **>(&d.perm_dev), sizeof(int) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_of), sizeof(int) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_range), sizeof(int2) * n_sys));
@@ -661,8 +662,10 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   int rc = upload_pulses(c);
   if(rc != QUEST_OK) return rc;
 
-  // pair-kernel work list sized for one wave of resident blocks
+  // pair-kernel work list sized for one wave of resident blocks; the items of
+  // a batch differ in size, so its waves claim them dynamically
   c->blocks_per_cu = pair_blocks_per_cu(k, p->precision);
+  k.dyn_items = p->item_claim == QUEST_CLAIM_DYNAMIC || (p->item_claim == QUEST_CLAIM_AUTO && n_sys > 1);
   rc = build_work(c, p->source_chunk > 0 ? p->source_chunk : default_chunk(c));
   if(rc != QUEST_OK) return rc;
 
@@ -708,6 +711,8 @@ int check_params(const quest_params *p, int n)
      !std::isfinite(p->laser_omega))
     return fail(nullptr, QUEST_ERR_ARG, "constants must be finite, dt, c0, hbar > 0");
   if(p->source_chunk < 0) return fail(nullptr, QUEST_ERR_ARG, "source_chunk must be >= 0");
+  if(p->item_claim < QUEST_CLAIM_AUTO || p->item_claim > QUEST_CLAIM_DYNAMIC)
+    return fail(nullptr, QUEST_ERR_ARG, "item_claim must be QUEST_CLAIM_AUTO, _STATIC or _DYNAMIC");
   return QUEST_OK;
 }
 
@@ -1062,7 +1067,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   // every step recomputes from scratch, so timing it between steps changes no
   // state; the device step counter is restored below.
   HIPC(c, launch_set_step(c->d, std::max(c->cur_step, 0), c->stream));
-  int best = c->chunk_len;
+  int best = c->chunk_len, best_dyn = c->k.dyn_items;
   float best_ms = INFINITY;
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
@@ -1070,20 +1075,26 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   for(int len : cand) {
     int rc = build_work(c, len);
     if(rc != QUEST_OK) return rc;
-    HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
-    HIPC(c, hipEventRecord(e0, c->stream));
-    for(int i = 0; i < launches; ++i) HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));
-    HIPC(c, hipEventRecord(e1, c->stream));
-    HIPC(c, hipEventSynchronize(e1));
-    float ms = 0;
-    HIPC(c, hipEventElapsedTime(&ms, e0, e1));
-    if(ms < best_ms) {
-      best_ms = ms;
-      best = len;
+    for(int dyn : {0, 1}) {
+      if(c->P.item_claim != QUEST_CLAIM_AUTO && dyn != (c->P.item_claim == QUEST_CLAIM_DYNAMIC)) continue;
+      c->k.dyn_items = dyn;
+      HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
+      HIPC(c, hipEventRecord(e0, c->stream));
+      for(int i = 0; i < launches; ++i) HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));
+      HIPC(c, hipEventRecord(e1, c->stream));
+      HIPC(c, hipEventSynchronize(e1));
+      float ms = 0;
+      HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+      if(ms < best_ms) {
+        best_ms = ms;
+        best = len;
+        best_dyn = dyn;
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  c->k.dyn_items = best_dyn;
   int rc = build_work(c, best);
   if(rc != QUEST_OK) return rc;
   if(c->mode == 1 && c->cur_step >= 0) {
@@ -1202,6 +1213,7 @@ int quest_ctx_info(const quest_ctx *c, int64_t *info)
   info[11] = c->graph_exec != nullptr;
   info[12] = c->step_block;
   info[13] = c->max_group;
+  info[14] = c->k.dyn_items;
   return QUEST_OK;
 }
 

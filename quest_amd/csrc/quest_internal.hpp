@@ -54,6 +54,7 @@ struct Consts {
   int dyadic;         // QUEST_DYADIC_*
   int n_systems;      // independent dot systems (pairs only within a system)
   int n_work;         // pair-kernel work items (system, target tile, source chunk)
+  int dyn_items;      // 1: pair-kernel waves claim items from a device counter (else a static stride)
   int max_chunks;     // largest number of source chunks of one system (rows of `partial`)
   // coefficient-table mode (the reference's precomputed per-pair coefficients)
   int pair_table;     // 1: the pair sum streams `table` instead of evaluating on the fly
@@ -97,7 +98,8 @@ struct DeviceState {
   PulseParams *pulses = nullptr;       // [systems][QUEST_MAX_PULSES], used ones first
   int4 *work = nullptr;                // [n_work] (target begin, target end, source begin, source end)
   int *work_row = nullptr;             // [n_work] partial row written by the item
-  int *step = nullptr;                 // the step being integrated (device counter; graph replays read it)
+  int *step = nullptr;                 // [4]: the step being integrated (graph replays read it), then the
+                                       // pair kernels' item counters (zero between launches)
   // coefficient-table mode only
   double2 *samp = nullptr;             // rho01 sample ring [n][samp_slots] (finished steps)
   double2 *table = nullptr;            // [order + 1][table_plane]: system s, source b, target a at

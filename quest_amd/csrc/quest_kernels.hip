@@ -310,20 +310,62 @@ __device__ __forceinline__ int first_item()
 }
 __device__ __forceinline__ int items_stride() { return static_cast<int>(gridDim.x) * (kBlock / kWave); }
 
+// Work items of the persistent pair kernels, one wave per item.  Static
+// (DYN false): wave j of block i takes items 4i + j, 4i + j + 4 grid, ...
+// (XCD-aware with the list's chunk grouping).  Dynamic: each wave claims the
+// next item from a device counter, so uneven items (the systems of a batch)
+// balance; every wave makes exactly one failing claim and then counts itself
+// out, and the last wave out resets both counters for the next launch.
+// ctr: [0] next item, [1] waves out (both 0 between launches).
+template <bool DYN>
+struct ItemCursor {
+  static constexpr bool dyn = DYN;
+  int *ctr;
+  int w;
+  __device__ explicit ItemCursor(int *c) : ctr(c), w(0) {}
+  __device__ int claim()
+  {
+    int t = 0;
+    if((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(ctr, 1);
+    return __builtin_amdgcn_readfirstlane(t);
+  }
+  __device__ int first()
+  {
+    w = dyn ? claim() : first_item();
+    return w;
+  }
+  __device__ int next()
+  {
+    w = dyn ? claim() : w + items_stride();
+    return w;
+  }
+  __device__ void finish()
+  {
+    if(!dyn || (threadIdx.x & (kWave - 1)) != 0) return;
+    __threadfence();
+    const int waves = static_cast<int>(gridDim.x) * (kBlock / kWave);
+    if(atomicAdd(ctr + 1, 1) == waves - 1) {
+      atomicExch(ctr, 0);
+      atomicExch(ctr + 1, 0);
+    }
+  }
+};
+
 constexpr int kChunk = 64;
 #ifndef QUEST_F64_ATTR
 #define QUEST_F64_ATTR
 #endif   // sources summed in the pair type before the FP64 fold
 
-template <int M, bool LAB>
+template <int M, bool LAB, bool DYN>
 __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
     const double4 *__restrict__ pos, const double4 *__restrict__ dip,
     const double2 *__restrict__ ring, const int4 *__restrict__ work, const int *__restrict__ work_row,
-    const int *__restrict__ step_ptr, double2 *__restrict__ partial, Consts k)
+    int *step_ptr, double2 *__restrict__ partial, Consts k)
 {
   const PairK<double> K(k);
   const int step = *step_ptr;
-  for(int w = first_item(); w < k.n_work; w += items_stride()) {
+  ItemCursor<DYN> items(step_ptr + 1);
+  for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
@@ -350,6 +392,7 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
     }
     if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
   }
+  items.finish();
 }
 
 // SGPR budget: at 82 SGPRs (what the compiler picks for this loop nest) the
@@ -358,11 +401,11 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
 // compiler reports 8 waves/SIMD for both: fewer waves were resident than
 // that.  Capping at 64 costs a handful of SGPR spills to VGPR lanes outside
 // the inner loop and keeps all 8 waves/SIMD resident.
-template <int M, bool LAB>
+template <int M, bool LAB, bool DYN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void pair_past_f32(
     const float4 *__restrict__ pos, const float4 *__restrict__ dip,
     const double4 *__restrict__ pos64, const float2 *__restrict__ ring,
-    const int4 *__restrict__ work, const int *__restrict__ work_row, const int *__restrict__ step_ptr,
+    const int4 *__restrict__ work, const int *__restrict__ work_row, int *step_ptr,
     double2 *__restrict__ partial, Consts k)
 {
   const PairK<float> K(k);
@@ -372,7 +415,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
   const unsigned mask8 = static_cast<unsigned>(k.ring_mask) << 3;
   const float4 *ring2 = reinterpret_cast<const float4 *>(ring);
   const float2 *ring1 = ring + 2 * kPairs * E;
-  for(int w = first_item(); w < k.n_work; w += items_stride()) {
+  ItemCursor<DYN> items(step_ptr + 1);
+  for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
@@ -417,6 +461,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
     if(tgt < wi.y)
       partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.f32_scale, acci * k.f32_scale);
   }
+  items.finish();
 }
 
 // Past-only pair sum from the coefficient table (the reference's evaluate,
@@ -431,12 +476,13 @@ template <int M, bool LAB>
 __global__ __launch_bounds__(kBlock) void pair_table_f64(
     const double2 *__restrict__ table, const int16_t *__restrict__ table_n,
     const double2 *__restrict__ samp, const int4 *__restrict__ work, const int *__restrict__ work_row,
-    const int *__restrict__ step_ptr, const int *__restrict__ sys_of, const int2 *__restrict__ sys_range,
+    int *step_ptr, const int *__restrict__ sys_of, const int2 *__restrict__ sys_range,
     const long long *__restrict__ sys_toff, double2 *__restrict__ partial, Consts k)
 {
   const int step = *step_ptr;
   const size_t P = static_cast<size_t>(k.table_plane);
-  for(int w = first_item(); w < k.n_work; w += items_stride()) {
+  ItemCursor<false> items(step_ptr + 1);  // table items are uniform: static stride
+  for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int s = sys_of[wi.x];
     const int2 rg = sys_range[s];
@@ -460,6 +506,7 @@ __global__ __launch_bounds__(kBlock) void pair_table_f64(
     }
     if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
   }
+  items.finish();
 }
 
 // ------------------------------------------------------- per-dot kernels
@@ -477,6 +524,8 @@ __device__ __forceinline__ double2 reduce_partials(const Consts &k, const Device
 {
   double2 s = make_double2(0.0, 0.0);
   const int rows = d.sys_chunks[d.sys_of[a]];
+  // unrolled so that several rows' loads are in flight; the sum stays in row order
+#pragma unroll 8
   for(int j = 0; j < rows; ++j) s = cadd(s, d.partial[static_cast<size_t>(j) * k.n + a]);
   return s;
 }
@@ -997,26 +1046,27 @@ __global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_
 
 inline dim3 dot_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
+template <int M, bool LAB, bool DYN>
+void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t s)
+{
+  const dim3 grid(k.grid), block(kBlock);
+  if(precision == QUEST_PREC_FP64)
+    hipLaunchKernelGGL((pair_past_f64<M, LAB, DYN>), grid, block, 0, s, d.pos64, d.dip64,
+                       static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
+  else
+    hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
+                       static_cast<const float2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
+}
+
 template <int M>
 hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
-  const dim3 grid(k.grid), block(kBlock);
   const bool lab = k.dyadic == QUEST_DYADIC_LAB;
-  if(precision == QUEST_PREC_FP64) {
-    if(lab)
-      hipLaunchKernelGGL((pair_past_f64<M, true>), grid, block, 0, s, d.pos64, d.dip64,
-                         static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
-    else
-      hipLaunchKernelGGL((pair_past_f64<M, false>), grid, block, 0, s, d.pos64, d.dip64,
-                         static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
-  } else {
-    if(lab)
-      hipLaunchKernelGGL((pair_past_f32<M, true>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
-                         static_cast<const float2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
-    else
-      hipLaunchKernelGGL((pair_past_f32<M, false>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
-                         static_cast<const float2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
-  }
+  if(lab)
+    k.dyn_items ? launch_pair_mld<M, true, true>(k, d, precision, s) : launch_pair_mld<M, true, false>(k, d, precision, s);
+  else
+    k.dyn_items ? launch_pair_mld<M, false, true>(k, d, precision, s)
+                : launch_pair_mld<M, false, false>(k, d, precision, s);
   return hipGetLastError();
 }
 
@@ -1041,9 +1091,9 @@ int blocks_per_cu_m(int precision, bool table)
   if(table)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_table_f64<M, false>, kBlock, 0);
   else if(precision == QUEST_PREC_FP64)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false>, kBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false>, kBlock, 0);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false>, kBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false>, kBlock, 0);
   return e == hipSuccess && nb > 0 ? nb : 1;
 }
 

@@ -147,3 +147,29 @@ def test_ensemble_generator_and_batch_smoke(ref_weights):
         rho, _ = ctx.state()
         assert np.all(np.isfinite(rho))
         assert ctx.info()["systems"] == 256
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_dynamic_item_claiming_is_bitwise_static(ref_weights, precision):
+    """Waves that claim items from the device counter compute the same items
+    as the static stride, so the trajectories agree bit for bit; the counters
+    reset themselves between launches (step graph replays, the autotuner's
+    back-to-back launches; after it the chunk lengths may differ, so the
+    batch tolerances apply)."""
+    from quest_amd import CLAIM_DYNAMIC, CLAIM_STATIC, Context, systems
+    consts = systems.consts()
+    sys_list, pulses = _systems()
+    first, second = {}, {}
+    for claim in (CLAIM_STATIC, CLAIM_DYNAMIC):
+        with Context(sys_list, consts, precision=precision, pulse=pulses, source_chunk=48,
+                     item_claim=claim) as ctx:
+            ctx.set_pc_weights(*ref_weights)
+            assert ctx.info()["dynamic_items"] == (claim == CLAIM_DYNAMIC)
+            ctx.run(60)
+            first[claim] = ctx.state()[0]
+            assert ctx.autotune(2) >= 32
+            assert ctx.info()["dynamic_items"] == (claim == CLAIM_DYNAMIC)
+            ctx.run(60)
+            second[claim] = ctx.state()[0]
+    assert np.array_equal(first[CLAIM_STATIC], first[CLAIM_DYNAMIC])
+    assert rel_err(second[CLAIM_DYNAMIC], second[CLAIM_STATIC]) < (1e-12 if precision == 0 else 1e-8)

@@ -393,7 +393,7 @@ void free_device(quest_ctx *c)
 {
   drop_graph(c);
   DeviceState &d = c->d;
-  void *ptrs[] = {d.pos64, d.dip64, d.pos32, d.dip32, d.freq, d.t1, d.t2, d.ring, d.y0, d.y1,
+  void *ptrs[] = {d.pos64, d.dip64, d.pd32, d.freq, d.t1, d.t2, d.ring, d.y0, d.y1,
                   d.d0, d.d1, d.cur0, d.cur1, d.yc0, d.yc1, d.field, d.partial, d.phase, d.out,
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.sys_of, d.sys_range, d.sys_chunks,
@@ -586,7 +586,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
 
   // sorted host copies
   std::vector<double4> p64(n), d64(n);
-  std::vector<float4> p32(n), d32(n);
+  std::vector<float4> pd32(2 * static_cast<size_t>(n));  // (position, dipole) per dot
   std::vector<double> f(n, 0.0), a1(n, 1.0), a2(n, 1.0);
   std::vector<int> sys_of(n);
   std::vector<int2> ranges(n_sys);
@@ -598,10 +598,11 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
     const int i = c->perm[s];
     p64[s] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
     d64[s] = make_double4(dip[3 * i], dip[3 * i + 1], dip[3 * i + 2], 0.0);
-    p32[s] = make_float4(static_cast<float>((p64[s].x - ctr[0]) / cdt), static_cast<float>((p64[s].y - ctr[1]) / cdt),
-                         static_cast<float>((p64[s].z - ctr[2]) / cdt), 0.0f);
-    d32[s] = make_float4(static_cast<float>(d64[s].x), static_cast<float>(d64[s].y),
-                         static_cast<float>(d64[s].z), 0.0f);
+    pd32[2 * s] = make_float4(static_cast<float>((p64[s].x - ctr[0]) / cdt),
+                              static_cast<float>((p64[s].y - ctr[1]) / cdt),
+                              static_cast<float>((p64[s].z - ctr[2]) / cdt), 0.0f);
+    pd32[2 * s + 1] = make_float4(static_cast<float>(d64[s].x), static_cast<float>(d64[s].y),
+                                  static_cast<float>(d64[s].z), 0.0f);
     if(freq) f[s] = freq[i];
     if(t1) a1[s] = t1[i];
     if(t2) a2[s] = t2[i];
@@ -611,12 +612,16 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   const size_t nn = static_cast<size_t>(n);
   const size_t r2size = p->precision == QUEST_PREC_FP64 ? sizeof(double2) : sizeof(float2);
   c->ring_bytes = nn * k.ring_slots * k.kstride * r2size;
+  // the FP32 pair kernel addresses a 16-B coefficient plane with 32-bit byte offsets
+  if(p->precision == QUEST_PREC_MIXED && nn * k.ring_slots * sizeof(float4) > 0xFFFFFFFFull)
+    return fail(c, QUEST_ERR_ARG,
+                "mixed precision: dots x ring slots (" + std::to_string(nn) + " x " + std::to_string(k.ring_slots) +
+                    ") exceeds 2^28; use QUEST_PREC_FP64 or split the system");
   const size_t ss = static_cast<size_t>(k.state_slots) * nn;
   auto alloc = [&](void **ptr, size_t bytes) -> hipError_t { return hipMalloc(ptr, std::max<size_t>(bytes, 16)); };
   HIPC(c, alloc(reinterpret_cast<void **>(&d.pos64), sizeof(double4) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.dip64), sizeof(double4) * nn));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.pos32), sizeof(float4) * nn));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.dip32), sizeof(float4) * nn));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.pd32), sizeof(float4) * 2 * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.freq), sizeof(double) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.t1), sizeof(double) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.t2), sizeof(double) * nn));
@@ -650,8 +655,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
 
   HIPC(c, hipMemcpyAsync(d.pos64, p64.data(), sizeof(double4) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.dip64, d64.data(), sizeof(double4) * nn, hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(d.pos32, p32.data(), sizeof(float4) * nn, hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(d.dip32, d32.data(), sizeof(float4) * nn, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(d.pd32, pd32.data(), sizeof(float4) * 2 * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.freq, f.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.t1, a1.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.t2, a2.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));

@@ -72,7 +72,7 @@ struct PulseParams {
 // Device buffers of one context (sorted dot order).
 struct DeviceState {
   double4 *pos64 = nullptr, *dip64 = nullptr;   // (x, y, z, 0)
-  float4 *pos32 = nullptr, *dip32 = nullptr;      // pos32: (pos - centre) / (c0 dt)
+  float4 *pd32 = nullptr;              // [n][2]: position (pos - centre) / (c0 dt), dipole (FP32)
   double *freq = nullptr, *t1 = nullptr, *t2 = nullptr;
   void *ring = nullptr;                // coefficient planes, kstride x [n][ring_slots] complex
   double2 *y0 = nullptr, *y1 = nullptr;  // state ring [state_slots][n]: rho00, rho01

@@ -47,12 +47,15 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b)
 // (the pair sum and the zero-delay list builder) gets bit-identical results.
 // r2 is clamped to 1e-30 (physical pairs are ~0.01 and above in these units)
 // so that tau is finite for coincident dots too.
-__device__ __forceinline__ float tau_f32(float dx, float dy, float dz, float &r2, float &ir)
+// r^2 (units of (c0 dt)^2) and 1/r of the FP32 pair arithmetic.  r^2 starts
+// from 1e-20 (below the rounding of any r^2 >= 1e-12, i.e. of any pair that
+// is not deep inside the zero-delay range), so the geometry of a dot with
+// itself or a coincident one stays finite (1/r <= 1e10) and its term is an
+// exact 0 against the zeroed slot it reads.
+__device__ __forceinline__ void r2_f32(float dx, float dy, float dz, float &r2, float &ir)
 {
-#pragma clang fp contract(off)
-  r2 = __builtin_fmaxf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), 1e-30f);
+  r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, 1e-20f)));
   ir = __builtin_amdgcn_rsqf(r2);
-  return r2 * ir;
 }
 
 // FP64 delay split, bit-identical to the reference's
@@ -67,7 +70,10 @@ __device__ __forceinline__ int split_f64(double dx, double dy, double dz, double
 {
 #pragma clang fp contract(off)
   r2 = (dx * dx + dy * dy) + dz * dz;
-  double y = __builtin_amdgcn_rsq(r2);
+  // estimate from r^2 >= 1e-20 um^2: pairs closer than 1e-10 um (delay 0;
+  // the self pair) get a finite 1/r <= 2.25e10 and a finite fraction, so their
+  // pair-kernel terms are an exact 0 against the zeroed slot they read
+  double y = __builtin_amdgcn_rsq(__builtin_fmax(r2, 1e-20));
   {
     const double h = r2 * y;
     const double e = __builtin_fma(-h, y, 1.0);
@@ -99,28 +105,41 @@ __device__ __forceinline__ int split_f64(double dx, double dy, double dz, double
 // factor two of margin) of an integer -- tested as |fr - 1/2| > 1/2 - eps --
 // and then the exact FP64 split decides.  The integer delays are therefore the
 // reference's in both precisions, and the pair kernel and the zero-delay list
-// builder classify every pair alike.  half_m_eps_abs = 1/2 - eps_abs.
-// Returns n as an integer-valued float and the fraction centred on 1/2,
-// xc = x - 1/2 in [-1/2, 1/2) (the FP32 interpolant's variable).
-__device__ __forceinline__ float split_mixed(float dx, float dy, float dz, float eps_rel,
-                                           float half_m_eps_abs, const double4 &pa64,
-                                           const double4 *__restrict__ pos64, int b, double cdt,
-                                           double icdt64, float &xc, float &ir)
+// builder classify every pair alike.  The estimate's integer part is
+// n = rint(tau - 1/2), which is floor(tau) whenever the test passes, taken
+// with a rounding constant ms = kRoundMagic + sm (0 <= sm < 2^22): no
+// float-to-int conversion, and the bit pattern of ms - n, returned, is
+// kRoundMagicBits + sm - n, so with sm = step & mask its low bits are the
+// ring slot (step - n) & mask.  lim0 = 1/2 - eps_abs - eps_rel / 2.  Also
+// returns the fraction centred on 1/2, xc = x - 1/2 (the FP32 interpolant's
+// variable).
+//
+// 1.5 * 2^23: for 0 <= sm < 2^22 and |v| < 2^22 the float
+// (kRoundMagic + sm) - v is exactly kRoundMagic + sm - rint(v) (a float in
+// [2^23, 2^24), unit in the last place 1), whose bit pattern is
+// kRoundMagicBits + sm - rint(v).
+constexpr float kRoundMagic = 12582912.0f;
+constexpr int kRoundMagicBits = 0x4B400000;
+
+__device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float eps_rel, float lim0, float ms,
+                                           const double4 &pa64, const double4 *__restrict__ pos64, int b,
+                                           double cdt, double icdt64, float &xc, float &ir)
 {
   float r2;
-  const float tau = tau_f32(dx, dy, dz, r2, ir);
-  const float fl = __builtin_floorf(tau);
-  const float fr = tau - fl;
-  const float lim = __builtin_fmaf(-eps_rel, tau, half_m_eps_abs);
-  xc = fr - 0.5f;  // exact (Sterbenz, or fr a multiple of tau's ulp below 1/2)
+  r2_f32(dx, dy, dz, r2, ir);
+  const float u = __builtin_fmaf(r2, ir, -0.5f);  // tau - 1/2, tau = r / (c0 dt)
+  const float t = ms - u;                         // ms - n, n = rint(tau - 1/2)
+  xc = u - (ms - t);                              // exact: tau - n - 1/2 in [-1/2, 1/2]
+  const float lim = __builtin_fmaf(-eps_rel, u, lim0);  // 1/2 - eps_rel tau - eps_abs
+  int code = __builtin_bit_cast(int, t);
   if(__builtin_expect(__builtin_fabsf(xc) > lim, 0)) {
     const double4 pb = pos64[b];
     double xd, ird, r2d;
     const int n = split_f64(pa64.x - pb.x, pa64.y - pb.y, pa64.z - pb.z, cdt, icdt64, xd, ird, r2d);
-    xc = xd == xd ? static_cast<float>(xd - 0.5) : 0.0f;  // coincident dots: n = 0, finite xc
-    return static_cast<float>(n);
+    xc = static_cast<float>(xd - 0.5);
+    code = __builtin_bit_cast(int, ms) - n;
   }
-  return fl;  // tau is finite and >= 0
+  return code;  // bit pattern of ms - n
 }
 
 // sin and cos of theta x for x in [-1/2, 1/2), |theta x| <= 0.114: Taylor
@@ -234,14 +253,14 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
 // f32_scale = two_idt2 / L so that the g1 q2 term needs no multiply.
 template <typename R> struct PairK {
   static constexpr bool kF32 = sizeof(R) == sizeof(float);
-  R cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, half_m_eps_abs;
+  R cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, lim0;
   R sn[3], cs[2];  // sincos_theta coefficients (FP32)
   double cdt, icdt64;
   __host__ __device__ explicit PairK(const Consts &k)
       : cc((R)(kF32 ? k.cc_l : k.cc)), oo((R)(kF32 ? k.oo_l : k.oo)), oc((R)(kF32 ? k.oc_l : k.oc)),
         m_cdt((R)(kF32 ? k.m_cdt_l : k.m_cdt)), m_o2dt((R)(kF32 ? k.m_o2dt_l : k.m_o2dt)),
         two_idt2((R)(kF32 ? 1.0 : k.two_idt2)), theta((R)k.theta), eps_rel((R)k.eps_rel),
-        half_m_eps_abs((R)(0.5 - k.eps_abs)),
+        lim0((R)(0.5 - k.eps_abs - 0.5 * k.eps_rel)),
         sn{(R)k.sin_c[0], (R)k.sin_c[1], (R)k.sin_c[2]}, cs{(R)k.cos_c[0], (R)k.cos_c[1]},
         cdt(k.cdt), icdt64(k.icdt) {}
 };
@@ -253,12 +272,16 @@ template <typename R> struct PairK {
 // T1 = c g3, T2 = g1, g3 = (d_a.d_b - 3 B)/r^2, g1 = (d_a.d_b - B)/r,
 // B = (d_a.dr)(d_b.dr)/r^2 (rotating_green_function.cpp:13-20 projected on the
 // dipoles, history_interaction.cpp:40-43).  LAB drops the rotation and the
-// i w mixing (green_function.cpp:12-17).
+// i w mixing (green_function.cpp:12-17).  Grouped by the two geometric
+// factors, the bracket is g3 G3 + g1 G1 with
+//   G3 = (c^2/r) q0 + i (w c) q0 - (c/dt) q1,
+//   G1 = (2/dt^2) q2 - w^2 q0 - i (2w/dt) q1,
+// and the rotated term is added to (accr, acci) in the same fused operations.
 template <typename R, bool LAB>
 __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R dz,
                                           const typename V4<R>::type &da,
                                           const typename V4<R>::type &db, R x, R q0r, R q0i,
-                                          R q1r, R q1i, R q2r, R q2i, R &outr, R &outi)
+                                          R q1r, R q1i, R q2r, R q2i, R &accr, R &acci)
 {
   const R ir2 = ir * ir;
   const R ba = da.x * dx + da.y * dy + da.z * dz;
@@ -267,25 +290,29 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
   const R bq = ba * bb * ir2;
   const R g3 = (aab - 3 * bq) * ir2;
   const R g1 = (aab - bq) * ir;
-  const R a1 = K.m_cdt * g3;
-  const R a2 = PairK<R>::kF32 ? g1 : K.two_idt2 * g1;  // FP32 constants are divided by it
+  // FP32 constants are divided by two_idt2, so there the q2 factor is 1
+  const R h2r = PairK<R>::kF32 ? q2r : K.two_idt2 * q2r;
+  const R h2i = PairK<R>::kF32 ? q2i : K.two_idt2 * q2i;
+  const R t = K.cc * ir;
   if constexpr(LAB) {
-    const R a0 = K.cc * ir * g3;
-    outr = a0 * q0r + a1 * q1r + a2 * q2r;
-    outi = a0 * q0i + a1 * q1i + a2 * q2i;
+    const R g3r = t * q0r + K.m_cdt * q1r;
+    const R g3i = t * q0i + K.m_cdt * q1i;
+    accr = fmaT(g3, g3r, fmaT(g1, h2r, accr));
+    acci = fmaT(g3, g3i, fmaT(g1, h2i, acci));
   } else {
-    const R a0 = K.cc * ir * g3 - K.oo * g1;
-    const R b0 = K.oc * g3;
-    const R b1 = K.m_o2dt * g1;
-    const R cr = a0 * q0r - b0 * q0i + a1 * q1r - b1 * q1i + a2 * q2r;
-    const R ci = a0 * q0i + b0 * q0r + a1 * q1i + b1 * q1r + a2 * q2i;
+    const R g3r = t * q0r - K.oc * q0i + K.m_cdt * q1r;
+    const R g3i = t * q0i + K.oc * q0r + K.m_cdt * q1i;
+    const R g1r = h2r - K.oo * q0r - K.m_o2dt * q1i;
+    const R g1i = h2i - K.oo * q0i + K.m_o2dt * q1r;
+    const R cr = g3 * g3r + g1 * g1r;
+    const R ci = g3 * g3i + g1 * g1i;
     R sn, cs;
     if constexpr(sizeof(R) == sizeof(float))
       sincos_theta(K.sn, K.cs, x, sn, cs);
     else
       sincos_small(K.theta * x, sn, cs);
-    outr = cr * cs + ci * sn;
-    outi = ci * cs - cr * sn;
+    accr = fmaT(cr, cs, fmaT(ci, sn, accr));
+    acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
   }
 }
 
@@ -375,8 +402,9 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
       const double4 pb = pos[b], db = dip[b];
       const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
       double x, ir, r2;
+      // delay 0 (the self pair, zero-delay partners) reads the current step's
+      // slot, which holds zeros until the step commits: its term is 0
       const int n = split_f64(dx, dy, dz, K.cdt, K.icdt64, x, ir, r2);
-      const bool valid = n >= 1;  // the self pair has r = 0, hence n = 0
       const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
       const double2 *c = ring + static_cast<size_t>(b) * k.ring_slots * (M + 1);
       const unsigned off = slot * static_cast<unsigned>((M + 1) * sizeof(double2));
@@ -385,10 +413,7 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
       for(int i = 0; i <= M; ++i) cv[i] = load_off(c + i, off);
       double q0r, q0i, q1r, q1i, q2r, q2i;
       horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-      double tr, ti;
-      pair_term<double, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
-      accr += valid ? tr : 0.0;
-      acci += valid ? ti : 0.0;
+      pair_term<double, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
     }
     if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
   }
@@ -401,59 +426,61 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
 // compiler reports 8 waves/SIMD for both: fewer waves were resident than
 // that.  Capping at 64 costs a handful of SGPR spills to VGPR lanes outside
 // the inner loop and keeps all 8 waves/SIMD resident.
+// Base of each coefficient plane of the FP32 ring, passed as separate kernel
+// arguments so that each stays a scalar base of its loads (computed in the
+// kernel, ring + E would be folded into the per-lane 64-bit address).
+struct RingPlanesF32 {
+  const float4 *pair[3];  // planes of (c_2i, c_2i+1)
+  const float2 *last;     // plane of c_M (even M)
+};
+
 template <int M, bool LAB, bool DYN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void pair_past_f32(
-    const float4 *__restrict__ pos, const float4 *__restrict__ dip,
-    const double4 *__restrict__ pos64, const float2 *__restrict__ ring,
+    const float4 *__restrict__ pd, const double4 *__restrict__ pos64, const RingPlanesF32 planes,
     const int4 *__restrict__ work, const int *__restrict__ work_row, int *step_ptr,
     double2 *__restrict__ partial, Consts k)
 {
   const PairK<float> K(k);
   const int step = *step_ptr;
-  const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
   constexpr int kPairs = (M + 1) / 2;
-  const unsigned mask8 = static_cast<unsigned>(k.ring_mask) << 3;
-  const float4 *ring2 = reinterpret_cast<const float4 *>(ring);
-  const float2 *ring1 = ring + 2 * kPairs * E;
+  const unsigned mask = static_cast<unsigned>(k.ring_mask);
+  const float ms = kRoundMagic + static_cast<float>(step & k.ring_mask);  // slot = code & mask
+  // a lane's 32-bit byte offset into a 16-B plane is (b ring_slots + slot) * 16
+  // (the host keeps a plane below 4 GiB), formed as (slot * 16) | (b ring_slots * 16)
+  // since slot < ring_slots, a power of two; the 8-B plane's offset is half of it
+  const unsigned row_step16 = static_cast<unsigned>(k.ring_slots) << 4;
   ItemCursor<DYN> items(step_ptr + 1);
   for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
-    const float4 pa = pos[a], da = dip[a];
+    const float4 pa = pd[2 * a], da = pd[2 * a + 1];
     const double4 pa64 = pos64[a];
     double accr = 0, acci = 0;
     for(int bt = wi.z; bt < wi.w; bt += kChunk) {
       const int be = min(wi.w, bt + kChunk);
       float sr = 0, si = 0;
       for(int b = bt; b < be; ++b) {
-        const float4 pb = pos[b], db = dip[b];
+        const float4 pb = pd[2 * b], db = pd[2 * b + 1];  // one scalar load of 32 B
         const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
         float x, ir;  // x: the fraction centred on 1/2 (the FP32 ring's variable)
-        const float fn = split_mixed(dx, dy, dz, K.eps_rel, K.half_m_eps_abs, pa64, pos64, b, K.cdt,
-                                     K.icdt64, x, ir);
-        const int mn = static_cast<int>(-fn);  // -n
-        // n >= 1 pairs only (the self pair has r = 0, hence n = 0): 1/r = 0
-        // zeroes every coefficient of the term, all other factors being finite
-        ir = mn <= -1 ? ir : 0.0f;
-        // byte offsets of the slot (step - n) & mask in the 8-B and 16-B planes
-        const unsigned off8 = (static_cast<unsigned>(step + mn) << 3) & mask8;
-        const unsigned off16 = off8 << 1;
-        const size_t row = static_cast<size_t>(b) * k.ring_slots;
+        const int code = split_mixed(dx, dy, dz, K.eps_rel, K.lim0, ms, pa64, pos64, b, K.cdt, K.icdt64, x, ir);
+        // byte offsets of the slot (step - n) & mask in the 16-B and 8-B
+        // planes; delay 0 (the self pair, zero-delay partners) reads the
+        // current step's slot, which holds zeros until the step commits
+        const unsigned off16 = ((static_cast<unsigned>(code) & mask) << 4) | (static_cast<unsigned>(b) * row_step16);
+        const unsigned off8 = off16 >> 1;
         float2 cv[M + 1];
 #pragma unroll
         for(int i = 0; i < kPairs; ++i) {
-          const float4 v = load_off(ring2 + i * E + row, off16);
+          const float4 v = load_off(planes.pair[i], off16);
           cv[2 * i] = make_float2(v.x, v.y);
           cv[2 * i + 1] = make_float2(v.z, v.w);
         }
-        if constexpr(2 * kPairs == M) cv[M] = load_off(ring1 + row, off8);
+        if constexpr(2 * kPairs == M) cv[M] = load_off(planes.last, off8);
         float q0r, q0i, q1r, q1i, q2r, q2i;
         horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-        float tr, ti;
-        pair_term<float, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, tr, ti);
-        sr += tr;
-        si += ti;
+        pair_term<float, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, sr, si);
       }
       accr += static_cast<double>(sr);
       acci += static_cast<double>(si);
@@ -617,7 +644,11 @@ __device__ __forceinline__ double2 near_sum(const Consts &k, const DeviceState &
 
 // Writes the interpolant coefficients of slot s for source a:
 // c_k = e^{i theta s} sum_m vinv[k][m] rho01(s - m)  (rotating), or of
-// 2 Re rho01 without the phase (lab).
+// 2 Re rho01 without the phase (lab), and zeros into slot s + 1.  The pair
+// kernels of step s + 1 read slot s + 1 for exactly the pairs of delay 0 (a
+// dot with itself, zero-delay partners, which the near list evaluates), so
+// those terms vanish without a per-pair test; the slot's previous content
+// (step s + 1 - ring_slots) is older than any delay.
 template <typename R2>
 __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase, R2 *ring,
                                            double2 *samp, const double2 *y1, double2 rho_s, int a,
@@ -655,6 +686,12 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
     R2 &dst = ring[ring_index(ring, k.order, E, e, kk)];
     dst.x = c.x;
     dst.y = c.y;
+  }
+  const size_t e_next = static_cast<size_t>(a) * k.ring_slots + ((s + 1) & k.ring_mask);
+  for(int kk = 0; kk <= k.order; ++kk) {
+    R2 &dst = ring[ring_index(ring, k.order, E, e_next, kk)];
+    dst.x = 0;
+    dst.y = 0;
   }
 }
 
@@ -898,14 +935,14 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
       }
     }
   } else {
-    const float4 pa = d.pos32[a];
+    const float4 pa = d.pd32[2 * a];
     const double4 pa64 = d.pos64[a];
     const PairK<float> K(k);
     for(int b = range.x; b < range.y; ++b) {
-      const float4 pb = d.pos32[b];
+      const float4 pb = d.pd32[2 * b];
       float x, ir;
-      const int n = static_cast<int>(split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.eps_rel,
-                                                 K.half_m_eps_abs, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir));
+      const int n = kRoundMagicBits - split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.eps_rel, K.lim0,
+                                                  kRoundMagic, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
       if(b != a && n < 1) {
         if(FILL) d.near_src[base + cnt] = b;
         ++cnt;
@@ -1053,9 +1090,16 @@ void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t
   if(precision == QUEST_PREC_FP64)
     hipLaunchKernelGGL((pair_past_f64<M, LAB, DYN>), grid, block, 0, s, d.pos64, d.dip64,
                        static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
-  else
-    hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN>), grid, block, 0, s, d.pos32, d.dip32, d.pos64,
-                       static_cast<const float2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
+  else {
+    // FP32 ring layout (see ring_index): (M + 1) / 2 float4 planes, then the c_M plane
+    const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
+    const float4 *r4 = static_cast<const float4 *>(d.ring);
+    RingPlanesF32 planes{};
+    for(int i = 0; i < (M + 1) / 2; ++i) planes.pair[i] = r4 + i * E;
+    planes.last = reinterpret_cast<const float2 *>(r4 + ((M + 1) / 2) * E);
+    hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN>), grid, block, 0, s, d.pd32, d.pos64, planes, d.work,
+                       d.work_row, d.step, d.partial, k);
+  }
 }
 
 template <int M>

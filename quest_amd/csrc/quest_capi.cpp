@@ -358,11 +358,11 @@ int enqueue_step(quest_ctx *c)
   if(c->step_block > 0) {
     HIPC(c, launch_pc_step(c->k, c->d, c->P.precision, c->fc, c->P.history_window, c->step_block,
                            c->step_blocks, c->step_sync_blocks, c->stream));
-  } else {
-    HIPC(c, launch_pc_begin(c->k, c->d, c->P.history_window, c->stream));
-    for(int it = 0; it <= kNumCorrectorSteps; ++it)
-      HIPC(c, launch_pc_iter(c->k, c->d, c->P.precision, c->fc, it, c->stream));
+    return QUEST_OK;  // the step kernel advanced the step counter
   }
+  HIPC(c, launch_pc_begin(c->k, c->d, c->P.history_window, c->stream));
+  for(int it = 0; it <= kNumCorrectorSteps; ++it)
+    HIPC(c, launch_pc_iter(c->k, c->d, c->P.precision, c->fc, it, c->stream));
   HIPC(c, launch_advance_step(c->d, c->stream));
   return QUEST_OK;
 }

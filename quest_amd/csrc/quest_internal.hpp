@@ -98,8 +98,9 @@ struct DeviceState {
   PulseParams *pulses = nullptr;       // [systems][QUEST_MAX_PULSES], used ones first
   int4 *work = nullptr;                // [n_work] (target begin, target end, source begin, source end)
   int *work_row = nullptr;             // [n_work] partial row written by the item
-  int *step = nullptr;                 // [4]: the step being integrated (graph replays read it), then the
-                                       // pair kernels' item counters (zero between launches)
+  int *step = nullptr;                 // [4]: the step being integrated (graph replays read it), the
+                                       // pair kernels' item counters, the step kernel's finished
+                                       // blocks (all three zero between launches)
   // coefficient-table mode only
   double2 *samp = nullptr;             // rho01 sample ring [n][samp_slots] (finished steps)
   double2 *table = nullptr;            // [order + 1][table_plane]: system s, source b, target a at
@@ -126,7 +127,8 @@ hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipS
 hipError_t launch_pc_begin(const Consts &k, DeviceState &d, int window, hipStream_t s);
 // The whole step of every dot (predictor + 11 iterations) in one launch of
 // `blocks` blocks of `block` (<= kBlock) threads over d.step_map; the first
-// `sync_blocks` blocks hold every dot with zero-delay partners.
+// `sync_blocks` blocks hold every dot with zero-delay partners.  Advances
+// the device step counter at its end.
 hipError_t launch_pc_step(const Consts &k, DeviceState &d, int precision, double future_coef, int window,
                           int block, int blocks, int sync_blocks, hipStream_t s);
 hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, double future_coef,

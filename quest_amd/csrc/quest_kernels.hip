@@ -47,14 +47,15 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b)
 // (the pair sum and the zero-delay list builder) gets bit-identical results.
 // r2 is clamped to 1e-30 (physical pairs are ~0.01 and above in these units)
 // so that tau is finite for coincident dots too.
-// r^2 (units of (c0 dt)^2) and 1/r of the FP32 pair arithmetic.  r^2 starts
-// from 1e-20 (below the rounding of any r^2 >= 1e-12, i.e. of any pair that
-// is not deep inside the zero-delay range), so the geometry of a dot with
-// itself or a coincident one stays finite (1/r <= 1e10) and its term is an
-// exact 0 against the zeroed slot it reads.
+// r^2 (units of (c0 dt)^2) and 1/r of the FP32 pair arithmetic.  r^2 is
+// clamped to 1e-20 (pairs that close have delay 0), so the geometry of a dot
+// with itself or a coincident one stays finite (1/r <= 1e10) and its term is
+// an exact 0 against the zeroed slot it reads.  (A max, not a seed added in
+// the first FMA: FP32 FMAs issue at half the rate of muls and maxes here.)
 __device__ __forceinline__ void r2_f32(float dx, float dy, float dz, float &r2, float &ir)
 {
-  r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, 1e-20f)));
+#pragma clang fp contract(off)
+  r2 = __builtin_fmaxf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)), 1e-20f);
   ir = __builtin_amdgcn_rsqf(r2);
 }
 
@@ -366,10 +367,11 @@ struct ItemCursor {
     w = dyn ? claim() : w + items_stride();
     return w;
   }
+  // No fence: a wave's failing claim returned before it counts itself out,
+  // which is all the reset needs (a device-scope fence writes back the L2).
   __device__ void finish()
   {
     if(!dyn || (threadIdx.x & (kWave - 1)) != 0) return;
-    __threadfence();
     const int waves = static_cast<int>(gridDim.x) * (kBlock / kWave);
     if(atomicAdd(ctr + 1, 1) == waves - 1) {
       atomicExch(ctr, 0);
@@ -827,7 +829,9 @@ __global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d
 // partners sits in one of the first `sync_blocks` blocks; there a dot with
 // partners publishes each iterate to the cur1 double buffer, and a block
 // barrier (a workgroup-scope fence on gfx950) separates the iterations.
-// Blocks of dots without partners run barrier-free.
+// Blocks of dots without partners run barrier-free.  The last block to finish
+// advances the step counter (what advance_step_kernel does after the
+// per-iteration kernels): every block has read it by then.
 template <typename R2>
 __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d, R2 *ring,
                                                          double future_coef, int window, int sync_blocks)
@@ -863,6 +867,16 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
     if(sync && iter < kNumCorrectorSteps) __syncthreads();
   }
   if(a >= 0) pc_commit(k, d, ring, a, step, y0, y1, m0, m1);
+  // No fence: the counters order nothing but themselves (every thread's read
+  // of the step completed before the barrier; data reaches the next kernel at
+  // the kernel boundary), and a device-scope fence writes back the L2 here.
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    if(atomicAdd(d.step + 3, 1) == static_cast<int>(gridDim.x) - 1) {
+      atomicExch(d.step + 3, 0);
+      atomicAdd(d.step, 1);
+    }
+  }
 }
 
 // Per-call mode: the values passed for `step` become final -- state ring

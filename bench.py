@@ -9,10 +9,12 @@ does it (src/integrator.cpp:120-185).  The workload is N = 65536 dots at
 (dt = 1e-4 ps, order-4 interpolation, 2000-step run); `value` is the
 mixed-precision path (FP32 pair arithmetic, FP64 accumulation and FP64 state)
 and the same run in plain FP64 is reported beside it, with the max |d rho|
-between the two.  Inputs live in HBM before the timed region starts; the
-timed steps replay the captured step graph, and the pair kernel's mean
-duration for the roofline comes from 10 steps before them launched with HIP
-events around every pair-kernel launch.
+between the two.  Inputs live in HBM before the timed region starts.  The
+timed steps are launched kernel by kernel with HIP events on the context's
+stream around every pair-kernel launch, so the roofline's kernel duration is
+the mean over the timed region itself; the same number of steps replayed as
+one captured graph per step (quest_run's default) follows, reported as
+graph_ms_per_step.
 
 The CPU baseline is the FP64 oracle's reference method (pair table plus 11
 scatter sums per step, history_interaction.cpp:47-70 + integrator.cpp:129-140)
@@ -38,11 +40,35 @@ FP32_PEAK = 157.3      # TFLOP/s, MI355X vector FP32 (datasheet)
 FP64_PEAK = 78.6       # TFLOP/s, MI355X vector FP64
 
 
-def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0, timing_steps=10):
-    """Warm-up; `timing_steps` steps launched kernel by kernel with HIP events
-    on the context's stream around every pair-kernel launch (the roofline's
-    kernel duration); then the timed region: `steps` steps as in production,
-    one captured graph replayed per step."""
+def timed_steps(ctx, warmup, steps):
+    """Warm-up, then the timed region: `steps` steps launched kernel by kernel
+    with HIP events on the context's stream around every pair-kernel launch
+    (wall time and the pair kernel's mean duration over the same steps); then
+    `steps` steps replayed as one captured graph per step (captured outside
+    the timed loop)."""
+    if warmup:
+        ctx.run(warmup)
+    ctx.sync()
+    ctx.set_timing(True)
+    ctx.pair_kernel_time(reset=True)
+    t0 = time.perf_counter()
+    ctx.run(steps)              # returns after the stream has drained
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    kms, launches = ctx.pair_kernel_time()
+    ctx.set_timing(False)
+    ctx.run(1)                  # captures the step graph
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.run(steps)
+    ctx.sync()
+    graph_secs = time.perf_counter() - t0
+    return {"seconds": secs, "pair_ms": kms / max(launches, 1), "launches": launches,
+            "graph_ms_per_step": 1e3 * graph_secs / max(steps, 1)}
+
+
+def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0):
+    """One context on the workload; see timed_steps."""
     from quest_amd import Context
     ps, cs, fc = weights
     with Context(sysd, consts, precision=precision, pulse=pulse) as ctx:
@@ -50,54 +76,23 @@ def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0, ti
             ctx.set_pair_method(method)
         ctx.set_pc_weights(ps, cs, fc)
         info = ctx.info()
-        if warmup:
-            ctx.run(warmup)
-        ctx.sync()
-        ctx.set_timing(True)
-        ctx.pair_kernel_time(reset=True)
-        ctx.run(timing_steps)
-        ctx.sync()
-        kms, launches = ctx.pair_kernel_time()
-        ctx.set_timing(False)
-        ctx.run(1)              # captures the step graph outside the timed region
-        ctx.sync()
-        t0 = time.perf_counter()
-        ctx.run(steps)          # returns after the stream has drained
-        ctx.sync()
-        secs = time.perf_counter() - t0
+        out = timed_steps(ctx, warmup, steps)
         rho, _ = ctx.state()
-    return {"seconds": secs, "pair_ms": kms / max(launches, 1), "launches": launches,
-            "rho": rho, "info": info}
+    return dict(out, rho=rho, info=info)
 
 
-def run_ensemble(ens, consts, precision, warmup, steps, weights, timing_steps=10):
-    """The batched workload: all systems in one context; the timed region
-    replays the captured step graph.  The pair-kernel mean comes from a
-    kernel-by-kernel pass of timing_steps steps with HIP events just before it."""
+def run_ensemble(ens, consts, precision, warmup, steps, weights):
+    """The batched workload: all systems in one context, work split autotuned
+    before the warm-up; see timed_steps."""
     from quest_amd import Context
     ps, cs, fc = weights
     with Context([s for s, _ in ens], consts, precision=precision, pulse=[p for _, p in ens]) as ctx:
         ctx.set_pc_weights(ps, cs, fc)
         chunk = ctx.autotune(3)
-        if warmup:
-            ctx.run(warmup)
-        ctx.sync()
-        ctx.set_timing(True)
-        ctx.pair_kernel_time(reset=True)
-        ctx.run(timing_steps)
-        ctx.sync()
-        kms, launches = ctx.pair_kernel_time()
-        ctx.set_timing(False)
-        ctx.run(1)              # captures the step graph outside the timed region
-        ctx.sync()
-        t0 = time.perf_counter()
-        ctx.run(steps)
-        ctx.sync()
-        secs = time.perf_counter() - t0
+        out = timed_steps(ctx, warmup, steps)
         info = ctx.info()
         rho, _ = ctx.state()
-    return {"seconds": secs, "pair_ms": kms / max(launches, 1), "rho": rho, "info": info,
-            "chunk": chunk}
+    return dict(out, rho=rho, info=info, chunk=chunk)
 
 
 def ensemble_main(args):
@@ -120,18 +115,21 @@ def ensemble_main(args):
         "dtype": "f32 pair arithmetic + f64 accumulation/state", "data": "synthetic",
         "config": {"workload": "ensemble: 256 systems, N log-uniform in [200, 2000] at 512 um^-3, "
                                "random dipoles, 2 rad/ps detuning, pulse area uniform in [pi/4, pi], "
-                               "dt=1e-4 ps, order 4, 5000-step run, one graph replay per step",
+                               "dt=1e-4 ps, order 4, 5000-step run",
                    "systems": 256, "dots": int(sizes.sum()), "pairs_per_step": pairs,
                    "source_chunk_autotuned": mixed["chunk"], "work_items": mixed["info"]["work_items"],
-                   "persistent_grid": mixed["info"]["grid"]},
+                   "persistent_grid": mixed["info"]["grid"],
+                   "dynamic_items": mixed["info"]["dynamic_items"]},
+        "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK, "traffic": None,
-                     "kernel": "pair_past_f32<4,false>", "kernel_ms": mixed["pair_ms"],
-                     "flop_per_pair": F_ALG},
+                     "kernel": "pair_past_f32<4,false,{}>".format("true" if mixed["info"]["dynamic_items"] else "false"),
+                     "kernel_ms": mixed["pair_ms"], "flop_per_pair": F_ALG},
     }
     if fp64 is not None:
         v64 = pairs * args.steps / fp64["seconds"]
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
+                       "graph_ms_per_step": fp64["graph_ms_per_step"],
                        "pair_kernel_ms": fp64["pair_ms"], "source_chunk_autotuned": fp64["chunk"]}
         out["mixed_over_fp64"] = value / v64
         out["max_abs_drho_vs_fp64"] = float(np.abs(mixed["rho"] - fp64["rho"]).max())
@@ -169,7 +167,10 @@ def table_main(args):
         "config": {"workload": f"{args.workload}: N={n} dots, table of {table_bytes / 1e9:.2f} GB, "
                                f"{full_steps}-step run", "n_dots": n},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None, "kernel": "pair_table_f64<4,false>",
+                     "frac": achieved / 8000.0,
+                     "traffic": (profiled_traffic("pair_table_f64<4, false>", "pair_table_n16384.md")
+                                 if args.workload == "n16384" else None),
+                     "kernel": "pair_table_f64<4,false>",
                      "kernel_ms": tab["pair_ms"], "bytes_per_pair": 82},
         "on_the_fly_fp64": {"value": pairs * args.steps / fly["seconds"], "pair_kernel_ms": fly["pair_ms"]},
         "rel_rho_table_vs_on_the_fly": float(np.abs(tab["rho"] - fly["rho"]).max() / np.abs(fly["rho"]).max()),
@@ -202,18 +203,23 @@ def copy_probe(nbytes: int) -> None:
         hip.hipFree(b)
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from profiles/pair_kernels_n65536_pmc.md
-    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "pair_kernels_n65536_pmc.md")
+def profiled_traffic(kernel: str, profile: str = "pair_kernels_n65536_pmc.md"):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 summary in
+    profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration):
+    the traffic_bytes column of the pair-kernel table, or HBM read + write
+    bytes of the table-kernel one; None if absent."""
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("| `" + kernel + "`") and "traffic_bytes" not in line:
+        if line.startswith("| `" + kernel + "`"):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             try:
-                return float(cells[-1])
-            except ValueError:
+                if profile == "pair_kernels_n65536_pmc.md":
+                    return float(cells[-1])
+                if len(cells) == 7:  # the traffic table (the trace table above it has 5 columns)
+                    return float(cells[2]) + float(cells[3])
+            except (ValueError, IndexError):
                 return None
     return None
 
@@ -308,7 +314,7 @@ def main(argv=None) -> int:
     ordered = float(n) * (n - 1)
     achieved = F_ALG * ordered / (mixed["pair_ms"] * 1e-3) / 1e12
     out = {
-        "metric": "pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N=65536, ρ err≤1e-6",
+        "metric": f"pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N={n}, ρ err≤1e-6",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair arithmetic + f64 accumulation/state",
@@ -321,16 +327,18 @@ def main(argv=None) -> int:
                    "persistent_grid": mixed["info"]["grid"],
                    "ring_slots": mixed["info"]["ring_slots"],
                    "zero_delay_pairs": mixed["info"]["near_pairs"]},
+        "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK,
-                     "traffic": profiled_traffic("pair_past_f32<4, false>") if args.workload == "n65536" else None,
-                     "kernel": "pair_past_f32<4,false>", "kernel_ms": mixed["pair_ms"],
+                     "traffic": profiled_traffic("pair_past_f32<4, false, false>") if args.workload == "n65536" else None,
+                     "kernel": "pair_past_f32<4,false,false>", "kernel_ms": mixed["pair_ms"],
                      "flop_per_pair": F_ALG},
     }
     if fp64 is not None:
         v64 = pairs * args.steps / fp64["seconds"]
         a64 = F_ALG * ordered / (fp64["pair_ms"] * 1e-3) / 1e12
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
+                       "graph_ms_per_step": fp64["graph_ms_per_step"],
                        "pair_kernel_ms": fp64["pair_ms"], "achieved_tflops": a64,
                        "frac_of_fp64_peak": a64 / FP64_PEAK}
         out["mixed_over_fp64"] = value / v64

@@ -68,11 +68,12 @@ typedef struct {
                           (8 chunks of the largest system, halved until the
                           work list fills the GPU); see quest_autotune      */
   int item_claim;      /* how the pair kernel's waves take work items:
-                          QUEST_CLAIM_AUTO (dynamic for batches, static for
-                          one system; quest_autotune times both),
-                          QUEST_CLAIM_STATIC (a fixed stride, XCD-aware) or
-                          QUEST_CLAIM_DYNAMIC (a device counter; balances
-                          items of uneven size)                              */
+                          QUEST_CLAIM_AUTO (static until quest_autotune,
+                          which times both and keeps the faster),
+                          QUEST_CLAIM_STATIC (a fixed stride over an
+                          XCD-aware list: a batch's systems each on one XCD)
+                          or QUEST_CLAIM_DYNAMIC (a device counter; balances
+                          items of uneven size, in list order)               */
 } quest_params;
 
 #define QUEST_CLAIM_AUTO 0

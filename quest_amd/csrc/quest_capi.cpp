@@ -192,26 +192,113 @@ int count_work(const quest_ctx *c, int chunk_len)
 // the chunks of a group of tiles consecutive (see pair_past_*): for systems
 // of at least kBlock dots a block is a kBlock-target tile, as with one item
 // per block, while small systems leave at most a wave's worth of lanes idle.
+// Work list of the pair kernel for a chunk length.  Items come in groups of
+// up to four wave tiles of one system (256 targets) with one source chunk:
+// the waves of one block.  One system, or dynamic claiming: tile groups in
+// order, the chunks of a tile group consecutive (with 8 chunks, round-robin
+// block dispatch gives each XCD one source range).  A batch under the static
+// stride (block i's waves take items 4i..4i+3, then 4(i + grid)..): each
+// system is placed on one XCD (systems longest first, each to the XCD with
+// the least pair work), its groups chunk-major on that XCD's blocks (blocks
+// i = x mod 8), so the source rows of a chunk are fetched into one L2 and read
+// there by all its tile groups together; unused slots hold empty items (one
+// target, dot 0, no sources) whose partial row is a spare row past the real
+// ones.  (Spread over all XCDs instead, the 256-system ensemble hit L2 21 % of
+// the time and fetched 3.4 GB per launch.)
 int build_work(quest_ctx *c, int chunk_len)
 {
   constexpr int kPer = kBlock / kWave;
+  constexpr int kXcds = 8;
   std::vector<int4> work;
   std::vector<int> row;
   std::vector<int> chunks(c->n_sys, 0);
   int max_chunks = 1;
   for(int s = 0; s < c->n_sys; ++s) {
-    const int b = c->sys_begin[s], e = c->sys_begin[s + 1], ns = e - b;
+    const int ns = c->sys_begin[s + 1] - c->sys_begin[s];
     if(ns < 2) continue;
-    const int wtiles = (ns + kWave - 1) / kWave, nch = (ns + chunk_len - 1) / chunk_len;
-    chunks[s] = nch;
-    max_chunks = std::max(max_chunks, nch);
-    for(int g = 0; g < wtiles; g += kPer)
-      for(int q = 0; q < nch; ++q)
-        for(int t = g; t < std::min(wtiles, g + kPer); ++t) {
-          work.push_back(make_int4(b + t * kWave, std::min(e, b + (t + 1) * kWave), b + q * chunk_len,
-                                   std::min(e, b + (q + 1) * chunk_len)));
-          row.push_back(q);
+    chunks[s] = (ns + chunk_len - 1) / chunk_len;
+    max_chunks = std::max(max_chunks, chunks[s]);
+  }
+  // items of tile group g, chunk q of system s
+  auto emit = [&](std::vector<int4> &w, std::vector<int> &r, int s, int g, int q) {
+    const int b = c->sys_begin[s], e = c->sys_begin[s + 1], wtiles = (e - b + kWave - 1) / kWave;
+    for(int t = g; t < std::min(wtiles, g + kPer); ++t) {
+      w.push_back(make_int4(b + t * kWave, std::min(e, b + (t + 1) * kWave), b + q * chunk_len,
+                            std::min(e, b + (q + 1) * chunk_len)));
+      r.push_back(q);
+    }
+  };
+  const int resident = std::max(kXcds, c->cu_count * c->blocks_per_cu / kXcds * kXcds);
+  int grid = 0;
+  if(c->n_sys == 1 || c->k.dyn_items) {
+    for(int s = 0; s < c->n_sys; ++s) {
+      const int ns = c->sys_begin[s + 1] - c->sys_begin[s];
+      if(ns < 2) continue;
+      const int wtiles = (ns + kWave - 1) / kWave;
+      for(int g = 0; g < wtiles; g += kPer)
+        for(int q = 0; q < chunks[s]; ++q) emit(work, row, s, g, q);
+    }
+    // persistent grid: one full wave of resident blocks (a multiple of 8, see
+    // pair_past_*), at most the blocks the items need, rounded up to 8
+    const int blocks8 = ((static_cast<int>(work.size()) + kPer - 1) / kPer + kXcds - 1) / kXcds * kXcds;
+    grid = std::max(kXcds, std::min(resident, blocks8));
+  } else {
+    // systems to XCDs, longest first (pair work ~ wave tiles x sources)
+    std::vector<int> order;
+    std::vector<long long> cost(c->n_sys, 0);
+    int n_groups = 0;
+    for(int s = 0; s < c->n_sys; ++s) {
+      const int ns = c->sys_begin[s + 1] - c->sys_begin[s];
+      if(ns < 2) continue;
+      const int wtiles = (ns + kWave - 1) / kWave;
+      cost[s] = static_cast<long long>(wtiles) * ns;
+      n_groups += (wtiles + kPer - 1) / kPer * chunks[s];
+      order.push_back(s);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    long long load[kXcds] = {0};
+    std::vector<std::vector<int>> on_xcd(kXcds);
+    for(int s : order) {
+      const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
+      on_xcd[x].push_back(s);
+      load[x] += cost[s];
+    }
+    const int blocks8 = (n_groups + kXcds - 1) / kXcds * kXcds;
+    grid = std::max(kXcds, std::min(resident, blocks8));
+    const int per_xcd = grid / kXcds;  // blocks of one XCD: x, x + 8, x + 16, ...
+    // each XCD's groups, chunk-major per system
+    struct Group {
+      int s, g, q;  // system, first wave tile, chunk
+    };
+    std::vector<std::vector<Group>> seq(kXcds);
+    size_t rounds = 0;
+    for(int x = 0; x < kXcds; ++x) {
+      for(int s : on_xcd[x]) {
+        const int wtiles = (c->sys_begin[s + 1] - c->sys_begin[s] + kWave - 1) / kWave;
+        for(int q = 0; q < chunks[s]; ++q)
+          for(int g = 0; g < wtiles; g += kPer) seq[x].push_back({s, g, q});
+      }
+      rounds = std::max(rounds, (seq[x].size() + per_xcd - 1) / per_xcd);
+    }
+    if(static_cast<size_t>(kPer) * grid * rounds > static_cast<size_t>(INT_MAX))
+      return fail(c, QUEST_ERR_ARG, "pair-kernel work list too long");
+    const size_t slots = static_cast<size_t>(kPer) * grid * rounds;
+    work.assign(slots, make_int4(0, 1, 0, 0));  // empty item
+    row.assign(slots, max_chunks);              // the spare row
+    std::vector<int4> w;
+    std::vector<int> r;
+    for(int x = 0; x < kXcds; ++x)
+      for(size_t k = 0; k < seq[x].size(); ++k) {
+        const size_t blk = x + static_cast<size_t>(kXcds) * (k % per_xcd), rnd = k / per_xcd;
+        const size_t base = static_cast<size_t>(kPer) * (blk + static_cast<size_t>(grid) * rnd);
+        w.clear();
+        r.clear();
+        emit(w, r, seq[x][k].s, seq[x][k].g, seq[x][k].q);
+        for(size_t j = 0; j < w.size(); ++j) {
+          work[base + j] = w[j];
+          row[base + j] = r[j];
         }
+      }
   }
   DeviceState &d = c->d;
   drop_graph(c);  // the graph's pair kernel holds the old list
@@ -227,21 +314,17 @@ int build_work(quest_ctx *c, int chunk_len)
     HIPC(c, hipMemcpyAsync(d.work_row, row.data(), sizeof(int) * row.size(), hipMemcpyHostToDevice, c->stream));
   }
   HIPC(c, hipMemcpyAsync(d.sys_chunks, chunks.data(), sizeof(int) * c->n_sys, hipMemcpyHostToDevice, c->stream));
-  if(max_chunks > c->partial_rows) {
+  if(max_chunks + 1 > c->partial_rows) {  // + the spare row of empty items
     if(d.partial) HIPC(c, hipFree(d.partial));
     d.partial = nullptr;
-    HIPC(c, hipMalloc(&d.partial, sizeof(double2) * static_cast<size_t>(c->n) * max_chunks));
-    c->partial_rows = max_chunks;
+    HIPC(c, hipMalloc(&d.partial, sizeof(double2) * static_cast<size_t>(c->n) * (max_chunks + 1)));
+    c->partial_rows = max_chunks + 1;
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   c->chunk_len = chunk_len;
   c->k.n_work = static_cast<int>(work.size());
   c->k.max_chunks = max_chunks;
-  // persistent grid: one full wave of resident blocks (a multiple of 8, see
-  // pair_past_*), at most the blocks the items need, rounded up to 8
-  const int resident = std::max(8, c->cu_count * c->blocks_per_cu / 8 * 8);
-  const int blocks8 = ((c->k.n_work + kPer - 1) / kPer + 7) / 8 * 8;
-  c->k.grid = std::max(8, std::min(resident, blocks8));
+  c->k.grid = grid;
   return QUEST_OK;
 }
 
@@ -666,10 +749,10 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   int rc = upload_pulses(c);
   if(rc != QUEST_OK) return rc;
 
-  // pair-kernel work list sized for one wave of resident blocks; the items of
-  // a batch differ in size, so its waves claim them dynamically
+  // pair-kernel work list sized for one wave of resident blocks; static
+  // claiming unless asked otherwise (quest_autotune times both)
   c->blocks_per_cu = pair_blocks_per_cu(k, p->precision);
-  k.dyn_items = p->item_claim == QUEST_CLAIM_DYNAMIC || (p->item_claim == QUEST_CLAIM_AUTO && n_sys > 1);
+  k.dyn_items = p->item_claim == QUEST_CLAIM_DYNAMIC;
   rc = build_work(c, p->source_chunk > 0 ? p->source_chunk : default_chunk(c));
   if(rc != QUEST_OK) return rc;
 
@@ -1077,11 +1160,11 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
   for(int len : cand) {
-    int rc = build_work(c, len);
-    if(rc != QUEST_OK) return rc;
     for(int dyn : {0, 1}) {
       if(c->P.item_claim != QUEST_CLAIM_AUTO && dyn != (c->P.item_claim == QUEST_CLAIM_DYNAMIC)) continue;
       c->k.dyn_items = dyn;
+      int rc = build_work(c, len);  // the list's layout depends on the claiming mode
+      if(rc != QUEST_OK) return rc;
       HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
       HIPC(c, hipEventRecord(e0, c->stream));
       for(int i = 0; i < launches; ++i) HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));

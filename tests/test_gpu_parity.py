@@ -239,6 +239,20 @@ def test_per_call_sequence_errors():
             ctx.run(1)
 
 
+def test_mixed_ring_plane_limit_is_an_error():
+    """The FP32 pair kernel addresses a coefficient plane (16 B per dot and
+    ring slot) with 32-bit offsets: a mixed-precision context whose plane
+    would pass 4 GiB is refused at create, before any ring is allocated.
+    20000 dots spread over 150 um need 16384 slots: 5.2 GB per plane."""
+    from quest_amd import Context, QuestError, systems
+    rng = np.random.default_rng(8)
+    n = 20000
+    sysd = {"pos": rng.uniform(-75.0, 75.0, size=(n, 3)), "dip": np.tile([0.0, 0.0, systems.DIPOLE], (n, 1)),
+            "freq": np.full(n, systems.OMEGA0), "t1": np.full(n, systems.T1), "t2": np.full(n, systems.T2)}
+    with pytest.raises(QuestError, match="exceeds 2"):
+        Context(sysd, systems.consts(), precision=1)
+
+
 @pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-6)])
 def test_n4096_full_run_matches_oracle_fixture(precision, tol):
     """The N = 4096 system (512 um^-3, pi/2 pulse), all 5000 steps, against the

@@ -45,10 +45,10 @@ def test_step_block_follows_largest_group(n_chain, block):
     assert info["step_block"] == block
 
 
-@pytest.mark.parametrize("n_chain", [100, 300])
+@pytest.mark.parametrize("n_chain", [100, 200, 300])
 def test_grouped_run_matches_oracle(oracle_lib, ref_weights, n_chain):
-    """n_chain 100: one-launch step with 128-thread blocks and padding;
-    n_chain 300: per-iteration kernels."""
+    """n_chain 100 / 200: one-launch step with 128- / 256-thread blocks and
+    padding; n_chain 300: per-iteration kernels."""
     from quest_amd import Context, systems
     oracle = oracle_lib
     consts = systems.consts()
@@ -65,7 +65,7 @@ def test_grouped_run_matches_oracle(oracle_lib, ref_weights, n_chain):
         ctx.run(steps - steps // 2)
         info = ctx.info()
         rho, drho = ctx.state()
-    assert info["step_block"] == (128 if n_chain == 100 else 0)
+    assert info["step_block"] == {100: 128, 200: 256, 300: 0}[n_chain]
     e_rho, e_d = rel_err(rho, ref["y"]), rel_err(drho, ref["yp"])
     print(f"chain {n_chain}: block {info['step_block']}, rho rel err {e_rho:.3e}, drho {e_d:.3e}")
     assert np.abs(ref["y"][:, 1]).max() > 0.1, "the pulse did not drive the system"

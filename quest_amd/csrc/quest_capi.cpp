@@ -1156,9 +1156,17 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   HIPC(c, launch_set_step(c->d, std::max(c->cur_step, 0), c->stream));
   int best = c->chunk_len, best_dyn = c->k.dyn_items;
   float best_ms = INFINITY;
-  hipEvent_t e0, e1;
-  HIPC(c, hipEventCreate(&e0));
-  HIPC(c, hipEventCreate(&e1));
+  struct Events {  // destroyed on every return path
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events()
+    {
+      if(e0) (void)hipEventDestroy(e0);
+      if(e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  HIPC(c, hipEventCreate(&ev.e0));
+  HIPC(c, hipEventCreate(&ev.e1));
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   for(int len : cand) {
     for(int dyn : {0, 1}) {
       if(c->P.item_claim != QUEST_CLAIM_AUTO && dyn != (c->P.item_claim == QUEST_CLAIM_DYNAMIC)) continue;
@@ -1179,8 +1187,6 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
       }
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   c->k.dyn_items = best_dyn;
   int rc = build_work(c, best);
   if(rc != QUEST_OK) return rc;

@@ -637,12 +637,20 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   k.m_cdt_l = k.m_cdt / (cdt * cdt) / k.f32_scale;
   k.m_o2dt_l = k.m_o2dt / cdt / k.f32_scale;
   {
-    const double t = k.theta, t2 = t * t;
-    k.sin_c[0] = t;
-    k.sin_c[1] = -t * t2 / 6.0;
-    k.sin_c[2] = t * t2 * t2 / 120.0;
+    // FP32 rotation e^{-i theta xc}, |xc| <= 1/2, u = theta xc, |u| <= U = theta/2:
+    // sin u = u - u^3/6 + u^5/120 - ... with u^5 economized over [-U, U]
+    // (T5: t^5 ~ (20 t^3 - 5 t)/16, t = u/U), error <= U^5/1920 + U^7/5040;
+    // cos u = 1 - u^2/2 + u^4/24, error <= U^6/720.  Both stay below 3e-8 (half
+    // an FP32 unit of a unit term) for theta <= 0.28.  The FP64 series
+    // (sincos_small, u = theta x in [0, theta)) stays below 1e-16 for
+    // theta <= 0.33.  Beyond these the pair kernels call the library sincos.
+    const double t = k.theta, t2 = t * t, U = 0.5 * std::fabs(t), U2 = U * U;
+    k.sin_c[0] = t * (1.0 - 5.0 * U2 * U2 / 1920.0);
+    k.sin_c[1] = t * t2 * (-1.0 / 6.0 + U2 / 96.0);
+    k.sin_c[2] = 0.0;
     k.cos_c[0] = -t2 / 2.0;
     k.cos_c[1] = t2 * t2 / 24.0;
+    k.trig_poly = std::fabs(t) <= (p->precision == QUEST_PREC_FP64 ? 0.33 : 0.28) ? 1 : 0;
   }
   lagrange_monomials(p->interp_order, k.vinv);
   centre_basis(p->interp_order, k.vinv, k.vinv_c);

@@ -41,9 +41,13 @@ struct Consts {
   // divided by f32_scale = two_idt2 / L (which the FP32 kernel applies once
   // per target, in FP64): cc / L^3, oo / L, oc / L^2, m_cdt / L^2, m_o2dt / L
   double cc_l, oo_l, oc_l, m_cdt_l, m_o2dt_l, f32_scale;
-  // FP32 sin/cos of theta x, |x| <= 1/2: theta, -theta^3/6, theta^5/120 and
-  // -theta^2/2, theta^4/24
+  // FP32 sin/cos of theta x, |x| <= 1/2 (sincos_theta): sin_c[0..1] the
+  // Chebyshev-economized u - u^3/6 + u^5/120 over |u| <= theta/2 (sin_c[2] = 0),
+  // cos_c = -theta^2/2, theta^4/24
   double sin_c[3], cos_c[2];
+  // 1: the rotation e^{-i theta x} of the pair term uses those short series
+  // (theta small enough for rounding-level accuracy); 0: library sincos
+  int trig_poly;
   // error bound of the FP32 delay estimate: eps = eps_rel tau + eps_abs
   double eps_rel, eps_abs;
   // monomial coefficients of the Lagrange basis: vinv[k][m] = [x^k] L_m(x)

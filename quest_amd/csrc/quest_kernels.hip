@@ -143,19 +143,20 @@ __device__ __forceinline__ int split_mixed(float dx, float dy, float dz, float e
   return code;  // bit pattern of ms - n
 }
 
-// sin and cos of theta x for x in [-1/2, 1/2), |theta x| <= 0.114: Taylor
-// series in x^2 with theta folded into the coefficients (sn[] = theta,
-// -theta^3/6, theta^5/120; cs[] = -theta^2/2, theta^4/24), truncated where
-// the next term (u^7/5040 <= 4e-11, u^6/720 <= 3e-9) is below FP32 rounding.
+// sin and cos of theta x for x in [-1/2, 1/2), used while theta <= 0.28
+// (|theta x| <= 0.14): series in x^2 with theta folded into the coefficients;
+// sn[] = the sine's two Chebyshev-economized terms (error <= (theta/2)^5/1920,
+// 1e-8 at theta = 0.228), cs[] = -theta^2/2, theta^4/24 (error <= (theta/2)^6/720,
+// 3e-9 at theta = 0.228): below FP32 rounding of the term.
 __device__ __forceinline__ void sincos_theta(const float *sn, const float *cs, float x, float &s,
                                              float &c)
 {
   const float x2 = x * x;
-  s = x * __builtin_fmaf(x2, __builtin_fmaf(x2, sn[2], sn[1]), sn[0]);
+  s = x * __builtin_fmaf(x2, sn[1], sn[0]);
   c = __builtin_fmaf(x2, __builtin_fmaf(x2, cs[1], cs[0]), 1.0f);
 }
-// sin and cos of u in [0, 0.23): Taylor series (Horner in u^2), truncated
-// where the next term is below FP64 rounding.
+// sin and cos of u in [0, 0.33): Taylor series (Horner in u^2), truncated
+// where the next term is below FP64 rounding (used while theta <= 0.33).
 __device__ __forceinline__ void sincos_small(double u, double &s, double &c)
 {
   const double u2 = u * u;
@@ -255,14 +256,14 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
 template <typename R> struct PairK {
   static constexpr bool kF32 = sizeof(R) == sizeof(float);
   R cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, lim0;
-  R sn[3], cs[2];  // sincos_theta coefficients (FP32)
+  R sn[2], cs[2];  // sincos_theta coefficients (FP32)
   double cdt, icdt64;
   __host__ __device__ explicit PairK(const Consts &k)
       : cc((R)(kF32 ? k.cc_l : k.cc)), oo((R)(kF32 ? k.oo_l : k.oo)), oc((R)(kF32 ? k.oc_l : k.oc)),
         m_cdt((R)(kF32 ? k.m_cdt_l : k.m_cdt)), m_o2dt((R)(kF32 ? k.m_o2dt_l : k.m_o2dt)),
         two_idt2((R)(kF32 ? 1.0 : k.two_idt2)), theta((R)k.theta), eps_rel((R)k.eps_rel),
         lim0((R)(0.5 - k.eps_abs - 0.5 * k.eps_rel)),
-        sn{(R)k.sin_c[0], (R)k.sin_c[1], (R)k.sin_c[2]}, cs{(R)k.cos_c[0], (R)k.cos_c[1]},
+        sn{(R)k.sin_c[0], (R)k.sin_c[1]}, cs{(R)k.cos_c[0], (R)k.cos_c[1]},
         cdt(k.cdt), icdt64(k.icdt) {}
 };
 
@@ -278,7 +279,9 @@ template <typename R> struct PairK {
 //   G3 = (c^2/r) q0 + i (w c) q0 - (c/dt) q1,
 //   G1 = (2/dt^2) q2 - w^2 q0 - i (2w/dt) q1,
 // and the rotated term is added to (accr, acci) in the same fused operations.
-template <typename R, bool LAB>
+// GEN: e^{-i theta x} from the library sincos (theta beyond the short series'
+// range); a separate kernel variant, so the common one carries none of it.
+template <typename R, bool LAB, bool GEN = false>
 __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R dz,
                                           const typename V4<R>::type &da,
                                           const typename V4<R>::type &db, R x, R q0r, R q0i,
@@ -308,10 +311,16 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
     const R cr = g3 * g3r + g1 * g1r;
     const R ci = g3 * g3i + g1 * g1i;
     R sn, cs;
-    if constexpr(sizeof(R) == sizeof(float))
+    if constexpr(GEN) {
+      if constexpr(sizeof(R) == sizeof(float))
+        __sincosf(K.theta * x, &sn, &cs);
+      else
+        sincos(K.theta * x, &sn, &cs);
+    } else if constexpr(sizeof(R) == sizeof(float)) {
       sincos_theta(K.sn, K.cs, x, sn, cs);
-    else
+    } else {
       sincos_small(K.theta * x, sn, cs);
+    }
     accr = fmaT(cr, cs, fmaT(ci, sn, accr));
     acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
   }
@@ -385,7 +394,7 @@ constexpr int kChunk = 64;
 #define QUEST_F64_ATTR
 #endif   // sources summed in the pair type before the FP64 fold
 
-template <int M, bool LAB, bool DYN>
+template <int M, bool LAB, bool DYN, bool GEN>
 __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
     const double4 *__restrict__ pos, const double4 *__restrict__ dip,
     const double2 *__restrict__ ring, const int4 *__restrict__ work, const int *__restrict__ work_row,
@@ -415,7 +424,7 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
       for(int i = 0; i <= M; ++i) cv[i] = load_off(c + i, off);
       double q0r, q0i, q1r, q1i, q2r, q2i;
       horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-      pair_term<double, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
+      pair_term<double, LAB, GEN>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
     }
     if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
   }
@@ -436,7 +445,7 @@ struct RingPlanesF32 {
   const float2 *last;     // plane of c_M (even M)
 };
 
-template <int M, bool LAB, bool DYN>
+template <int M, bool LAB, bool DYN, bool GEN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void pair_past_f32(
     const float4 *__restrict__ pd, const double4 *__restrict__ pos64, const RingPlanesF32 planes,
     const int4 *__restrict__ work, const int *__restrict__ work_row, int *step_ptr,
@@ -482,7 +491,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
         if constexpr(2 * kPairs == M) cv[M] = load_off(planes.last, off8);
         float q0r, q0i, q1r, q1i, q2r, q2i;
         horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-        pair_term<float, LAB>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, sr, si);
+        pair_term<float, LAB, GEN>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, sr, si);
       }
       accr += static_cast<double>(sr);
       acci += static_cast<double>(si);
@@ -1097,12 +1106,12 @@ __global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_
 
 inline dim3 dot_grid(int n) { return dim3((n + kBlock - 1) / kBlock); }
 
-template <int M, bool LAB, bool DYN>
+template <int M, bool LAB, bool DYN, bool GEN>
 void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
   const dim3 grid(k.grid), block(kBlock);
   if(precision == QUEST_PREC_FP64)
-    hipLaunchKernelGGL((pair_past_f64<M, LAB, DYN>), grid, block, 0, s, d.pos64, d.dip64,
+    hipLaunchKernelGGL((pair_past_f64<M, LAB, DYN, GEN>), grid, block, 0, s, d.pos64, d.dip64,
                        static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
   else {
     // FP32 ring layout (see ring_index): (M + 1) / 2 float4 planes, then the c_M plane
@@ -1111,7 +1120,7 @@ void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t
     RingPlanesF32 planes{};
     for(int i = 0; i < (M + 1) / 2; ++i) planes.pair[i] = r4 + i * E;
     planes.last = reinterpret_cast<const float2 *>(r4 + ((M + 1) / 2) * E);
-    hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN>), grid, block, 0, s, d.pd32, d.pos64, planes, d.work,
+    hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN>), grid, block, 0, s, d.pd32, d.pos64, planes, d.work,
                        d.work_row, d.step, d.partial, k);
   }
 }
@@ -1120,11 +1129,15 @@ template <int M>
 hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
   const bool lab = k.dyadic == QUEST_DYADIC_LAB;
-  if(lab)
-    k.dyn_items ? launch_pair_mld<M, true, true>(k, d, precision, s) : launch_pair_mld<M, true, false>(k, d, precision, s);
+  if(lab)  // no rotation in the lab frame
+    k.dyn_items ? launch_pair_mld<M, true, true, false>(k, d, precision, s)
+                : launch_pair_mld<M, true, false, false>(k, d, precision, s);
+  else if(k.trig_poly)
+    k.dyn_items ? launch_pair_mld<M, false, true, false>(k, d, precision, s)
+                : launch_pair_mld<M, false, false, false>(k, d, precision, s);
   else
-    k.dyn_items ? launch_pair_mld<M, false, true>(k, d, precision, s)
-                : launch_pair_mld<M, false, false>(k, d, precision, s);
+    k.dyn_items ? launch_pair_mld<M, false, true, true>(k, d, precision, s)
+                : launch_pair_mld<M, false, false, true>(k, d, precision, s);
   return hipGetLastError();
 }
 
@@ -1142,16 +1155,18 @@ hipError_t launch_table_m(const Consts &k, DeviceState &d, hipStream_t s)
 }
 
 template <int M>
-int blocks_per_cu_m(int precision, bool table)
+int blocks_per_cu_m(int precision, bool table, bool gen)
 {
   int nb = 0;
   hipError_t e;
   if(table)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_table_f64<M, false>, kBlock, 0);
   else if(precision == QUEST_PREC_FP64)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false>, kBlock, 0);
+    e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false, true>, kBlock, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false, false>, kBlock, 0);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false>, kBlock, 0);
+    e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, true>, kBlock, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, false>, kBlock, 0);
   return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
@@ -1237,13 +1252,14 @@ hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipS
 int pair_blocks_per_cu(const Consts &k, int precision)
 {
   const bool t = k.pair_table != 0;
+  const bool g = k.dyadic != QUEST_DYADIC_LAB && !k.trig_poly;
   switch(k.order) {
-  case 1: return blocks_per_cu_m<1>(precision, t);
-  case 2: return blocks_per_cu_m<2>(precision, t);
-  case 3: return blocks_per_cu_m<3>(precision, t);
-  case 4: return blocks_per_cu_m<4>(precision, t);
-  case 5: return blocks_per_cu_m<5>(precision, t);
-  default: return blocks_per_cu_m<6>(precision, t);
+  case 1: return blocks_per_cu_m<1>(precision, t, g);
+  case 2: return blocks_per_cu_m<2>(precision, t, g);
+  case 3: return blocks_per_cu_m<3>(precision, t, g);
+  case 4: return blocks_per_cu_m<4>(precision, t, g);
+  case 5: return blocks_per_cu_m<5>(precision, t, g);
+  default: return blocks_per_cu_m<6>(precision, t, g);
   }
 }
 

@@ -307,3 +307,31 @@ def test_full_size_mixed_matches_fp64(ref_weights, name):
     assert big > 0.1
     assert np.all(np.isfinite(out[1]))
     assert err01 < 1e-6 and err00 < 1e-6
+
+
+@pytest.mark.parametrize("dt", [1.3e-4, 1e-3])
+def test_large_laser_phase_per_step_matches_oracle(oracle_lib, ref_weights, dt):
+    """theta = omega dt beyond the short rotation series (FP32 series up to
+    0.28, FP64 up to 0.33): dt = 1.3e-4 ps (theta = 0.296: FP64 series, FP32
+    library sincos) and 1e-3 ps (theta = 2.28: library sincos in both).  The
+    FP64 trajectory matches the oracle (complex exp of the reference,
+    rotating_green_function.cpp:14) to 1e-10 and mixed stays within 1e-6 of
+    it."""
+    from quest_amd import systems
+    oracle = oracle_lib
+    consts = systems.consts(dt=dt)
+    side = 0.22 if dt < 5e-4 else 1.6
+    sysd = dense_system(48, side, seed=23, plant_near=False)
+    steps = 160 if dt < 5e-4 else 120
+    pl = short_pulse(delay=steps * dt * 0.4, sigma_t=steps * dt * 0.1)
+    ps, cs, fc = ref_weights
+    ref = oracle.solve(oracle_params(oracle, consts), sysd, oracle_pulse(oracle, pl), ps, cs, fc, steps, mode=1)
+    assert ref["status"] == 0
+    r64, d64 = _run_gpu(sysd, consts, pl, ref_weights, steps, precision=0)
+    r32, _ = _run_gpu(sysd, consts, pl, ref_weights, steps, precision=1)
+    e64 = rel_err(r64, ref["y"])
+    e32 = float(np.abs(r32 - r64).max())
+    print(f"dt={dt}: theta={consts['laser_omega'] * dt:.3f} fp64 rel err {e64:.2e}, mixed |d rho| {e32:.2e}, "
+          f"max|rho01| {np.abs(ref['y'][:, 1]).max():.3f}")
+    assert np.abs(ref["y"][:, 1]).max() > 0.1, "the pulse did not drive the system"
+    assert e64 < 1e-10 and e32 < 1e-6

@@ -603,6 +603,8 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
     diag = std::max(diag, std::sqrt(dd));
   }
   const double cdt = p->c0 * p->dt;
+  // delays below 1e6 steps: also keeps the FP32 delay split's rounding
+  // constant (exact below 2^22 steps) valid
   if(!(diag / cdt < 1e6)) return fail(c, QUEST_ERR_ARG, "system extent / (c0 dt) too large");
   c->dmax_bound = static_cast<int>(std::floor(diag / cdt)) + 2;
 

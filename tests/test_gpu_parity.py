@@ -253,6 +253,18 @@ def test_mixed_ring_plane_limit_is_an_error():
         Context(sysd, systems.consts(), precision=1)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_delay_range_limit_is_an_error(precision):
+    """Delays are limited to 1e6 steps (which also keeps the FP32 delay
+    split's float rounding constant, exact below 2^22 steps, valid): two dots
+    1.5e5 um apart (5e6 steps at c0 dt = 0.03 um) are refused at create."""
+    from quest_amd import Context, QuestError, systems
+    sysd = {"pos": np.array([[0.0, 0.0, 0.0], [1.5e5, 0.0, 0.0]]), "dip": np.tile([0.0, 0.0, systems.DIPOLE], (2, 1)),
+            "freq": np.full(2, systems.OMEGA0), "t1": np.full(2, systems.T1), "t2": np.full(2, systems.T2)}
+    with pytest.raises(QuestError, match="too large"):
+        Context(sysd, systems.consts(), precision=precision)
+
+
 @pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-6)])
 def test_n4096_full_run_matches_oracle_fixture(precision, tol):
     """The N = 4096 system (512 um^-3, pi/2 pulse), all 5000 steps, against the

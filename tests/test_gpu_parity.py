@@ -347,3 +347,30 @@ def test_large_laser_phase_per_step_matches_oracle(oracle_lib, ref_weights, dt):
           f"max|rho01| {np.abs(ref['y'][:, 1]).max():.3f}")
     assert np.abs(ref["y"][:, 1]).max() > 0.1, "the pulse did not drive the system"
     assert e64 < 1e-10 and e32 < 1e-6
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 5, 6])
+@pytest.mark.parametrize("precision,dyadic", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_run_every_order_matches_oracle(oracle_lib, ref_weights, order, precision, dyadic):
+    """Whole runs at interpolation orders 1-6 other than 4 (the FP32 ring's
+    plane layout differs with the order: pairs of coefficients, plus a last
+    plane when the order is even), rotating and lab dyadic: FP64 within 1e-10
+    of the oracle, mixed within 1e-6 (absolute, |rho| <= 1)."""
+    from quest_amd import Context, systems
+    oracle = oracle_lib
+    consts = systems.consts()
+    sysd = dense_system(40, 0.24, seed=50 + order)
+    pl = short_pulse()
+    steps = 150
+    ps, cs, fc = ref_weights
+    ref = oracle.solve(oracle_params(oracle, consts, order=order), sysd, oracle_pulse(oracle, pl), ps, cs, fc,
+                       steps, mode=1, kind=dyadic)
+    assert ref["status"] == 0
+    with Context(sysd, consts, precision=precision, dyadic=dyadic, order=order, pulse=pl) as ctx:
+        ctx.set_pc_weights(ps, cs, fc)
+        ctx.run(steps)
+        rho, _ = ctx.state()
+    err = rel_err(rho, ref["y"]) if precision == 0 else float(np.abs(rho - ref["y"]).max())
+    print(f"order {order} precision {precision} dyadic {dyadic}: err {err:.2e}")
+    assert np.abs(ref["y"][:, 1]).max() > 0.1
+    assert err < (1e-10 if precision == 0 else 1e-6)

@@ -633,25 +633,35 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   k.m_o2dt = -2 * p->laser_omega / p->dt;
   k.two_idt2 = 2 / (p->dt * p->dt);
   k.f32_scale = k.two_idt2 / cdt;
-  k.cc_l = k.cc / (cdt * cdt * cdt) / k.f32_scale;
+  // In these units (lengths in cdt, the term divided by f32_scale) the pair
+  // constants are c^2 -> 1/2, w c -> theta/2, c/dt -> 1/2, 2w/dt -> theta and
+  // w^2 -> theta^2/2 (oo_l); the FP32 pair term is written with those values.
   k.oo_l = k.oo / cdt / k.f32_scale;
-  k.oc_l = k.oc / (cdt * cdt) / k.f32_scale;
-  k.m_cdt_l = k.m_cdt / (cdt * cdt) / k.f32_scale;
-  k.m_o2dt_l = k.m_o2dt / cdt / k.f32_scale;
   {
     // FP32 rotation e^{-i theta xc}, |xc| <= 1/2, u = theta xc, |u| <= U = theta/2:
     // sin u = u - u^3/6 + u^5/120 - ... with u^5 economized over [-U, U]
     // (T5: t^5 ~ (20 t^3 - 5 t)/16, t = u/U), error <= U^5/1920 + U^7/5040;
     // cos u = 1 - u^2/2 + u^4/24, error <= U^6/720.  Both stay below 3e-8 (half
     // an FP32 unit of a unit term) for theta <= 0.28.  The FP64 series
-    // (sincos_small, u = theta x in [0, theta)) stays below 1e-16 for
-    // theta <= 0.33.  Beyond these the pair kernels call the library sincos.
+    // (sincos_small, u = theta x in [0, theta), terms to u^11 and u^12) stays
+    // below 1e-16 for theta <= 0.33.  Beyond these the pair kernels call the
+    // library sincos.
     const double t = k.theta, t2 = t * t, U = 0.5 * std::fabs(t), U2 = U * U;
     k.sin_c[0] = t * (1.0 - 5.0 * U2 * U2 / 1920.0);
     k.sin_c[1] = t * t2 * (-1.0 / 6.0 + U2 / 96.0);
     k.sin_c[2] = 0.0;
     k.cos_c[0] = -t2 / 2.0;
     k.cos_c[1] = t2 * t2 / 24.0;
+    // the FP64 series' coefficients in x: sin: (-1)^k t^(2k+1)/(2k+1)!, k = 5..0;
+    // cos: (-1)^k t^(2k)/(2k)!, k = 6..1
+    for(int j = 0; j < 6; ++j) {
+      const int ks = 5 - j, kc = 6 - j;
+      double a = t, b = 1.0;  // t^(2 ks + 1) / (2 ks + 1)!, t^(2 kc) / (2 kc)!
+      for(int i = 2; i <= 2 * ks + 1; ++i) a = a * t / i;
+      for(int i = 1; i <= 2 * kc; ++i) b = b * t / i;
+      k.trig64[j] = (ks % 2 ? -a : a);
+      k.trig64[6 + j] = (kc % 2 ? -b : b);
+    }
     k.trig_poly = std::fabs(t) <= (p->precision == QUEST_PREC_FP64 ? 0.33 : 0.28) ? 1 : 0;
   }
   lagrange_monomials(p->interp_order, k.vinv);

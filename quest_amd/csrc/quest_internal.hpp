@@ -37,14 +37,19 @@ struct Consts {
   double hbar, laser_omega;
   // combination constants of the pair term, see pair_term()
   double cc, oo, oc, m_cdt, m_o2dt, two_idt2;
-  // the same for the FP32 pair sum, whose positions are in units of L = c0 dt,
-  // divided by f32_scale = two_idt2 / L (which the FP32 kernel applies once
-  // per target, in FP64): cc / L^3, oo / L, oc / L^2, m_cdt / L^2, m_o2dt / L
-  double cc_l, oo_l, oc_l, m_cdt_l, m_o2dt_l, f32_scale;
+  // the FP32 pair sum's positions are in units of L = c0 dt and its terms are
+  // divided by f32_scale = two_idt2 / L (applied once per target, in FP64);
+  // there w^2 becomes oo_l = oo / L / f32_scale = theta^2 / 2
+  double oo_l, f32_scale;
   // FP32 sin/cos of theta x, |x| <= 1/2 (sincos_theta): sin_c[0..1] the
   // Chebyshev-economized u - u^3/6 + u^5/120 over |u| <= theta/2 (sin_c[2] = 0),
   // cos_c = -theta^2/2, theta^4/24
   double sin_c[3], cos_c[2];
+  // FP64 sin/cos of theta x, x in [0, 1), theta <= 0.33 (sincos_small): Taylor
+  // series in x^2 with theta folded in, highest first: trig64[0..5] =
+  // (-1)^k theta^(2k+1) / (2k+1)!, k = 5..0; trig64[6..11] = (-1)^k theta^(2k) / (2k)!,
+  // k = 6..1 (the constant 1 is the last addend)
+  double trig64[12];
   // 1: the rotation e^{-i theta x} of the pair term uses those short series
   // (theta small enough for rounding-level accuracy); 0: library sincos
   int trig_poly;

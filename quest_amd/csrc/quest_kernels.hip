@@ -155,25 +155,21 @@ __device__ __forceinline__ void sincos_theta(const float *sn, const float *cs, f
   s = x * __builtin_fmaf(x2, sn[1], sn[0]);
   c = __builtin_fmaf(x2, __builtin_fmaf(x2, cs[1], cs[0]), 1.0f);
 }
-// sin and cos of u in [0, 0.33): Taylor series (Horner in u^2), truncated
-// where the next term is below FP64 rounding (used while theta <= 0.33).
-__device__ __forceinline__ void sincos_small(double u, double &s, double &c)
+// sin and cos of theta x for x in [0, 1), used while theta <= 0.33: Taylor
+// series in x^2 with theta folded into the coefficients (Consts::trig64,
+// kernel arguments, so each FMA takes its coefficient from a scalar register),
+// truncated where the next term is below FP64 rounding.
+__device__ __forceinline__ void sincos_small(const double *tg, double x, double &s, double &c)
 {
-  const double u2 = u * u;
-  double ps = -1.0 / 39916800.0;
-  ps = __builtin_fma(ps, u2, 1.0 / 362880.0);
-  ps = __builtin_fma(ps, u2, -1.0 / 5040.0);
-  ps = __builtin_fma(ps, u2, 1.0 / 120.0);
-  ps = __builtin_fma(ps, u2, -1.0 / 6.0);
-  ps = __builtin_fma(ps, u2, 1.0);
-  s = u * ps;
-  double pc = 1.0 / 479001600.0;
-  pc = __builtin_fma(pc, u2, -1.0 / 3628800.0);
-  pc = __builtin_fma(pc, u2, 1.0 / 40320.0);
-  pc = __builtin_fma(pc, u2, -1.0 / 720.0);
-  pc = __builtin_fma(pc, u2, 1.0 / 24.0);
-  pc = __builtin_fma(pc, u2, -0.5);
-  c = __builtin_fma(pc, u2, 1.0);
+  const double x2 = x * x;
+  double ps = tg[0];
+#pragma unroll
+  for(int k = 1; k < 6; ++k) ps = __builtin_fma(ps, x2, tg[k]);
+  s = x * ps;
+  double pc = tg[6];
+#pragma unroll
+  for(int k = 7; k < 12; ++k) pc = __builtin_fma(pc, x2, tg[k]);
+  c = __builtin_fma(pc, x2, 1.0);
 }
 
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -254,17 +250,24 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
 // multiply them carry the inverse factors (the *_l constants), all divided by
 // f32_scale = two_idt2 / L so that the g1 q2 term needs no multiply.
 template <typename R> struct PairK {
-  static constexpr bool kF32 = sizeof(R) == sizeof(float);
-  R cc, oo, oc, m_cdt, m_o2dt, two_idt2, theta, eps_rel, lim0;
+  R oo, theta, eps_rel, lim0;
   R sn[2], cs[2];  // sincos_theta coefficients (FP32)
+  // FP64 pair_term only: the constants divided by two_idt2 (the kernel
+  // multiplies each target's sum by two_idt2 once), with m_cdt factored out of
+  // G3: tc = cc / m_cdt, ocm = oc / m_cdt, g3m = m_cdt / two_idt2, mo = m_o2dt / two_idt2
+  R tc, ocm, g3m, mo;
+  double tg[12];  // sincos_small coefficients (FP64)
   double cdt, icdt64;
+  // FP32: oo = theta^2 / 2 (oo_l); FP64: oo = w^2 / two_idt2
   __host__ __device__ explicit PairK(const Consts &k)
-      : cc((R)(kF32 ? k.cc_l : k.cc)), oo((R)(kF32 ? k.oo_l : k.oo)), oc((R)(kF32 ? k.oc_l : k.oc)),
-        m_cdt((R)(kF32 ? k.m_cdt_l : k.m_cdt)), m_o2dt((R)(kF32 ? k.m_o2dt_l : k.m_o2dt)),
-        two_idt2((R)(kF32 ? 1.0 : k.two_idt2)), theta((R)k.theta), eps_rel((R)k.eps_rel),
-        lim0((R)(0.5 - k.eps_abs - 0.5 * k.eps_rel)),
+      : oo((R)(sizeof(R) == sizeof(float) ? k.oo_l : k.oo / k.two_idt2)), theta((R)k.theta),
+        eps_rel((R)k.eps_rel), lim0((R)(0.5 - k.eps_abs - 0.5 * k.eps_rel)),
         sn{(R)k.sin_c[0], (R)k.sin_c[1]}, cs{(R)k.cos_c[0], (R)k.cos_c[1]},
-        cdt(k.cdt), icdt64(k.icdt) {}
+        tc((R)(k.cc / k.m_cdt)), ocm((R)(k.oc / k.m_cdt)), g3m((R)(k.m_cdt / k.two_idt2)),
+        mo((R)(k.m_o2dt / k.two_idt2)), cdt(k.cdt), icdt64(k.icdt)
+  {
+    for(int i = 0; i < 12; ++i) tg[i] = k.trig64[i];
+  }
 };
 
 // The field at a target from one source, given the geometry and the source's
@@ -279,48 +282,91 @@ template <typename R> struct PairK {
 //   G3 = (c^2/r) q0 + i (w c) q0 - (c/dt) q1,
 //   G1 = (2/dt^2) q2 - w^2 q0 - i (2w/dt) q1,
 // and the rotated term is added to (accr, acci) in the same fused operations.
-// GEN: e^{-i theta x} from the library sincos (theta beyond the short series'
-// range); a separate kernel variant, so the common one carries none of it.
-template <typename R, bool LAB, bool GEN = false>
-__device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R dz,
-                                          const typename V4<R>::type &da,
-                                          const typename V4<R>::type &db, R x, R q0r, R q0i,
-                                          R q1r, R q1i, R q2r, R q2i, R &accr, R &acci)
+// This FP64 form carries the whole term divided by 2/dt^2 (the kernel scales
+// each target's sum back once) and m_cdt = -c/dt factored out of G3:
+//   g3 G3 / (2/dt^2) = g3m' [tc (1/r) q0 + i ocm q0 + q1],  g3m' = g3 m_cdt dt^2/2,
+//   G1 / (2/dt^2) = q2 - oo q0 - i mo q1,
+// with tc = c^2 / m_cdt, ocm = w c / m_cdt, oo = w^2 dt^2/2, mo = -w dt
+// (PairK).  GEN: e^{-i theta x} from the library sincos (theta beyond the
+// short series' range); a separate kernel variant, so the common one carries
+// none of it.
+template <bool LAB, bool GEN = false>
+__device__ __forceinline__ void pair_term_f64(const PairK<double> &K, double ir, double dx, double dy,
+                                              double dz, const double4 &da, const double4 &db, double x,
+                                              double q0r, double q0i, double q1r, double q1i, double q2r,
+                                              double q2i, double &accr, double &acci)
 {
-  const R ir2 = ir * ir;
-  const R ba = da.x * dx + da.y * dy + da.z * dz;
-  const R bb = db.x * dx + db.y * dy + db.z * dz;
-  const R aab = da.x * db.x + da.y * db.y + da.z * db.z;
-  const R bq = ba * bb * ir2;
-  const R g3 = (aab - 3 * bq) * ir2;
-  const R g1 = (aab - bq) * ir;
-  // FP32 constants are divided by two_idt2, so there the q2 factor is 1
-  const R h2r = PairK<R>::kF32 ? q2r : K.two_idt2 * q2r;
-  const R h2i = PairK<R>::kF32 ? q2i : K.two_idt2 * q2i;
-  const R t = K.cc * ir;
+  const double ir2 = ir * ir;
+  const double ba = da.x * dx + da.y * dy + da.z * dz;
+  const double bb = db.x * dx + db.y * dy + db.z * dz;
+  const double aab = da.x * db.x + da.y * db.y + da.z * db.z;
+  const double bq = ba * bb * ir2;
+  const double g3 = (aab - 3 * bq) * (ir2 * K.g3m);
+  const double g1 = (aab - bq) * ir;
+  const double t = K.tc * ir;
   if constexpr(LAB) {
-    const R g3r = t * q0r + K.m_cdt * q1r;
-    const R g3i = t * q0i + K.m_cdt * q1i;
-    accr = fmaT(g3, g3r, fmaT(g1, h2r, accr));
-    acci = fmaT(g3, g3i, fmaT(g1, h2i, acci));
+    const double g3r = fmaT(t, q0r, q1r);
+    const double g3i = fmaT(t, q0i, q1i);
+    accr = fmaT(g3, g3r, fmaT(g1, q2r, accr));
+    acci = fmaT(g3, g3i, fmaT(g1, q2i, acci));
   } else {
-    const R g3r = t * q0r - K.oc * q0i + K.m_cdt * q1r;
-    const R g3i = t * q0i + K.oc * q0r + K.m_cdt * q1i;
-    const R g1r = h2r - K.oo * q0r - K.m_o2dt * q1i;
-    const R g1i = h2i - K.oo * q0i + K.m_o2dt * q1r;
-    const R cr = g3 * g3r + g1 * g1r;
-    const R ci = g3 * g3i + g1 * g1i;
-    R sn, cs;
-    if constexpr(GEN) {
-      if constexpr(sizeof(R) == sizeof(float))
-        __sincosf(K.theta * x, &sn, &cs);
-      else
-        sincos(K.theta * x, &sn, &cs);
-    } else if constexpr(sizeof(R) == sizeof(float)) {
+    const double g3r = fmaT(t, q0r, fmaT(-K.ocm, q0i, q1r));
+    const double g3i = fmaT(t, q0i, fmaT(K.ocm, q0r, q1i));
+    const double g1r = fmaT(-K.oo, q0r, fmaT(-K.mo, q1i, q2r));
+    const double g1i = fmaT(-K.oo, q0i, fmaT(K.mo, q1r, q2i));
+    const double cr = g3 * g3r + g1 * g1r;
+    const double ci = g3 * g3i + g1 * g1i;
+    double sn, cs;
+    if constexpr(GEN)
+      sincos(K.theta * x, &sn, &cs);
+    else
+      sincos_small(K.tg, x, sn, cs);
+    accr = fmaT(cr, cs, fmaT(ci, sn, accr));
+    acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
+  }
+}
+
+// The same term in the FP32 pair arithmetic's units (lengths in L = c0 dt,
+// the whole term divided by f32_scale = 2 / (dt^2 L)).  There the constants
+// of G3 and G1 reduce to
+//   c^2/r -> (1/2) ir,  w c -> theta / 2,  c/dt -> 1/2,
+//   w^2 -> theta^2 / 2,  2w/dt -> theta,  2/dt^2 -> 1      (theta = w dt),
+// so with g3h = g3 / 2
+//   g3 G3 = g3h [ir q0 - q1 + i theta q0],  G1 = q2 - (theta^2/2) q0 - i theta q1.
+// g3h and g1 come from the halved target dipole dah = d_a / 2 (an exact
+// power-of-two scale): aab/2 = dah.d_b, g3h = (aab/2 - (3/2) B) / r^2 and
+// g1 = (2 (aab/2) - B) / r, the same value as (aab - B) / r.
+template <bool LAB, bool GEN = false>
+__device__ __forceinline__ void pair_term_f32(const PairK<float> &K, float ir, float dx, float dy, float dz,
+                                              const float4 &da, const float4 &dah, const float4 &db, float x,
+                                              float q0r, float q0i, float q1r, float q1i, float q2r, float q2i,
+                                              float &accr, float &acci)
+{
+  const float ir2 = ir * ir;
+  const float ba = da.x * dx + da.y * dy + da.z * dz;
+  const float bb = db.x * dx + db.y * dy + db.z * dz;
+  const float aabh = dah.x * db.x + dah.y * db.y + dah.z * db.z;
+  const float bq = ba * bb * ir2;
+  const float g3h = fmaT(-1.5f, bq, aabh) * ir2;
+  const float g1 = fmaT(2.0f, aabh, -bq) * ir;
+  if constexpr(LAB) {
+    const float g3r = fmaT(ir, q0r, -q1r);
+    const float g3i = fmaT(ir, q0i, -q1i);
+    accr = fmaT(g3h, g3r, fmaT(g1, q2r, accr));
+    acci = fmaT(g3h, g3i, fmaT(g1, q2i, acci));
+  } else {
+    const float th = K.theta;
+    const float g3r = fmaT(ir, q0r, fmaT(-th, q0i, -q1r));
+    const float g3i = fmaT(ir, q0i, fmaT(th, q0r, -q1i));
+    const float g1r = fmaT(-K.oo, q0r, fmaT(th, q1i, q2r));
+    const float g1i = fmaT(-K.oo, q0i, fmaT(-th, q1r, q2i));
+    const float cr = g3h * g3r + g1 * g1r;
+    const float ci = g3h * g3i + g1 * g1i;
+    float sn, cs;
+    if constexpr(GEN)
+      __sincosf(K.theta * x, &sn, &cs);
+    else
       sincos_theta(K.sn, K.cs, x, sn, cs);
-    } else {
-      sincos_small(K.theta * x, sn, cs);
-    }
     accr = fmaT(cr, cs, fmaT(ci, sn, accr));
     acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
   }
@@ -424,9 +470,10 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
       for(int i = 0; i <= M; ++i) cv[i] = load_off(c + i, off);
       double q0r, q0i, q1r, q1i, q2r, q2i;
       horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-      pair_term<double, LAB, GEN>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
+      pair_term_f64<LAB, GEN>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
     }
-    if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
+    if(tgt < wi.y)
+      partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.two_idt2, acci * k.two_idt2);
   }
   items.finish();
 }
@@ -466,6 +513,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
     const float4 pa = pd[2 * a], da = pd[2 * a + 1];
+    const float4 dah = make_float4(0.5f * da.x, 0.5f * da.y, 0.5f * da.z, 0.0f);
     const double4 pa64 = pos64[a];
     double accr = 0, acci = 0;
     for(int bt = wi.z; bt < wi.w; bt += kChunk) {
@@ -491,7 +539,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
         if constexpr(2 * kPairs == M) cv[M] = load_off(planes.last, off8);
         float q0r, q0i, q1r, q1i, q2r, q2i;
         horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-        pair_term<float, LAB, GEN>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, sr, si);
+        pair_term_f32<LAB, GEN>(K, ir, dx, dy, dz, da, dah, db, x, q0r, q0i, q1r, q1i, q2r, q2i, sr, si);
       }
       accr += static_cast<double>(sr);
       acci += static_cast<double>(si);

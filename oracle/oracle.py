@@ -55,6 +55,8 @@ def lib():
         L.qo_build_table.argtypes = [C.POINTER(QoParams), C.c_int, C.c_int, dp, dp, ip, dp]
         L.qo_evaluate_history.argtypes = [C.POINTER(QoParams), C.c_int, C.c_int, C.c_int,
                                           ip, dp, dp, C.c_int, dp]
+        L.qo_evaluate_observers.argtypes = [C.POINTER(QoParams), C.c_int, C.c_int, dp, dp, C.c_int,
+                                            dp, C.c_int, C.c_int, ip, dp]
         L.qo_pulse_evaluate.argtypes = [C.POINTER(QoPulse), C.c_double, C.c_int, dp, dp,
                                         C.c_double, dp]
         L.qo_liouville_rhs.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double,
@@ -155,6 +157,21 @@ def evaluate_history(p: QoParams, delays, coef, y, step: int, kind: int = 0):
                               d.ctypes.data_as(C.POINTER(C.c_int32)),
                               _dp(cf.view(np.float64)), _dp(y.view(np.float64)), step,
                               _dp(out.view(np.float64)))
+    return out
+
+
+def evaluate_observers(p: QoParams, pos, dip, rho01, step: int, observers, kind: int = 0):
+    """evaluate_history at the given observers without a table.
+    rho01: complex [n_t][n] samples at t = 0 .. n_t - 1 (earlier: 0)."""
+    pos, dip = _f64(pos), _f64(dip)
+    r = np.ascontiguousarray(rho01, dtype=np.complex128)
+    n_t, n = r.shape
+    assert n == pos.shape[0] and 0 <= step < n_t
+    obs = np.ascontiguousarray(observers, dtype=np.int32)
+    out = np.zeros(obs.shape[0], dtype=np.complex128)
+    lib().qo_evaluate_observers(C.byref(p), kind, n, _dp(pos), _dp(dip), n_t,
+                                _dp(r.view(np.float64)), step, obs.shape[0],
+                                obs.ctypes.data_as(C.POINTER(C.c_int32)), _dp(out.view(np.float64)))
     return out
 
 

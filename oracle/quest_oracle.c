@@ -17,6 +17,7 @@
  *   qo_lab_coefs          analytic cases
  *   qo_evaluate_history   N = 2 closed form, linearity, zero history => 0,
  *                         O(N^2) brute force on tiny N (numpy)
+ *   qo_evaluate_observers qo_evaluate_history on every observer (both frames)
  *   qo_pulse_evaluate     closed-form Gaussian
  *   qo_liouville_rhs      the reference's known-answer test
  *                         (test/quantum_dot_test.cpp:13-43), restated
@@ -263,6 +264,57 @@ void qo_evaluate_history(const qo_params *p, int kind, int n, int n_slots,
   }
   for(int i = 0; i < n; ++i) stc(out + 2 * i, res[i]);
   free(res);
+}
+
+/* The same evaluation for a subset of observers, each pair's delay and
+ * coefficients formed on the fly exactly as qo_build_table forms them
+ * (history_interaction.cpp:21-45, then :47-70 for one observer): no table, so
+ * that a few thousand observers of a system of 1e5 dots fit in memory.  The
+ * delay split and the coefficients are symmetric in the pair, so each
+ * (observer, source) term is the table's term for that unordered pair; only
+ * the order of the additions differs from qo_evaluate_history.
+ * rho01: [n_t][n] complex samples at t = 0 .. n_t - 1 (the prehistory t < 0
+ * is 0, main.cpp:32-36); needs step < n_t.  out: [n_obs] complex. */
+void qo_evaluate_observers(const qo_params *p, int kind, int n, const double *pos,
+                           const double *dip, int n_t, const double *rho01, int step,
+                           int n_obs, const int32_t *obs, double *out)
+{
+  const int M = p->order;
+  if(step < 0 || step >= n_t) return;
+#pragma omp parallel
+  {
+    double *w = (double *)malloc(sizeof(double) * 3 * (M + 1));
+    double *cf = (double *)malloc(sizeof(double) * 2 * (M + 1));
+#pragma omp for schedule(dynamic, 4)
+    for(int j = 0; j < n_obs; ++j) {
+      const int o = obs[j];
+      const double *po = pos + 3 * o;
+      cplx acc = 0;
+      for(int src = 0; src < n; ++src) {
+        if(src == o) continue;
+        const double *ps = pos + 3 * src;
+        const double dr[3] = {po[0] - ps[0], po[1] - ps[1], po[2] - ps[2]};
+        double frac;
+        const int delay = qo_split_delay(ps, po, p->c0, p->dt, &frac);
+        qo_lagrange_weights(frac, M, p->dt, w);
+        if(kind == 0)
+          qo_rotating_coefs(p, dr, w, dip + 3 * o, dip + 3 * src, cf);
+        else
+          qo_lab_coefs(p, dr, w, dip + 3 * o, dip + 3 * src, cf);
+        const int s = step - delay;
+        for(int i = 0; i <= M; ++i) {
+          const int t = s - i;
+          if(t < 0) continue;
+          const double *r = rho01 + 2 * ((int64_t)t * n + src);
+          const cplx pf = kind == 0 ? CMPLX(r[0], r[1]) : CMPLX(2 * r[0], 0.0);
+          acc += pf * ldc(cf + 2 * i);
+        }
+      }
+      stc(out + 2 * j, acc);
+    }
+    free(w);
+    free(cf);
+  }
 }
 
 /* ------------------------------------------------------------------------ */

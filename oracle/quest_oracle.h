@@ -75,6 +75,12 @@ void qo_evaluate_history(const qo_params *p, int kind, int n, int n_slots,
                          const int32_t *delays, const double *coef,
                          const double *y, int step, double *out);
 
+/* qo_evaluate_history for the observers obs[n_obs] only, coefficients formed
+ * on the fly (no table).  rho01: [n_t][n] complex, t = 0 .. n_t - 1. */
+void qo_evaluate_observers(const qo_params *p, int kind, int n, const double *pos,
+                           const double *dip, int n_t, const double *rho01, int step,
+                           int n_obs, const int32_t *obs, double *out);
+
 /* pulse.cpp:15-20 + pulse_interaction.cpp:9-20 + math_utils.cpp:27-30. */
 void qo_pulse_evaluate(const qo_pulse *pulse, double hbar, int n,
                        const double *pos, const double *dip, double t,

@@ -69,3 +69,28 @@ def rel_err(a, b):
     """max |a - b| / max |b| (norm-wise relative error)."""
     a, b = np.asarray(a), np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# The full-size (N = 131072) single-evaluation fixture: a smooth synthetic
+# rho01 history of every dot, evaluated at OBS_STEP for OBS_COUNT observers.
+OBS_STEP = 380        # beyond the largest delay (~370 steps) plus the stencil
+OBS_COUNT = 4096
+
+
+def smooth_history(n, n_t, seed=7):
+    """rho01[t][b], t = 0 .. n_t - 1: a_b (1 - exp(-t / tau_b)) exp(i (phi_b +
+    delta_b t dt)) with a_b in [0.05, 0.5], tau_b in [20, 200] steps, detuning
+    delta_b ~ N(0, 2 rad/ps), dt = 1e-4 ps (a rotating-frame envelope)."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.05, 0.5, n)
+    tau = rng.uniform(20.0, 200.0, n)
+    phi = rng.uniform(0.0, 2 * math.pi, n)
+    delta = rng.normal(0.0, 2.0, n) * 1e-4
+    t = np.arange(n_t, dtype=np.float64)[:, None]
+    return (a * -np.expm1(-t / tau)) * np.exp(1j * (phi + delta * t))
+
+
+def observers(n, count=OBS_COUNT, seed=8):
+    """count distinct dots of n, sorted (seeded)."""
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(n, size=count, replace=False)).astype(np.int32)

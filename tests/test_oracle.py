@@ -240,6 +240,33 @@ def test_evaluate_linear_and_zero(oracle_lib):
     assert np.all(oracle_lib.evaluate_history(p, delays, coef, np.zeros_like(y1), 24) == 0)
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+def test_evaluate_observers_matches_table_evaluate(oracle_lib, kind):
+    """The table-free evaluation at a subset of observers equals the table
+    evaluation there (same terms, another order of additions), the brute
+    force too, and includes the zero-delay pairs with the current sample."""
+    p = _params(oracle_lib)
+    n, steps = 90, 40
+    sysd = dense_system(n, 0.5, seed=21)
+    rng = np.random.default_rng(4)
+    r = rng.normal(size=(n, steps)) + 1j * rng.normal(size=(n, steps))
+    y = np.zeros((n, 22 + steps, 2), dtype=complex)
+    y[:, 22:, 1] = r
+    delays, coef = oracle_lib.build_table(p, sysd["pos"], sysd["dip"], kind=kind)
+    assert (delays == 0).sum() >= 1
+    step = steps - 1
+    want = oracle_lib.evaluate_history(p, delays, coef, y, step, kind=kind)
+    obs = np.array([0, 7, 33, 34, 89, 5], dtype=np.int32)
+    got = oracle_lib.evaluate_observers(p, sysd["pos"], sysd["dip"], r.T, step, obs, kind=kind)
+    assert rel_err(got, want[obs]) < 1e-13
+    all_obs = oracle_lib.evaluate_observers(p, sysd["pos"], sysd["dip"], r.T, step, np.arange(n), kind=kind)
+    assert rel_err(all_obs, want) < 1e-13
+    if kind == 0:
+        brute = _brute_force_field(oracle_lib, p, sysd["pos"], sysd["dip"],
+                                   lambda b, t: r[b, t] if t >= 0 else 0.0, step)
+        assert rel_err(all_obs, brute) < 1e-13
+
+
 # ------------------------------------------------ pulse.cpp, quantum_dot.cpp
 
 def test_pulse_closed_form(oracle_lib):

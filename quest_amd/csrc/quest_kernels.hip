@@ -701,6 +701,38 @@ __device__ __forceinline__ double2 near_sum(const Consts &k, const DeviceState &
   return acc;
 }
 
+// The same sum split for the one-launch step, where every near pair has delay
+// 0 (the list holds the pairs with n < 1): the terms m >= 1 read finished
+// steps and are formed once per step (near_fixed); only the m = 0 term reads
+// the current iterate of the partner (near_current, every iteration).
+__device__ __forceinline__ double2 near_fixed(const Consts &k, const DeviceState &d, int a, int step)
+{
+  double2 acc = make_double2(0.0, 0.0);
+  const int e0 = d.near_start[a], e1 = d.near_start[a + 1];
+  for(int e = e0; e < e1; ++e) {
+    const int b = d.near_src[e];
+    const double2 *cf = d.near_coef + static_cast<size_t>(e) * (k.order + 1);
+    for(int m = 1; m <= k.order; ++m) {
+      double2 rho = d.y1[static_cast<size_t>((step - m) & k.state_mask) * k.n + b];
+      if(k.dyadic == QUEST_DYADIC_LAB) rho = make_double2(2.0 * rho.x, 0.0);
+      acc = cadd(acc, cmul(rho, cf[m]));
+    }
+  }
+  return acc;
+}
+__device__ __forceinline__ double2 near_current(const Consts &k, const DeviceState &d, const double2 *cur1,
+                                                int a)
+{
+  double2 acc = make_double2(0.0, 0.0);
+  const int e0 = d.near_start[a], e1 = d.near_start[a + 1];
+  for(int e = e0; e < e1; ++e) {
+    double2 rho = cur1[d.near_src[e]];
+    if(k.dyadic == QUEST_DYADIC_LAB) rho = make_double2(2.0 * rho.x, 0.0);
+    acc = cadd(acc, cmul(rho, d.near_coef[static_cast<size_t>(e) * (k.order + 1)]));
+  }
+  return acc;
+}
+
 // Writes the interpolant coefficients of slot s for source a:
 // c_k = e^{i theta s} sum_m vinv[k][m] rho01(s - m)  (rotating), or of
 // 2 Re rho01 without the phase (lab), and zeros into slot s + 1.  The pair
@@ -905,7 +937,10 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
     t1 = d.t1[a];
     t2 = d.t2[a];
     partners = sync && d.near_start[a + 1] > d.near_start[a];
-    if(partners) d.cur1[a] = y1;
+    if(partners) {
+      d.cur1[a] = y1;
+      field = cadd(field, near_fixed(k, d, a, step));
+    }
   }
   if(sync) __syncthreads();
   const double f = future_coef * k.dt;
@@ -913,7 +948,7 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
   for(int iter = 0; iter <= kNumCorrectorSteps; ++iter) {
     if(a >= 0) {
       double2 om = field;
-      if(partners) om = cadd(om, near_sum(k, d, d.cur1 + static_cast<size_t>(iter & 1) * k.n, a, step));
+      if(partners) om = cadd(om, near_current(k, d, d.cur1 + static_cast<size_t>(iter & 1) * k.n, a));
       bloch_rhs(y0, y1, om, k.laser_omega, freq, t1, t2, m0, m1);
       if(iter < kNumCorrectorSteps) {
         y0 = make_double2(f * m0.x + c0.x, f * m0.y + c0.y);

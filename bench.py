@@ -123,7 +123,7 @@ def ensemble_main(args):
         "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK, "traffic": None,
-                     "kernel": "pair_past_f32<4,false,{}>".format("true" if mixed["info"]["dynamic_items"] else "false"),
+                     "kernel": "pair_past_f32<4,false,{},false>".format("true" if mixed["info"]["dynamic_items"] else "false"),
                      "kernel_ms": mixed["pair_ms"], "flop_per_pair": F_ALG},
     }
     if fp64 is not None:
@@ -330,8 +330,8 @@ def main(argv=None) -> int:
         "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK,
-                     "traffic": profiled_traffic("pair_past_f32<4, false, false>") if args.workload == "n65536" else None,
-                     "kernel": "pair_past_f32<4,false,false>", "kernel_ms": mixed["pair_ms"],
+                     "traffic": profiled_traffic("pair_past_f32<4, false, false, false>") if args.workload == "n65536" else None,
+                     "kernel": "pair_past_f32<4,false,false,false>", "kernel_ms": mixed["pair_ms"],
                      "flop_per_pair": F_ALG},
     }
     if fp64 is not None:

@@ -345,6 +345,12 @@ def main(argv=None) -> int:
         out["max_abs_drho_vs_fp64"] = float(np.abs(mixed["rho"] - fp64["rho"]).max())
         out["max_abs_rho01"] = float(np.abs(fp64["rho"][:, 1]).max())
         out["rel_rho01_err_vs_fp64"] = rho01_rel_err(mixed["rho"], fp64["rho"])
+        if n > 4096:
+            # the same figures over a seeded 4096-dot subsample of the full-system runs
+            sub = np.sort(np.random.default_rng(8).choice(n, size=4096, replace=False))
+            out["subsample_4096"] = {
+                "seed": 8, "max_abs_drho_vs_fp64": float(np.abs(mixed["rho"][sub] - fp64["rho"][sub]).max()),
+                "rel_rho01_err_vs_fp64": rho01_rel_err(mixed["rho"][sub], fp64["rho"][sub])}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_best"] = cpu_best()

@@ -64,3 +64,25 @@ def test_solvers_agree_within_conditioning():
     dp = np.abs(w["oracle"][0] - w["product"][0]).max()
     dc = np.abs(w["oracle"][1] - w["product"][1]).max()
     assert dp < 5e-2 and dc < 5e-3
+
+
+def test_trajectory_insensitive_to_the_valid_weight_sets(oracle_lib):
+    """The two backward-stable weight sets (numpy gelsd, the product's Jacobi
+    SVD) drive the FP64 oracle to the same trajectory far below the parity
+    tolerances (1e-10 relative): the weights themselves stay parity unpinned
+    against Eigen's JacobiSVD, but any valid set gives the same physics.
+    A driven, coupled 48-dot system over 600 steps (the pulse passes)."""
+    from conftest import dense_system, oracle_params, oracle_pulse, rel_err, short_pulse
+    from quest_amd import systems
+    w = _both()
+    sysd = dense_system(48, 0.24, seed=31)
+    pl = oracle_pulse(oracle_lib, short_pulse())
+    p = oracle_params(oracle_lib, systems.consts())
+    runs = {k: oracle_lib.solve(p, sysd, pl, *w[k], 600, mode=1) for k in w}
+    assert all(r["status"] == 0 for r in runs.values())
+    y_o, y_p = runs["oracle"]["y"], runs["product"]["y"]
+    assert np.abs(y_o[:, 1]).max() > 0.1
+    e = rel_err(y_p, y_o)
+    print(f"weights differ by {np.abs(w['oracle'][0] - w['product'][0]).max():.1e} (predictor); "
+          f"trajectories by {e:.1e}")
+    assert e < 1e-11

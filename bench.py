@@ -135,6 +135,8 @@ def ensemble_main(args):
         out["max_abs_drho_vs_fp64"] = float(np.abs(mixed["rho"] - fp64["rho"]).max())
         out["max_abs_rho01"] = float(np.abs(fp64["rho"][:, 1]).max())
         out["rel_rho01_err_vs_fp64"] = rho01_rel_err(mixed["rho"], fp64["rho"])
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_ensemble(ens)
     print(json.dumps(out))
     return 0
 
@@ -246,6 +248,37 @@ def cpu_baseline(steps=5):
                        f"N={n} system of the same generator, {steps} steps in "
                        f"{r['step_seconds']:.2f} s on 1 core (table build {r['setup_seconds']:.1f} s "
                        f"excluded); N=65536 needs a 180 GB table")}
+
+
+def cpu_baseline_ensemble(ens, steps=8, every=32):
+    """The reference method on one host core, timed per system and summed
+    over a seeded sample of the ensemble (every 32nd system, the ensemble's
+    own seeds and pulses): sum of N^2 x steps over the summed step seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import pc_weights
+    from quest_amd import systems
+    oracle.build()
+    oracle.set_threads(1)
+    c = systems.consts()
+    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
+    ps, cs, fc = pc_weights.weights()
+    pairs = secs = setup = 0.0
+    sample = ens[::every]
+    for sysd, pl in sample:
+        r = oracle.time_reference_steps(
+            p, sysd, oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                                  pl["wavevector"], pl["polarization"]), ps, cs, fc, steps)
+        n = sysd["pos"].shape[0]
+        pairs += float(n * n * steps)
+        secs += r["step_seconds"]
+        setup += r["setup_seconds"]
+    sizes = [s["pos"].shape[0] for s, _ in sample]
+    return {"value": pairs / secs, "unit": "pair-interactions/s", "cores": 1, "kind": "port",
+            "sample": (f"FP64 oracle, reference method (pair table + 11 scatter sums/step), "
+                       f"{len(sample)} of the 256 systems (every {every}nd, N {min(sizes)}-{max(sizes)}), "
+                       f"{steps} steps each, {secs:.2f} s of steps on 1 core (table builds {setup:.1f} s "
+                       f"excluded), timed per system and summed")}
 
 
 def cpu_best(steps=20):

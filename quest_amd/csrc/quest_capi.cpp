@@ -649,7 +649,6 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
     const double t = k.theta, t2 = t * t, U = 0.5 * std::fabs(t), U2 = U * U;
     k.sin_c[0] = t * (1.0 - 5.0 * U2 * U2 / 1920.0);
     k.sin_c[1] = t * t2 * (-1.0 / 6.0 + U2 / 96.0);
-    k.sin_c[2] = 0.0;
     k.cos_c[0] = -t2 / 2.0;
     k.cos_c[1] = t2 * t2 / 24.0;
     // the FP64 series' coefficients in x: sin: (-1)^k t^(2k+1)/(2k+1)!, k = 5..0;

@@ -42,9 +42,9 @@ struct Consts {
   // there w^2 becomes oo_l = oo / L / f32_scale = theta^2 / 2
   double oo_l, f32_scale;
   // FP32 sin/cos of theta x, |x| <= 1/2 (sincos_theta): sin_c[0..1] the
-  // Chebyshev-economized u - u^3/6 + u^5/120 over |u| <= theta/2 (sin_c[2] = 0),
+  // Chebyshev-economized u - u^3/6 + u^5/120 over |u| <= theta/2 (two terms),
   // cos_c = -theta^2/2, theta^4/24
-  double sin_c[3], cos_c[2];
+  double sin_c[2], cos_c[2];
   // FP64 sin/cos of theta x, x in [0, 1), theta <= 0.33 (sincos_small): Taylor
   // series in x^2 with theta folded in, highest first: trig64[0..5] =
   // (-1)^k theta^(2k+1) / (2k+1)!, k = 5..0; trig64[6..11] = (-1)^k theta^(2k) / (2k)!,

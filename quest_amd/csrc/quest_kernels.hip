@@ -1029,32 +1029,50 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
   int cnt = 0;
   int base = FILL ? d.near_start[a] : 0;
   const int2 range = d.sys_range[d.sys_of[a]];
-  if constexpr(F64) {
-    const double4 pa = d.pos64[a];
-    for(int b = range.x; b < range.y; ++b) {
+  // FP32 prefilter: a pair whose FP32 distance (units of c0 dt, error at most
+  // eps_abs + eps_rel r) is at least 1 + 2 (eps_abs + eps_rel) + 1e-3 has delay
+  // >= 1 for certain and skips the exact split; the lists are unchanged.
+  const float4 pa32 = d.pd32[2 * a];
+  const double cut = 1.0 + 2.0 * (k.eps_abs + k.eps_rel) + 1e-3;
+  const float cut2 = static_cast<float>(cut * cut);
+  const double4 pa64 = d.pos64[a];
+  const PairK<float> K(k);
+  // the exact split of the precision's pair kernels for a candidate b
+  auto consider = [&](int b) {
+    int n;
+    if constexpr(F64) {
       const double4 pb = d.pos64[b];
       double x, ir, r2;
-      const int n = split_f64(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, k.cdt, k.icdt, x, ir, r2);
-      if(b != a && n < 1) {
-        if(FILL) d.near_src[base + cnt] = b;
-        ++cnt;
-      }
-    }
-  } else {
-    const float4 pa = d.pd32[2 * a];
-    const double4 pa64 = d.pos64[a];
-    const PairK<float> K(k);
-    for(int b = range.x; b < range.y; ++b) {
+      n = split_f64(pa64.x - pb.x, pa64.y - pb.y, pa64.z - pb.z, k.cdt, k.icdt, x, ir, r2);
+    } else {
       const float4 pb = d.pd32[2 * b];
       float x, ir;
-      const int n = kRoundMagicBits - split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.eps_rel, K.lim0,
-                                                  kRoundMagic, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
-      if(b != a && n < 1) {
-        if(FILL) d.near_src[base + cnt] = b;
-        ++cnt;
-      }
+      n = kRoundMagicBits - split_mixed(pa32.x - pb.x, pa32.y - pb.y, pa32.z - pb.z, K.eps_rel, K.lim0,
+                                        kRoundMagic, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
     }
+    if(b != a && n < 1) {
+      if(FILL) d.near_src[base + cnt] = b;
+      ++cnt;
+    }
+  };
+  auto far = [&](const float4 &pb) {
+    const float ex = pa32.x - pb.x, ey = pa32.y - pb.y, ez = pa32.z - pb.z;
+    return ex * ex + ey * ey + ez * ez >= cut2;
+  };
+  // eight source positions in flight per step (the loop is bound by their
+  // load latency, one dot per thread), tested in source order
+  constexpr int kAhead = 8;
+  int b = range.x;
+  for(; b + kAhead <= range.y; b += kAhead) {
+    float4 q[kAhead];
+#pragma unroll
+    for(int j = 0; j < kAhead; ++j) q[j] = d.pd32[2 * (b + j)];
+#pragma unroll
+    for(int j = 0; j < kAhead; ++j)
+      if(!far(q[j])) consider(b + j);
   }
+  for(; b < range.y; ++b)
+    if(!far(d.pd32[2 * b])) consider(b);
   if(!FILL) d.counts[a] = cnt;
   else
     for(int e = 0; e < cnt; ++e) d.near_tgt[base + e] = a;

@@ -732,13 +732,13 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
     if(e != hipSuccess)
       return fail(c, QUEST_ERR_NOMEM, "ring allocation of " + std::to_string(c->ring_bytes) + " bytes failed");
   }
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.y0), sizeof(double2) * ss));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.y0), sizeof(double) * ss));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.y1), sizeof(double2) * ss));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.d0), sizeof(double2) * ss));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.d0), sizeof(double) * ss));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.d1), sizeof(double2) * ss));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.cur0), sizeof(double2) * 2 * nn));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.cur0), sizeof(double) * 2 * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.cur1), sizeof(double2) * 2 * nn));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.yc0), sizeof(double2) * nn));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.yc0), sizeof(double) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.yc1), sizeof(double2) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.field), sizeof(double2) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.out), sizeof(double2) * nn));
@@ -1242,23 +1242,25 @@ int quest_get_state(quest_ctx *c, double *rho, double *drho)
   HIPC(c, hipSetDevice(c->P.device));
   const int n = c->n;
   const size_t off = static_cast<size_t>((c->cur_step - 1) & c->k.state_mask) * n;
-  std::vector<double2> h(4 * static_cast<size_t>(n));
-  double2 *srcs[4] = {c->d.y0, c->d.y1, c->d.d0, c->d.d1};
-  for(int q = 0; q < 4; ++q)
-    HIPC(c, hipMemcpyAsync(h.data() + static_cast<size_t>(q) * n, srcs[q] + off, sizeof(double2) * n,
-                           hipMemcpyDeviceToHost, c->stream));
+  // rho00 and its derivative are real on the device (imaginary part 0)
+  std::vector<double> h0(2 * static_cast<size_t>(n));
+  std::vector<double2> h1(2 * static_cast<size_t>(n));
+  HIPC(c, hipMemcpyAsync(h0.data(), c->d.y0 + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(h0.data() + n, c->d.d0 + off, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(h1.data(), c->d.y1 + off, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(h1.data() + n, c->d.d1 + off, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   for(int s = 0; s < n; ++s) {
     const int i = c->perm[s];
-    rho[4 * i + 0] = h[s].x;
-    rho[4 * i + 1] = h[s].y;
-    rho[4 * i + 2] = h[n + s].x;
-    rho[4 * i + 3] = h[n + s].y;
+    rho[4 * i + 0] = h0[s];
+    rho[4 * i + 1] = 0.0;
+    rho[4 * i + 2] = h1[s].x;
+    rho[4 * i + 3] = h1[s].y;
     if(drho) {
-      drho[4 * i + 0] = h[2 * n + s].x;
-      drho[4 * i + 1] = h[2 * n + s].y;
-      drho[4 * i + 2] = h[3 * n + s].x;
-      drho[4 * i + 3] = h[3 * n + s].y;
+      drho[4 * i + 0] = h0[n + s];
+      drho[4 * i + 1] = 0.0;
+      drho[4 * i + 2] = h1[n + s].x;
+      drho[4 * i + 3] = h1[n + s].y;
     }
   }
   return QUEST_OK;

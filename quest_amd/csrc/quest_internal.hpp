@@ -84,10 +84,18 @@ struct DeviceState {
   float4 *pd32 = nullptr;              // [n][2]: position (pos - centre) / (c0 dt), dipole (FP32)
   double *freq = nullptr, *t1 = nullptr, *t2 = nullptr;
   void *ring = nullptr;                // coefficient planes, kstride x [n][ring_slots] complex
-  double2 *y0 = nullptr, *y1 = nullptr;  // state ring [state_slots][n]: rho00, rho01
-  double2 *d0 = nullptr, *d1 = nullptr;  // derivative ring
-  double2 *cur0 = nullptr, *cur1 = nullptr;  // iterate double buffer [2][n]
-  double2 *yc0 = nullptr, *yc1 = nullptr;    // corrector base [n]
+  // rho00 is held real: from the real ground state its imaginary part stays an
+  // exact +-0 (the RHS's imaginary part is -Re(om conj(r1) - conj(om) r1) -
+  // Im(r00) / T1, whose first term is the difference of two identically
+  // rounded products), so its ring, iterate and corrector base are doubles
+  double *y0 = nullptr;                // state ring [state_slots][n]: rho00
+  double2 *y1 = nullptr;               // state ring [state_slots][n]: rho01
+  double *d0 = nullptr;                // derivative ring of rho00
+  double2 *d1 = nullptr;               // derivative ring of rho01
+  double *cur0 = nullptr;              // iterate double buffer [2][n] (rho00)
+  double2 *cur1 = nullptr;             // iterate double buffer [2][n] (rho01)
+  double *yc0 = nullptr;               // corrector base [n] (rho00)
+  double2 *yc1 = nullptr;              // corrector base [n] (rho01)
   double2 *field = nullptr;            // fixed part of the field at this step [n]
   double2 *partial = nullptr;          // [max_chunks][n]
   double2 *phase = nullptr;            // e^{i theta s}, s in [0, phase_len)

@@ -651,21 +651,19 @@ __device__ __forceinline__ double2 pulse_sum(const Consts &k, const DeviceState 
 }
 
 // QuantumDot::liouville_rhs (quantum_dot.cpp:10-19).
-__device__ __forceinline__ void bloch_rhs(double2 r0, double2 r1, double2 om, double laser,
-                                          double freq, double t1, double t2, double2 &m0,
+__device__ __forceinline__ void bloch_rhs(double r0, double2 r1, double2 om, double laser,
+                                          double freq, double t1, double t2, double &m0,
                                           double2 &m1)
 {
-  // z = om conj(r1) - conj(om) r1
-  const double2 a = make_double2(om.x * r1.x + om.y * r1.y, om.y * r1.x - om.x * r1.y);
-  const double2 b = make_double2(om.x * r1.x + om.y * r1.y, om.x * r1.y - om.y * r1.x);
-  const double2 z = make_double2(a.x - b.x, a.y - b.y);
-  // -i z - (r0 - 1) / t1
-  m0 = make_double2(z.y - (r0.x - 1.0) / t1, -z.x - r0.y / t1);
+  // z = om conj(r1) - conj(om) r1 = i 2 Im(om conj(r1)): its real part is the
+  // difference of two identically rounded products, an exact 0
+  const double zy = (om.y * r1.x - om.x * r1.y) - (om.x * r1.y - om.y * r1.x);
+  // -i z - (r0 - 1) / t1, real (its imaginary part -Re z - Im r0 / t1 is 0)
+  m0 = zy - (r0 - 1.0) / t1;
   // w = om (1 - 2 r0) + r1 (laser - freq);  -i w - r1 / t2
-  const double2 f = make_double2(1.0 - 2.0 * r0.x, -2.0 * r0.y);
+  const double f = 1.0 - 2.0 * r0;
   const double det = laser - freq;
-  const double2 w = make_double2(om.x * f.x - om.y * f.y + r1.x * det,
-                                 om.x * f.y + om.y * f.x + r1.y * det);
+  const double2 w = make_double2(om.x * f + r1.x * det, om.y * f + r1.y * det);
   m1 = make_double2(w.y - r1.x / t2, -w.x - r1.y / t2);
 }
 
@@ -787,15 +785,15 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
 }
 
 // Ground state in every state-ring slot (main.cpp:32-36), zero derivatives.
-__global__ void fill_ground_state_kernel(Consts k, double2 *y0, double2 *y1, double2 *d0,
+__global__ void fill_ground_state_kernel(Consts k, double *y0, double2 *y1, double *d0,
                                          double2 *d1, int *bad_step)
 {
   const size_t total = static_cast<size_t>(k.state_slots) * k.n;
   for(size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < total;
       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    y0[i] = make_double2(1.0, 0.0);
+    y0[i] = 1.0;
     y1[i] = make_double2(0.0, 0.0);
-    d0[i] = make_double2(0.0, 0.0);
+    d0[i] = 0.0;
     d1[i] = make_double2(0.0, 0.0);
   }
   if(blockIdx.x == 0 && threadIdx.x == 0) *bad_step = INT_MAX;
@@ -812,7 +810,7 @@ __global__ void clear_words_kernel(uint4 *p, size_t n16)
 // predictor (integrator.cpp:142-154) and the step-invariant part of the
 // corrector (integrator.cpp:156-169).
 __device__ __forceinline__ void pc_predict(const Consts &k, const DeviceState &d, int window, int a, int step,
-                                           double2 &field, double2 &p0, double2 &p1, double2 &c0, double2 &c1)
+                                           double2 &field, double &p0, double2 &p1, double &c0, double2 &c1)
 {
   const double dt = k.dt;
   double2 hist = make_double2(0.0, 0.0);
@@ -822,22 +820,21 @@ __device__ __forceinline__ void pc_predict(const Consts &k, const DeviceState &d
 
   const double *ps0 = d.pcw, *ps1 = d.pcw + window;
   const double *cs0 = d.pcw + 2 * window, *cs1 = d.pcw + 3 * window;
-  p0 = make_double2(0.0, 0.0);
-  p1 = p0;
-  c0 = p0;
-  c1 = p0;
+  p0 = 0.0;
+  c0 = 0.0;
+  p1 = make_double2(0.0, 0.0);
+  c1 = p1;
   const int start = step - window;
 #pragma unroll 4
   for(int h = 0; h < window; ++h) {
     const size_t o = static_cast<size_t>((start + h) & k.state_mask) * k.n + a;
-    const double2 y0 = d.y0[o], y1 = d.y1[o], g0 = d.d0[o], g1 = d.d1[o];
+    const double y0 = d.y0[o], g0 = d.d0[o];
+    const double2 y1 = d.y1[o], g1 = d.d1[o];
     const double pw = ps0[h], pd = ps1[h] * dt, cw = cs0[h], cd = cs1[h] * dt;
-    p0.x += y0.x * pw + g0.x * pd;
-    p0.y += y0.y * pw + g0.y * pd;
+    p0 += y0 * pw + g0 * pd;
     p1.x += y1.x * pw + g1.x * pd;
     p1.y += y1.y * pw + g1.y * pd;
-    c0.x += y0.x * cw + g0.x * cd;
-    c0.y += y0.y * cw + g0.y * cd;
+    c0 += y0 * cw + g0 * cd;
     c1.x += y1.x * cw + g1.x * cd;
     c1.y += y1.y * cw + g1.y * cd;
   }
@@ -845,7 +842,8 @@ __device__ __forceinline__ void pc_predict(const Consts &k, const DeviceState &d
 
 __device__ __forceinline__ void pc_begin_dot(const Consts &k, const DeviceState &d, int window, int a)
 {
-  double2 field, p0, p1, c0, c1;
+  double2 field, p1, c1;
+  double p0, c0;
   pc_predict(k, d, window, a, *d.step, field, p0, p1, c0, c1);
   d.field[a] = field;
   d.cur0[a] = p0;
@@ -868,14 +866,14 @@ __global__ __launch_bounds__(kBlock) void pc_begin_kernel(Consts k, DeviceState 
 // check (integrator.cpp:187-207); the source's interpolant slot.
 template <typename R2>
 __device__ __forceinline__ void pc_commit(const Consts &k, const DeviceState &d, R2 *ring, int a, int step,
-                                          double2 y0, double2 y1, double2 m0, double2 m1)
+                                          double y0, double2 y1, double m0, double2 m1)
 {
   const size_t o = static_cast<size_t>(step & k.state_mask) * k.n + a;
   d.y0[o] = y0;
   d.y1[o] = y1;
   d.d0[o] = m0;
   d.d1[o] = m1;
-  if(!(finite2(y0) && finite2(y1) && finite2(m0) && finite2(m1))) atomicMin(d.bad_step, step);
+  if(!(isfinite(y0) && finite2(y1) && isfinite(m0) && finite2(m1))) atomicMin(d.bad_step, step);
   build_slot(k, d.phase, ring, d.samp, d.y1, y1, a, step);
 }
 
@@ -885,16 +883,18 @@ __device__ __forceinline__ void pc_iter_dot(const Consts &k, const DeviceState &
 {
   const int step = *d.step;
   const int rd = iter & 1, wr = rd ^ 1;
-  const double2 *cur0 = d.cur0 + static_cast<size_t>(rd) * k.n;
+  const double *cur0 = d.cur0 + static_cast<size_t>(rd) * k.n;
   const double2 *cur1 = d.cur1 + static_cast<size_t>(rd) * k.n;
-  const double2 y0 = cur0[a], y1 = cur1[a];
+  const double y0 = cur0[a];
+  const double2 y1 = cur1[a];
   double2 om = d.field[a];
   if(d.near_start[a + 1] > d.near_start[a]) om = cadd(om, near_sum(k, d, cur1, a, step));
-  double2 m0, m1;
+  double m0;
+  double2 m1;
   bloch_rhs(y0, y1, om, k.laser_omega, d.freq[a], d.t1[a], d.t2[a], m0, m1);
   if(iter < kNumCorrectorSteps) {
     const double f = future_coef * k.dt;
-    d.cur0[static_cast<size_t>(wr) * k.n + a] = make_double2(f * m0.x + d.yc0[a].x, f * m0.y + d.yc0[a].y);
+    d.cur0[static_cast<size_t>(wr) * k.n + a] = f * m0 + d.yc0[a];
     d.cur1[static_cast<size_t>(wr) * k.n + a] = make_double2(f * m1.x + d.yc1[a].x, f * m1.y + d.yc1[a].y);
     return;
   }
@@ -928,7 +928,8 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
   const int a = d.step_map[blockIdx.x * blockDim.x + threadIdx.x];
   const bool sync = static_cast<int>(blockIdx.x) < sync_blocks;
   const int step = *d.step;
-  double2 field, y0, y1, c0, c1;
+  double2 field, y1, c1;
+  double y0, c0;
   double freq = 0, t1 = 1, t2 = 1;
   bool partners = false;
   if(a >= 0) {
@@ -944,14 +945,15 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
   }
   if(sync) __syncthreads();
   const double f = future_coef * k.dt;
-  double2 m0, m1;
+  double m0;
+  double2 m1;
   for(int iter = 0; iter <= kNumCorrectorSteps; ++iter) {
     if(a >= 0) {
       double2 om = field;
       if(partners) om = cadd(om, near_current(k, d, d.cur1 + static_cast<size_t>(iter & 1) * k.n, a));
       bloch_rhs(y0, y1, om, k.laser_omega, freq, t1, t2, m0, m1);
       if(iter < kNumCorrectorSteps) {
-        y0 = make_double2(f * m0.x + c0.x, f * m0.y + c0.y);
+        y0 = f * m0 + c0;
         y1 = make_double2(f * m1.x + c1.x, f * m1.y + c1.y);
         if(partners) d.cur1[static_cast<size_t>((iter & 1) ^ 1) * k.n + a] = y1;
       }

@@ -481,7 +481,7 @@ void free_device(quest_ctx *c)
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.sys_of, d.sys_range, d.sys_chunks,
                   d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff,
-                  d.step_map};
+                  d.step_map, d.near_lane};
   for(void *p : ptrs)
     if(p) (void)hipFree(p);
   d = DeviceState{};
@@ -550,9 +550,24 @@ int pack_step_blocks(quest_ctx *c, const std::vector<int> &start, const std::vec
     if(size[r] == 1) map.push_back(r);
   if(map.size() % block) map.resize(map.size() + block - map.size() % block, -1);
   const size_t blocks = map.size() / block;
+  // each near pair's source as a thread of its target's block
+  std::vector<int> slot_of(n, -1);
+  for(size_t t = 0; t < map.size(); ++t)
+    if(map[t] >= 0) slot_of[map[t]] = static_cast<int>(t);
+  std::vector<uint8_t> lane(near_src.size());
+  for(int a = 0; a < n; ++a)
+    for(int e = start[a]; e < start[a + 1]; ++e) {
+      const int ta = slot_of[a], tb = slot_of[near_src[e]];
+      if(ta / block != tb / block) return fail(c, QUEST_ERR_HIP, "zero-delay partners split over two blocks");
+      lane[e] = static_cast<uint8_t>(tb % block);
+    }
   DeviceState &d = c->d;
   HIPC(c, hipMalloc(reinterpret_cast<void **>(&d.step_map), sizeof(int) * map.size()));
   HIPC(c, hipMemcpyAsync(d.step_map, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice, c->stream));
+  if(!lane.empty()) {
+    HIPC(c, hipMalloc(reinterpret_cast<void **>(&d.near_lane), lane.size()));
+    HIPC(c, hipMemcpyAsync(d.near_lane, lane.data(), lane.size(), hipMemcpyHostToDevice, c->stream));
+  }
   HIPC(c, hipStreamSynchronize(c->stream));
   c->step_block = block;
   c->step_blocks = static_cast<int>(blocks);

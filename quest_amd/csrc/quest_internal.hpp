@@ -127,6 +127,9 @@ struct DeviceState {
   // one-launch predictor-corrector step: thread -> dot (-1 idle), each group of
   // zero-delay partners within one block, all groups in the leading blocks
   int *step_map = nullptr;
+  // [near]: thread of the near pair's source within the block of the one-launch
+  // step (groups of partners lie whole in one block), for the LDS exchange
+  uint8_t *near_lane = nullptr;
 };
 
 // ---------------------------------------------------------------- launchers

@@ -718,13 +718,13 @@ __device__ __forceinline__ double2 near_fixed(const Consts &k, const DeviceState
   }
   return acc;
 }
-__device__ __forceinline__ double2 near_current(const Consts &k, const DeviceState &d, const double2 *cur1,
+__device__ __forceinline__ double2 near_current(const Consts &k, const DeviceState &d, const double2 *xchg,
                                                 int a)
 {
   double2 acc = make_double2(0.0, 0.0);
   const int e0 = d.near_start[a], e1 = d.near_start[a + 1];
   for(int e = e0; e < e1; ++e) {
-    double2 rho = cur1[d.near_src[e]];
+    double2 rho = xchg[d.near_lane[e]];  // the partner's iterate, from its thread's LDS entry
     if(k.dyadic == QUEST_DYADIC_LAB) rho = make_double2(2.0 * rho.x, 0.0);
     acc = cadd(acc, cmul(rho, d.near_coef[static_cast<size_t>(e) * (k.order + 1)]));
   }
@@ -916,8 +916,9 @@ __global__ __launch_bounds__(kBlock) void pc_iter_kernel(Consts k, DeviceState d
 // a zero-delay partner's current iterate: the host packs dots (step_map:
 // thread -> dot, -1 idle) so that every connected group of zero-delay
 // partners sits in one of the first `sync_blocks` blocks; there a dot with
-// partners publishes each iterate to the cur1 double buffer, and a block
-// barrier (a workgroup-scope fence on gfx950) separates the iterations.
+// partners publishes each iterate to an LDS double buffer (near_lane: each
+// near pair's source as a thread of the block), and a block barrier
+// separates the iterations.
 // Blocks of dots without partners run barrier-free.  The last block to finish
 // advances the step counter (what advance_step_kernel does after the
 // per-iteration kernels): every block has read it by then.
@@ -928,6 +929,8 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
   const int a = d.step_map[blockIdx.x * blockDim.x + threadIdx.x];
   const bool sync = static_cast<int>(blockIdx.x) < sync_blocks;
   const int step = *d.step;
+  // iterates of zero-delay partners, exchanged through LDS (double buffer)
+  __shared__ double2 xchg[2][kBlock];
   double2 field, y1, c1;
   double y0, c0;
   double freq = 0, t1 = 1, t2 = 1;
@@ -939,7 +942,7 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
     t2 = d.t2[a];
     partners = sync && d.near_start[a + 1] > d.near_start[a];
     if(partners) {
-      d.cur1[a] = y1;
+      xchg[0][threadIdx.x] = y1;
       field = cadd(field, near_fixed(k, d, a, step));
     }
   }
@@ -950,12 +953,12 @@ __global__ __launch_bounds__(kBlock) void pc_step_kernel(Consts k, DeviceState d
   for(int iter = 0; iter <= kNumCorrectorSteps; ++iter) {
     if(a >= 0) {
       double2 om = field;
-      if(partners) om = cadd(om, near_current(k, d, d.cur1 + static_cast<size_t>(iter & 1) * k.n, a));
+      if(partners) om = cadd(om, near_current(k, d, xchg[iter & 1], a));
       bloch_rhs(y0, y1, om, k.laser_omega, freq, t1, t2, m0, m1);
       if(iter < kNumCorrectorSteps) {
         y0 = f * m0 + c0;
         y1 = make_double2(f * m1.x + c1.x, f * m1.y + c1.y);
-        if(partners) d.cur1[static_cast<size_t>((iter & 1) ^ 1) * k.n + a] = y1;
+        if(partners) xchg[(iter & 1) ^ 1][threadIdx.x] = y1;
       }
     }
     if(sync && iter < kNumCorrectorSteps) __syncthreads();

@@ -207,7 +207,11 @@ int quest_bad_step(const quest_ctx *ctx, int *step);
 
 /* State (rho00, rho01) and derivative of every dot at the last finished step:
  * the slot history[i][step][0] and [1] (history.h:13-14).  rho, drho: [n][2]
- * complex; drho may be NULL. */
+ * complex; drho may be NULL.  rho00 and its derivative come back with
+ * imaginary part 0: from the real ground state the reference's imaginary
+ * part stays an exact +-0 (its RHS is -Re z - Im rho00 / T1, z = Omega
+ * conj(rho01) - conj(Omega) rho01, whose real part is the difference of two
+ * identically rounded products), so the device holds rho00 real. */
 int quest_get_state(quest_ctx *ctx, double *rho, double *drho);
 
 /* ------------------------------------------------------ diagnostics */

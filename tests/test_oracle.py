@@ -432,3 +432,21 @@ def test_solve_reports_nonfinite(oracle_lib, ref_weights):
     r = oracle_lib.solve(oracle_params(oracle_lib, consts), s, oracle_pulse(oracle_lib, short_pulse()),
                          ps, cs, fc, 50, mode=0)
     assert r["status"] == 1 and r["bad_step"] == 0
+
+
+def test_rho00_stays_real(oracle_lib, ref_weights):
+    """The reference arithmetic keeps rho00 exactly real from the real ground
+    state (its RHS's imaginary part is -Re z - Im rho00 / T1 with Re z the
+    difference of two identically rounded products), which is what lets the
+    device hold rho00 as a real number: every recorded step of a driven,
+    coupled run has Im rho00 == 0 and Im drho00 == 0."""
+    from quest_amd import systems
+    ps, cs, fc = ref_weights
+    sysd = dense_system(40, 0.24, seed=12)
+    p = oracle_params(oracle_lib, systems.consts())
+    r = oracle_lib.solve(p, sysd, oracle_pulse(oracle_lib, short_pulse()), ps, cs, fc, 400, mode=1,
+                         record_every=1)
+    assert r["status"] == 0
+    assert np.abs(r["y"][:, 1]).max() > 0.1
+    assert np.all(r["rec"][:, :, 0].imag == 0)
+    assert np.all(r["y"][:, 0].imag == 0) and np.all(r["yp"][:, 0].imag == 0)

@@ -484,6 +484,14 @@ __global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
 // compiler reports 8 waves/SIMD for both: fewer waves were resident than
 // that.  Capping at 64 costs a handful of SGPR spills to VGPR lanes outside
 // the inner loop and keeps all 8 waves/SIMD resident.
+// One source of the FP32 pair loop, staged ahead of its arithmetic: dipole,
+// separation, centred fraction x, 1/r and the interpolant coefficients.
+template <int M> struct StagedF32 {
+  float4 db;
+  float dx, dy, dz, x, ir;
+  float2 cv[M + 1];
+};
+
 // Base of each coefficient plane of the FP32 ring, passed as separate kernel
 // arguments so that each stays a scalar base of its loads (computed in the
 // kernel, ring + E would be folded into the per-lane 64-bit address).
@@ -519,27 +527,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
     for(int bt = wi.z; bt < wi.w; bt += kChunk) {
       const int be = min(wi.w, bt + kChunk);
       float sr = 0, si = 0;
-      for(int b = bt; b < be; ++b) {
-        const float4 pb = pd[2 * b], db = pd[2 * b + 1];  // one scalar load of 32 B
-        const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-        float x, ir;  // x: the fraction centred on 1/2 (the FP32 ring's variable)
-        const int code = split_mixed(dx, dy, dz, K.eps_rel, K.lim0, ms, pa64, pos64, b, K.cdt, K.icdt64, x, ir);
-        // byte offsets of the slot (step - n) & mask in the 16-B and 8-B
-        // planes; delay 0 (the self pair, zero-delay partners) reads the
-        // current step's slot, which holds zeros until the step commits
+      // Software-pipelined by one source: source b + 1's position and dipole
+      // (one scalar load of 32 B), delay split and coefficient gathers are
+      // issued before source b's arithmetic, so the gathers' cache latency
+      // overlaps it (the last step re-stages b, in bounds and unused).
+      // Byte offsets of the slot (step - n) & mask in the 16-B and 8-B planes;
+      // delay 0 (the self pair, zero-delay partners) reads the current step's
+      // slot, which holds zeros until the step commits.
+      auto stage = [&](int b, StagedF32<M> &st) {
+        const float4 pb = pd[2 * b];
+        st.db = pd[2 * b + 1];
+        st.dx = pa.x - pb.x;
+        st.dy = pa.y - pb.y;
+        st.dz = pa.z - pb.z;
+        const int code =
+            split_mixed(st.dx, st.dy, st.dz, K.eps_rel, K.lim0, ms, pa64, pos64, b, K.cdt, K.icdt64, st.x, st.ir);
         const unsigned off16 = ((static_cast<unsigned>(code) & mask) << 4) | (static_cast<unsigned>(b) * row_step16);
-        const unsigned off8 = off16 >> 1;
-        float2 cv[M + 1];
 #pragma unroll
         for(int i = 0; i < kPairs; ++i) {
           const float4 v = load_off(planes.pair[i], off16);
-          cv[2 * i] = make_float2(v.x, v.y);
-          cv[2 * i + 1] = make_float2(v.z, v.w);
+          st.cv[2 * i] = make_float2(v.x, v.y);
+          st.cv[2 * i + 1] = make_float2(v.z, v.w);
         }
-        if constexpr(2 * kPairs == M) cv[M] = load_off(planes.last, off8);
+        if constexpr(2 * kPairs == M) st.cv[M] = load_off(planes.last, off16 >> 1);
+      };
+      StagedF32<M> cur;
+      stage(bt, cur);
+      for(int b = bt; b < be; ++b) {
+        StagedF32<M> nxt;
+        stage(min(b + 1, be - 1), nxt);
         float q0r, q0i, q1r, q1i, q2r, q2i;
-        horner3<float, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-        pair_term_f32<LAB, GEN>(K, ir, dx, dy, dz, da, dah, db, x, q0r, q0i, q1r, q1i, q2r, q2i, sr, si);
+        horner3<float, M>(cur.cv, cur.x, q0r, q0i, q1r, q1i, q2r, q2i);
+        pair_term_f32<LAB, GEN>(K, cur.ir, cur.dx, cur.dy, cur.dz, da, dah, cur.db, cur.x, q0r, q0i, q1r, q1i,
+                                q2r, q2i, sr, si);
+        cur = nxt;
       }
       accr += static_cast<double>(sr);
       acci += static_cast<double>(si);

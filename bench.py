@@ -284,7 +284,7 @@ def cpu_baseline_ensemble(ens, steps=8, every=32):
 def cpu_best(steps=20):
     """The oracle's split gather (one pair sum per step plus the zero-delay
     pairs per evaluation, the arithmetic the GPU path uses) on up to 16 host
-    threads (the box's CPU share per GPU): the best CPU figure beside the
+    threads (the host cores available per GPU): the best CPU figure beside the
     serial reference method."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle

@@ -41,7 +41,7 @@ def run_cli(workdir, extra):
 
 
 def oracle_traj(oracle, sysd, consts, pl, steps, order, mode):
-    """The oracle with the weights the driver uses (PredictorCorrector::Weights
+    """The oracle with the weights quest_hip uses (PredictorCorrector::Weights
     of the library, integrator.cpp:85-102)."""
     from quest_amd import pc_weights
     ps, cs, fc = pc_weights(18, 22, 3.15)

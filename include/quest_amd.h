@@ -74,11 +74,24 @@ typedef struct {
                           XCD-aware list: a batch's systems each on one XCD)
                           or QUEST_CLAIM_DYNAMIC (a device counter; balances
                           items of uneven size, in list order)               */
+  int source_loop;     /* mixed precision's pair loop over an item's sources:
+                          QUEST_LOOP_AUTO (pipelined when the largest system
+                          has at most 8192 dots, else plain, until
+                          quest_autotune times both), QUEST_LOOP_PLAIN, or
+                          QUEST_LOOP_PIPELINED (the next source's delay split
+                          and interpolant gathers issued before this
+                          source's arithmetic: faster where many ring reads
+                          miss L2, e.g. many small systems; slower on large
+                          ones).  Same arithmetic, same results.  FP64
+                          contexts use the plain loop.                       */
 } quest_params;
 
 #define QUEST_CLAIM_AUTO 0
 #define QUEST_CLAIM_STATIC 1
 #define QUEST_CLAIM_DYNAMIC 2
+#define QUEST_LOOP_AUTO 0
+#define QUEST_LOOP_PLAIN 1
+#define QUEST_LOOP_PIPELINED 2
 
 /* Pulse (pulse.h:10-24): amplitude, delay, width (dimensionless), freq,
  * wavevector, polarization.  The polarization is used as given, as on the
@@ -231,18 +244,21 @@ int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
 
 /* Times the pair kernel (`launches` launches each, after a warm-up) for
  * source-chunk lengths around the default, each with static and dynamic item
- * claiming (only the one item_claim names, unless QUEST_CLAIM_AUTO), and
- * keeps the fastest; returns its chunk length in *chunk_len.  Changes no simulation state.  No counterpart in the
- * reference, which has no tunable kernel. */
+ * claiming (only the one item_claim names, unless QUEST_CLAIM_AUTO) and, in
+ * mixed precision, the plain and the pipelined source loop (only the one
+ * source_loop names, unless QUEST_LOOP_AUTO), and keeps the fastest; returns
+ * its chunk length in *chunk_len.  Changes no simulation state.  No
+ * counterpart in the reference, which has no tunable kernel. */
 int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);
 
-/* Integer facts about the context.  info: [15] int64 = n_dots, ring slots,
+/* Integer facts about the context.  info: [16] int64 = n_dots, ring slots,
  * delay bound, zero-delay pairs, work items, persistent grid, state slots,
  * precision, systems, source-chunk length, chunks of the largest system,
  * step graph captured (0/1), block size of the one-launch predictor-corrector
  * step (0: one launch per iteration, used when a connected group of
  * zero-delay partners exceeds 256 dots), size of the largest such group,
- * pair-kernel items claimed dynamically (1) or by a static stride (0). */
+ * pair-kernel items claimed dynamically (1) or by a static stride (0),
+ * pipelined (1) or plain (0) source loop of the mixed-precision pair kernel. */
 int quest_ctx_info(const quest_ctx *ctx, int64_t *info);
 
 /* Blocks until all work queued on the context's stream is done. */

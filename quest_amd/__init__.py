@@ -4,10 +4,11 @@ The product is libquest_amd.so (C ABI: include/quest_amd.h, HIP kernels in
 csrc/); this package is its Python front end (capi) and the seeded system
 generators (systems).
 """
-from .capi import (CLAIM_AUTO, CLAIM_DYNAMIC, CLAIM_STATIC, PAIR_ON_THE_FLY, PAIR_TABLE, PREC_FP64,
-                   PREC_MIXED, DYADIC_LAB, DYADIC_ROTATING, Context, QuestError, UnboundedSolution, load,
-                   pc_weights)
+from .capi import (CLAIM_AUTO, CLAIM_DYNAMIC, CLAIM_STATIC, LOOP_AUTO, LOOP_PIPELINED, LOOP_PLAIN,
+                   PAIR_ON_THE_FLY, PAIR_TABLE, PREC_FP64, PREC_MIXED, DYADIC_LAB, DYADIC_ROTATING, Context,
+                   QuestError, UnboundedSolution, load, pc_weights)
 
-__all__ = ["CLAIM_AUTO", "CLAIM_DYNAMIC", "CLAIM_STATIC", "PAIR_ON_THE_FLY", "PAIR_TABLE", "PREC_FP64",
+__all__ = ["CLAIM_AUTO", "CLAIM_DYNAMIC", "CLAIM_STATIC", "LOOP_AUTO", "LOOP_PIPELINED", "LOOP_PLAIN",
+           "PAIR_ON_THE_FLY", "PAIR_TABLE", "PREC_FP64",
            "PREC_MIXED", "DYADIC_LAB", "DYADIC_ROTATING", "Context", "QuestError", "UnboundedSolution",
            "load", "pc_weights"]

@@ -783,6 +783,14 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   int rc = upload_pulses(c);
   if(rc != QUEST_OK) return rc;
 
+  // FP32 source loop: pipelined for systems of at most 8192 dots, whose ring
+  // reads miss L2 often enough for the staging to pay (measured: N = 4096 and
+  // the 256-system ensemble faster, N = 16384 and 65536 slower), unless asked
+  // otherwise (quest_autotune times both)
+  int max_sys = 0;
+  for(int s = 0; s < n_sys; ++s) max_sys = std::max(max_sys, c->sys_begin[s + 1] - c->sys_begin[s]);
+  k.pipelined = p->precision == QUEST_PREC_MIXED &&
+                (p->source_loop == QUEST_LOOP_PIPELINED || (p->source_loop == QUEST_LOOP_AUTO && max_sys <= 8192));
   // pair-kernel work list sized for one wave of resident blocks; static
   // claiming unless asked otherwise (quest_autotune times both)
   c->blocks_per_cu = pair_blocks_per_cu(k, p->precision);
@@ -834,6 +842,8 @@ int check_params(const quest_params *p, int n)
   if(p->source_chunk < 0) return fail(nullptr, QUEST_ERR_ARG, "source_chunk must be >= 0");
   if(p->item_claim < QUEST_CLAIM_AUTO || p->item_claim > QUEST_CLAIM_DYNAMIC)
     return fail(nullptr, QUEST_ERR_ARG, "item_claim must be QUEST_CLAIM_AUTO, _STATIC or _DYNAMIC");
+  if(p->source_loop < QUEST_LOOP_AUTO || p->source_loop > QUEST_LOOP_PIPELINED)
+    return fail(nullptr, QUEST_ERR_ARG, "source_loop must be QUEST_LOOP_AUTO, _PLAIN or _PIPELINED");
   return QUEST_OK;
 }
 
@@ -1188,7 +1198,8 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   // every step recomputes from scratch, so timing it between steps changes no
   // state; the device step counter is restored below.
   HIPC(c, launch_set_step(c->d, std::max(c->cur_step, 0), c->stream));
-  int best = c->chunk_len, best_dyn = c->k.dyn_items;
+  int best = c->chunk_len, best_dyn = c->k.dyn_items, best_pf = c->k.pipelined;
+  const bool mixed = c->P.precision == QUEST_PREC_MIXED;
   float best_ms = INFINITY;
   struct Events {  // destroyed on every return path
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1202,9 +1213,14 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   HIPC(c, hipEventCreate(&ev.e1));
   hipEvent_t e0 = ev.e0, e1 = ev.e1;
   for(int len : cand) {
-    for(int dyn : {0, 1}) {
+    for(int dyn : {0, 1})
+    for(int pf : {0, 1}) {
       if(c->P.item_claim != QUEST_CLAIM_AUTO && dyn != (c->P.item_claim == QUEST_CLAIM_DYNAMIC)) continue;
+      if(pf && !mixed) continue;
+      if(mixed && c->P.source_loop != QUEST_LOOP_AUTO && pf != (c->P.source_loop == QUEST_LOOP_PIPELINED)) continue;
       c->k.dyn_items = dyn;
+      c->k.pipelined = pf;
+      c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
       int rc = build_work(c, len);  // the list's layout depends on the claiming mode
       if(rc != QUEST_OK) return rc;
       HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
@@ -1218,10 +1234,13 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
         best_ms = ms;
         best = len;
         best_dyn = dyn;
+        best_pf = pf;
       }
     }
   }
   c->k.dyn_items = best_dyn;
+  c->k.pipelined = best_pf;
+  c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
   int rc = build_work(c, best);
   if(rc != QUEST_OK) return rc;
   if(c->mode == 1 && c->cur_step >= 0) {
@@ -1343,6 +1362,7 @@ int quest_ctx_info(const quest_ctx *c, int64_t *info)
   info[12] = c->step_block;
   info[13] = c->max_group;
   info[14] = c->k.dyn_items;
+  info[15] = c->k.pipelined;
   return QUEST_OK;
 }
 

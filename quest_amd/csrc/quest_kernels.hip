@@ -500,7 +500,7 @@ struct RingPlanesF32 {
   const float2 *last;     // plane of c_M (even M)
 };
 
-template <int M, bool LAB, bool DYN, bool GEN>
+template <int M, bool LAB, bool DYN, bool GEN, bool PF>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void pair_past_f32(
     const float4 *__restrict__ pd, const double4 *__restrict__ pos64, const RingPlanesF32 planes,
     const int4 *__restrict__ work, const int *__restrict__ work_row, int *step_ptr,
@@ -527,13 +527,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
     for(int bt = wi.z; bt < wi.w; bt += kChunk) {
       const int be = min(wi.w, bt + kChunk);
       float sr = 0, si = 0;
-      // Software-pipelined by one source: source b + 1's position and dipole
-      // (one scalar load of 32 B), delay split and coefficient gathers are
-      // issued before source b's arithmetic, so the gathers' cache latency
-      // overlaps it (the last step re-stages b, in bounds and unused).
-      // Byte offsets of the slot (step - n) & mask in the 16-B and 8-B planes;
-      // delay 0 (the self pair, zero-delay partners) reads the current step's
-      // slot, which holds zeros until the step commits.
+      // A source is staged (position and dipole: one scalar load of 32 B;
+      // delay split; coefficient gathers) and then evaluated.  PF: software-
+      // pipelined by one source -- source b + 1 is staged before source b's
+      // arithmetic, so the gathers' cache latency overlaps it (the last step
+      // re-stages b, in bounds and unused); the same arithmetic in the same
+      // order as the plain loop.  Byte offsets of the slot (step - n) & mask
+      // in the 16-B and 8-B planes; delay 0 (the self pair, zero-delay
+      // partners) reads the current step's slot, which holds zeros until the
+      // step commits.
       auto stage = [&](int b, StagedF32<M> &st) {
         const float4 pb = pd[2 * b];
         st.db = pd[2 * b + 1];
@@ -551,16 +553,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
         }
         if constexpr(2 * kPairs == M) st.cv[M] = load_off(planes.last, off16 >> 1);
       };
-      StagedF32<M> cur;
-      stage(bt, cur);
-      for(int b = bt; b < be; ++b) {
-        StagedF32<M> nxt;
-        stage(min(b + 1, be - 1), nxt);
+      auto evaluate = [&](const StagedF32<M> &st) {
         float q0r, q0i, q1r, q1i, q2r, q2i;
-        horner3<float, M>(cur.cv, cur.x, q0r, q0i, q1r, q1i, q2r, q2i);
-        pair_term_f32<LAB, GEN>(K, cur.ir, cur.dx, cur.dy, cur.dz, da, dah, cur.db, cur.x, q0r, q0i, q1r, q1i,
-                                q2r, q2i, sr, si);
-        cur = nxt;
+        horner3<float, M>(st.cv, st.x, q0r, q0i, q1r, q1i, q2r, q2i);
+        pair_term_f32<LAB, GEN>(K, st.ir, st.dx, st.dy, st.dz, da, dah, st.db, st.x, q0r, q0i, q1r, q1i, q2r,
+                                q2i, sr, si);
+      };
+      StagedF32<M> cur;
+      if constexpr(PF) {
+        stage(bt, cur);
+        for(int b = bt; b < be; ++b) {
+          StagedF32<M> nxt;
+          stage(min(b + 1, be - 1), nxt);
+          evaluate(cur);
+          cur = nxt;
+        }
+      } else {
+        for(int b = bt; b < be; ++b) {
+          stage(b, cur);
+          evaluate(cur);
+        }
       }
       accr += static_cast<double>(sr);
       acci += static_cast<double>(si);
@@ -1247,8 +1259,12 @@ void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t
     RingPlanesF32 planes{};
     for(int i = 0; i < (M + 1) / 2; ++i) planes.pair[i] = r4 + i * E;
     planes.last = reinterpret_cast<const float2 *>(r4 + ((M + 1) / 2) * E);
-    hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN>), grid, block, 0, s, d.pd32, d.pos64, planes, d.work,
-                       d.work_row, d.step, d.partial, k);
+    if(k.pipelined)
+      hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, true>), grid, block, 0, s, d.pd32, d.pos64, planes,
+                         d.work, d.work_row, d.step, d.partial, k);
+    else
+      hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, false>), grid, block, 0, s, d.pd32, d.pos64, planes,
+                         d.work, d.work_row, d.step, d.partial, k);
   }
 }
 
@@ -1282,7 +1298,7 @@ hipError_t launch_table_m(const Consts &k, DeviceState &d, hipStream_t s)
 }
 
 template <int M>
-int blocks_per_cu_m(int precision, bool table, bool gen)
+int blocks_per_cu_m(int precision, bool table, bool gen, bool pf)
 {
   int nb = 0;
   hipError_t e;
@@ -1291,9 +1307,15 @@ int blocks_per_cu_m(int precision, bool table, bool gen)
   else if(precision == QUEST_PREC_FP64)
     e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false, true>, kBlock, 0)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false, false>, kBlock, 0);
+  else if(pf)
+    e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, true, true>, kBlock, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, false, true>, kBlock,
+                                                           0);
   else
-    e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, true>, kBlock, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, false>, kBlock, 0);
+    e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, true, false>, kBlock,
+                                                           0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, false, false>, kBlock,
+                                                           0);
   return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
@@ -1380,13 +1402,14 @@ int pair_blocks_per_cu(const Consts &k, int precision)
 {
   const bool t = k.pair_table != 0;
   const bool g = k.dyadic != QUEST_DYADIC_LAB && !k.trig_poly;
+  const bool pf = k.pipelined != 0;
   switch(k.order) {
-  case 1: return blocks_per_cu_m<1>(precision, t, g);
-  case 2: return blocks_per_cu_m<2>(precision, t, g);
-  case 3: return blocks_per_cu_m<3>(precision, t, g);
-  case 4: return blocks_per_cu_m<4>(precision, t, g);
-  case 5: return blocks_per_cu_m<5>(precision, t, g);
-  default: return blocks_per_cu_m<6>(precision, t, g);
+  case 1: return blocks_per_cu_m<1>(precision, t, g, pf);
+  case 2: return blocks_per_cu_m<2>(precision, t, g, pf);
+  case 3: return blocks_per_cu_m<3>(precision, t, g, pf);
+  case 4: return blocks_per_cu_m<4>(precision, t, g, pf);
+  case 5: return blocks_per_cu_m<5>(precision, t, g, pf);
+  default: return blocks_per_cu_m<6>(precision, t, g, pf);
   }
 }
 

@@ -173,3 +173,56 @@ def test_dynamic_item_claiming_is_bitwise_static(ref_weights, precision):
             second[claim] = ctx.state()[0]
     assert np.array_equal(first[CLAIM_STATIC], first[CLAIM_DYNAMIC])
     assert rel_err(second[CLAIM_DYNAMIC], second[CLAIM_STATIC]) < (1e-12 if precision == 0 else 1e-8)
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_pipelined_source_loop_is_bitwise_plain(ref_weights, batch):
+    """The mixed-precision pair kernel's two source loops (plain; staged one
+    source ahead) do the same arithmetic in the same order: whole runs agree
+    bit for bit, for one system and for a batch."""
+    from quest_amd import LOOP_PIPELINED, LOOP_PLAIN, Context, systems
+    consts = systems.consts()
+    if batch:
+        sys_list, pulses = _systems()
+    else:
+        sys_list, pulses = dense_system(300, 0.3, seed=71), short_pulse()
+    rho = {}
+    for loop in (LOOP_PLAIN, LOOP_PIPELINED):
+        with Context(sys_list, consts, precision=1, pulse=pulses, source_loop=loop) as ctx:
+            assert ctx.info()["pipelined"] == (loop == LOOP_PIPELINED)
+            ctx.set_pc_weights(*ref_weights)
+            ctx.run(150)
+            rho[loop] = ctx.state()[0]
+    assert np.abs(rho[LOOP_PLAIN][:, 1]).max() > 0.05
+    assert np.array_equal(rho[LOOP_PLAIN], rho[LOOP_PIPELINED])
+
+
+def test_source_loop_default_and_autotune(ref_weights):
+    """QUEST_LOOP_AUTO: pipelined in mixed precision for systems of at most
+    8192 dots, plain above and in FP64; the autotuner may switch it and
+    changes no state; a pinned loop stays pinned."""
+    from quest_amd import LOOP_PLAIN, Context, systems
+    consts = systems.consts()
+    small = dense_system(400, 0.35, seed=72)
+    large = dense_system(8193, 1.2, seed=73, plant_near=False)
+    with Context(small, consts, precision=1) as ctx:
+        assert ctx.info()["pipelined"] == 1
+    with Context(small, consts, precision=0) as ctx:
+        assert ctx.info()["pipelined"] == 0
+    with Context(large, consts, precision=1) as ctx:
+        assert ctx.info()["pipelined"] == 0
+    with Context(small, consts, precision=1, source_loop=LOOP_PLAIN) as ctx:
+        ctx.set_pc_weights(*ref_weights)
+        ctx.autotune(2)
+        assert ctx.info()["pipelined"] == 0
+    with Context(small, consts, precision=1, pulse=short_pulse()) as a, \
+            Context(small, consts, precision=1, pulse=short_pulse()) as b:
+        for c in (a, b):
+            c.set_pc_weights(*ref_weights)
+        a.run(60)
+        b.run(30)
+        b.autotune(2)
+        b.run(30)
+        assert b.info()["pipelined"] in (0, 1)
+        ra, rb = a.state()[0], b.state()[0]
+    assert rel_err(rb, ra) < 1e-8

@@ -123,7 +123,9 @@ def ensemble_main(args):
         "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK, "traffic": None,
-                     "kernel": "pair_past_f32<4,false,{},false>".format("true" if mixed["info"]["dynamic_items"] else "false"),
+                     "kernel": "pair_past_f32<4,false,{},false,{}>".format(
+                         "true" if mixed["info"]["dynamic_items"] else "false",
+                         "true" if mixed["info"]["pipelined"] else "false"),
                      "kernel_ms": mixed["pair_ms"], "flop_per_pair": F_ALG},
     }
     if fp64 is not None:
@@ -346,6 +348,7 @@ def main(argv=None) -> int:
     ms_per_step = 1e3 * mixed["seconds"] / args.steps
     ordered = float(n) * (n - 1)
     achieved = F_ALG * ordered / (mixed["pair_ms"] * 1e-3) / 1e12
+    kpf = "true" if mixed["info"]["pipelined"] else "false"
     out = {
         "metric": f"pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N={n}, ρ err≤1e-6",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
@@ -363,8 +366,9 @@ def main(argv=None) -> int:
         "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK,
-                     "traffic": profiled_traffic("pair_past_f32<4, false, false, false>") if args.workload == "n65536" else None,
-                     "kernel": "pair_past_f32<4,false,false,false>", "kernel_ms": mixed["pair_ms"],
+                     "traffic": (profiled_traffic("pair_past_f32<4, false, false, false, {}>".format(kpf))
+                                 if args.workload == "n65536" else None),
+                     "kernel": "pair_past_f32<4,false,false,false,{}>".format(kpf), "kernel_ms": mixed["pair_ms"],
                      "flop_per_pair": F_ALG},
     }
     if fp64 is not None:

@@ -479,7 +479,7 @@ void free_device(quest_ctx *c)
   void *ptrs[] = {d.pos64, d.dip64, d.pd32, d.freq, d.t1, d.t2, d.ring, d.y0, d.y1,
                   d.d0, d.d1, d.cur0, d.cur1, d.yc0, d.yc1, d.field, d.partial, d.phase, d.out,
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
-                  d.bad_step, d.counts, d.perm_dev, d.sys_of, d.sys_range, d.sys_chunks,
+                  d.bad_step, d.counts, d.perm_dev, d.io, d.sys_of, d.sys_range, d.sys_chunks,
                   d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff,
                   d.step_map, d.near_lane};
   for(void *p : ptrs)
@@ -764,6 +764,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, alloc(reinterpret_cast<void **>(&d.step), 4 * sizeof(int)));
   HIPC(c, launch_clear(d.step, 4 * sizeof(int), c->stream));  // step and the pair kernels' item counters
   HIPC(c, alloc(reinterpret_cast<void **>(&d.perm_dev), sizeof(int) * nn));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.io), sizeof(double2) * 2 * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_of), sizeof(int) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_range), sizeof(int2) * n_sys));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_chunks), sizeof(int) * n_sys));
@@ -1021,20 +1022,17 @@ int quest_hist_evaluate(quest_ctx *c, int step, const double *rho01, double *out
     HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
     c->cur_step = step;
   }
+  // caller order crosses PCIe as is (sequential copies through the pinned
+  // stage); the permutation to and from the sorted order runs on the device
   const int n = c->n;
-  for(int s = 0; s < n; ++s) {
-    const int i = c->perm[s];
-    c->h_stage[s] = make_double2(rho01[2 * i], rho01[2 * i + 1]);
-  }
-  HIPC(c, hipMemcpyAsync(c->d.cur1, c->h_stage, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+  std::memcpy(c->h_stage, rho01, sizeof(double2) * n);
+  HIPC(c, hipMemcpyAsync(c->d.io, c->h_stage, sizeof(double2) * n, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, launch_permute_in(c->k, c->d, c->stream));
   HIPC(c, launch_hist_near(c->k, c->d, c->stream));
-  HIPC(c, hipMemcpyAsync(c->h_stage + n, c->d.out, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, launch_permute_out(c->k, c->d, c->stream));
+  HIPC(c, hipMemcpyAsync(c->h_stage + n, c->d.io + n, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  for(int s = 0; s < n; ++s) {
-    const int i = c->perm[s];
-    out[2 * i] = c->h_stage[n + s].x;
-    out[2 * i + 1] = c->h_stage[n + s].y;
-  }
+  std::memcpy(out, c->h_stage + n, sizeof(double2) * n);
   if(c->timing) return collect_timing(c);
   return QUEST_OK;
 }

@@ -101,6 +101,7 @@ struct DeviceState {
   double2 *partial = nullptr;          // [max_chunks][n]
   double2 *phase = nullptr;            // e^{i theta s}, s in [0, phase_len)
   double2 *out = nullptr;              // [n] staging of per-call results
+  double2 *io = nullptr;               // [2n] per-call mode: rho01 in caller order, then results in it
   int *near_start = nullptr;           // [n + 1]
   int *near_src = nullptr;             // [near]
   int *near_tgt = nullptr;             // [near]
@@ -157,6 +158,10 @@ hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, double
 hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, int step, hipStream_t s);
 hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, hipStream_t s);
 hipError_t launch_hist_near(const Consts &k, DeviceState &d, hipStream_t s);
+// Per-call mode: caller order <-> sorted order on the device (d.io, d.perm_dev):
+// cur1[sorted] = io[caller]; io[n + caller] = out[sorted].
+hipError_t launch_permute_in(const Consts &k, DeviceState &d, hipStream_t s);
+hipError_t launch_permute_out(const Consts &k, DeviceState &d, hipStream_t s);
 hipError_t launch_pulse(const Consts &k, DeviceState &d, hipStream_t s);
 // Occupancy of the pair kernel (blocks per CU) for sizing the persistent grid.
 int pair_blocks_per_cu(const Consts &k, int precision);

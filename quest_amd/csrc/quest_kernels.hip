@@ -1033,6 +1033,22 @@ __global__ __launch_bounds__(kBlock) void hist_reduce_kernel(Consts k, DeviceSta
   d.field[a] = hist;
 }
 
+// Per-call mode: the caller's rho01 (caller order, io[0..n)) into the sorted
+// iterate; the results back into caller order (io[n..2n)).  inv: caller ->
+// sorted index.
+__global__ __launch_bounds__(kBlock) void permute_in_kernel(int n, const double2 *__restrict__ io,
+                                                            const int *__restrict__ inv, double2 *__restrict__ cur1)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) cur1[inv[i]] = io[i];
+}
+__global__ __launch_bounds__(kBlock) void permute_out_kernel(int n, const double2 *__restrict__ out,
+                                                             const int *__restrict__ inv, double2 *__restrict__ io)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) io[n + i] = out[inv[i]];
+}
+
 // Per-call mode: out = past pair sum + zero-delay pairs with the current rho01.
 __global__ __launch_bounds__(kBlock) void hist_near_kernel(Consts k, DeviceState d)
 {
@@ -1463,6 +1479,18 @@ hipError_t launch_hist_reduce(const Consts &k, DeviceState &d, hipStream_t s)
 hipError_t launch_hist_near(const Consts &k, DeviceState &d, hipStream_t s)
 {
   hipLaunchKernelGGL(hist_near_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute_in(const Consts &k, DeviceState &d, hipStream_t s)
+{
+  hipLaunchKernelGGL(permute_in_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k.n, d.io, d.perm_dev, d.cur1);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute_out(const Consts &k, DeviceState &d, hipStream_t s)
+{
+  hipLaunchKernelGGL(permute_out_kernel, dot_grid(k.n), dim3(kBlock), 0, s, k.n, d.out, d.perm_dev, d.io);
   return hipGetLastError();
 }
 

@@ -1097,14 +1097,12 @@ int quest_pulse_evaluate(quest_ctx *c, int step, double *out)
   HIPC(c, launch_set_step(c->d, step, c->stream));
   HIPC(c, launch_pulse(c->k, c->d, c->stream));
   if(c->mode == 1 && c->cur_step >= 0) HIPC(c, launch_set_step(c->d, c->cur_step, c->stream));
+  // back to caller order on the device, then one sequential copy
   const int n = c->n;
-  HIPC(c, hipMemcpyAsync(c->h_stage + n, c->d.out, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, launch_permute_out(c->k, c->d, c->stream));
+  HIPC(c, hipMemcpyAsync(c->h_stage + n, c->d.io + n, sizeof(double2) * n, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  for(int s = 0; s < n; ++s) {
-    const int i = c->perm[s];
-    out[2 * i] = c->h_stage[n + s].x;
-    out[2 * i + 1] = c->h_stage[n + s].y;
-  }
+  std::memcpy(out, c->h_stage + n, sizeof(double2) * n);
   return QUEST_OK;
 }
 

@@ -1,0 +1,28 @@
+"""bench.py's contract pieces that need no GPU: the committed rocprof
+summaries it reads its HBM traffic from carry the kernel names it labels its
+line with, and a request for more than one GPU is refused with a message."""
+import json
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_profiled_traffic_matches_bench_labels():
+    sys.path.insert(0, ROOT)
+    import bench
+    # the N = 65536 line's kernel (plain source loop: the default above 8192 dots)
+    t32 = bench.profiled_traffic("pair_past_f32<4, false, false, false, false>")
+    assert t32 is not None and 5e8 < t32 < 1e10
+    ttab = bench.profiled_traffic("pair_table_f64<4, false>", "pair_table_n16384.md")
+    assert ttab is not None and ttab > 1e10
+    assert bench.profiled_traffic("no_such_kernel<1>") is None
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    msg = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in msg["error"]

@@ -57,3 +57,25 @@ def test_systems_generator_density():
     assert np.all(np.abs(s["pos"]) <= L / 2)
     assert abs(systems.pulse_amplitude(np.pi / 2) - 7794.6130486) < 1e-3
     assert abs(systems.pulse_amplitude(np.pi) - 15589.226097) < 1e-3
+
+
+def test_struct_layouts_match_header():
+    """The ctypes mirrors of quest_params and quest_pulse list the header's
+    fields in order (a field added on one side only would shift every later
+    one), and the documented info length is the one the front end reads."""
+    from quest_amd import capi
+    text = open(os.path.join(ROOT, "include", "quest_amd.h")).read()
+    for ctype, cname in ((capi.Params, "quest_params"), (capi.Pulse, "quest_pulse")):
+        end = text.index("} " + cname + ";")
+        body = text[text.rindex("typedef struct {", 0, end) + len("typedef struct {"):end]
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        names = []
+        for decl in body.split(";"):
+            m = re.match(r"\s*(?:double|int)\s+(.*)", decl, re.S)
+            if m:
+                names += [re.sub(r"\[\d+\]", "", v).strip() for v in m.group(1).split(",")]
+        assert names == [f[0] for f in ctype._fields_], cname
+    n_info = int(re.search(r"info:\s*\[(\d+)\]\s*int64", text).group(1))
+    assert n_info == 16
+    import inspect
+    assert f"c_int64 * {n_info}" in inspect.getsource(capi.Context.info)

@@ -42,6 +42,8 @@ def _oracle_y(rho01, window=22):
     (0, 0, 4, 1000, 1.2),
     (1, 0, 4, 64, 0.25),
     (1, 0, 4, 1000, 1.2),
+    (1, 1, 4, 64, 0.25),
+    (1, 0, 6, 64, 0.25),
 ])
 def test_hist_evaluate_matches_oracle(oracle_lib, ref_weights, precision, dyadic, order, n, side):
     from quest_amd import Context, systems

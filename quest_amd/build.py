@@ -90,19 +90,6 @@ def build_programs(lib: str = LIB) -> list:
     return out
 
 
-def build_probe() -> str:
-    """`valu_probe`: sustained VALU issue rates for a few instruction mixes (no
-    library dependency; compiled with the kernels' flags)."""
-    os.makedirs(BINDIR, exist_ok=True)
-    exe = os.path.join(BINDIR, "valu_probe")
-    cmd = [hipcc()] + common_flags() + [os.path.join(CSRC, "valu_probe.hip"), "-o", exe + ".tmp"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"build of valu_probe failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(exe + ".tmp", exe)
-    return exe
-
-
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=4)
@@ -111,7 +98,7 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     print(build(args.jobs, args.out, args.defines))
     if args.out == LIB:
-        print("\n".join(build_programs() + [build_probe()]))
+        print("\n".join(build_programs()))
     return 0
 
 

@@ -41,13 +41,11 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b)
   return make_double2(a.x + b.x, a.y + b.y);
 }
 
-// FP32 delay tau = r / (c0 dt) of a pair whose FP32 positions are stored in
-// units of c0 dt, and 1/r in those units.  Written with explicit fused
-// operations and contraction off so that every kernel that classifies a pair
-// (the pair sum and the zero-delay list builder) gets bit-identical results.
-// r2 is clamped to 1e-30 (physical pairs are ~0.01 and above in these units)
-// so that tau is finite for coincident dots too.
-// r^2 (units of (c0 dt)^2) and 1/r of the FP32 pair arithmetic.  r^2 is
+// r^2 (units of (c0 dt)^2) and 1/r of the FP32 pair arithmetic (positions
+// stored in units of c0 dt, so r is the delay tau).  Written with explicit
+// fused operations and contraction off so that every kernel that classifies a
+// pair (the pair sum and the zero-delay list builder) gets bit-identical
+// results.  r^2 is
 // clamped to 1e-20 (pairs that close have delay 0), so the geometry of a dot
 // with itself or a coincident one stays finite (1/r <= 1e10) and its term is
 // an exact 0 against the zeroed slot it reads.  (A max, not a seed added in
@@ -435,13 +433,10 @@ struct ItemCursor {
   }
 };
 
-constexpr int kChunk = 64;
-#ifndef QUEST_F64_ATTR
-#define QUEST_F64_ATTR
-#endif   // sources summed in the pair type before the FP64 fold
+constexpr int kChunk = 64;  // sources summed in the pair type before the FP64 fold
 
 template <int M, bool LAB, bool DYN, bool GEN>
-__global__ __launch_bounds__(kBlock) QUEST_F64_ATTR void pair_past_f64(
+__global__ __launch_bounds__(kBlock) void pair_past_f64(
     const double4 *__restrict__ pos, const double4 *__restrict__ dip,
     const double2 *__restrict__ ring, const int4 *__restrict__ work, const int *__restrict__ work_row,
     int *step_ptr, double2 *__restrict__ partial, Consts k)

@@ -16,10 +16,16 @@ the mean over the timed region itself; the same number of steps replayed as
 one captured graph per step (quest_run's default) follows, reported as
 graph_ms_per_step.
 
+After the timed runs, `driven` compares mixed with FP64 on the same system
+in a driven state (100 steps under a short, strong pi/2 pulse, |rho01| up to
+~0.5), against the metric's 1e-6 bound.
+
 The CPU baseline is the FP64 oracle's reference method (pair table plus 11
 scatter sums per step, history_interaction.cpp:47-70 + integrator.cpp:129-140)
-on one host core, over a bounded sample (5 steps of the N = 4096 system; the
-table of N = 65536 would need 180 GB).
+on one host core, over a bounded sample: 3 steps of the N = 4096 system at
+steady state (past its largest delay, every tap summed; the table of N = 65536
+would need 180 GB).  `cpu_c0` is the whole N = 64, 20000-step run of config 0
+on one core, `cpu_best` the split gather on up to 16 threads.
 """
 from __future__ import annotations
 
@@ -38,6 +44,10 @@ sys.path.insert(0, ROOT)
 F_ALG = 105            # flop per ordered pair per step (algorithmic count)
 FP32_PEAK = 157.3      # TFLOP/s, MI355X vector FP32 (datasheet)
 FP64_PEAK = 78.6       # TFLOP/s, MI355X vector FP64
+# TFLOP/s: non-packed FP32 FMAs issue at one 64-lane instruction per cycle per
+# CU (256 CUs x 2.4 GHz x 64 lanes x 2 flop), the same rate as FP64; the
+# 157.3 datasheet figure needs packed FP32, which this build does not use
+FMA_CEILING = 78.6
 
 
 def timed_steps(ctx, warmup, steps):
@@ -122,7 +132,8 @@ def ensemble_main(args):
                    "dynamic_items": mixed["info"]["dynamic_items"]},
         "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": None,
+                     "frac": achieved / FP32_PEAK, "traffic": None, "fma_issue_ceiling": FMA_CEILING,
+                     "frac_of_fma_issue_ceiling": achieved / FMA_CEILING,
                      "kernel": "pair_past_f32<4,false,{},false,{}>".format(
                          "true" if mixed["info"]["dynamic_items"] else "false",
                          "true" if mixed["info"]["pipelined"] else "false"),
@@ -228,50 +239,77 @@ def profiled_traffic(kernel: str, profile: str = "pair_kernels_n65536_pmc.md"):
     return None
 
 
-def cpu_baseline(steps=5):
+def _oracle_setup():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import pc_weights
     from quest_amd import systems
     oracle.build()
+    c = systems.consts()
+    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
+    return oracle, p, pc_weights.weights(), c
+
+
+def _qo_pulse(oracle, pl):
+    return oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"], pl["wavevector"],
+                        pl["polarization"])
+
+
+def cpu_baseline(steps=3):
+    """The reference method (pair table + 11 scatter sums per step,
+    history_interaction.cpp:47-70 + integrator.cpp:129-140) on one host core,
+    timed at steady state: the timed steps start past the largest delay plus
+    the order, on a non-zero synthetic prehistory, so every tap of every pair
+    is summed (the reference skips taps before its prehistory only in the
+    first ~D_max steps)."""
+    from quest_amd import systems
+    oracle, p, (ps, cs, fc), c = _oracle_setup()
     oracle.set_threads(1)
     sysd, _ = systems.workload("n4096")
-    c = systems.consts()
-    pl = systems.pulse(math.pi / 2)
-    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
-    ps, cs, fc = pc_weights.weights()
-    r = oracle.time_reference_steps(
-        p, sysd, oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
-                              pl["wavevector"], pl["polarization"]), ps, cs, fc, steps)
     n = sysd["pos"].shape[0]
+    start = oracle.steady_start(sysd["pos"], c["c0"], c["dt"], 4)
+    r = oracle.time_steps(p, sysd, _qo_pulse(oracle, systems.pulse(math.pi / 2)), ps, cs, fc, steps,
+                          start_step=start, mode=0)
+    full = r["taps"] == 11 * steps * (n * (n - 1) // 2) * 5 * 2
     return {"value": n * n * steps / r["step_seconds"], "unit": "pair-interactions/s", "cores": 1,
             "kind": "port",
-            "sample": (f"FP64 oracle, reference method (pair table + 11 scatter sums/step), "
-                       f"N={n} system of the same generator, {steps} steps in "
-                       f"{r['step_seconds']:.2f} s on 1 core (table build {r['setup_seconds']:.1f} s "
-                       f"excluded); N=65536 needs a 180 GB table")}
+            "sample": (f"FP64 oracle, reference method (pair table + 11 scatter sums/step), N={n} system "
+                       f"of the same generator, steps {start}-{start + steps - 1} after a synthetic "
+                       f"non-zero prehistory ({'every' if full else 'NOT every'} tap summed: "
+                       f"{r['taps']} pair-tap terms), {r['step_seconds']:.2f} s on 1 core (table build "
+                       f"{r['setup_seconds']:.1f} s excluded); N=65536 needs a 180 GB table")}
 
 
-def cpu_baseline_ensemble(ens, steps=8, every=32):
-    """The reference method on one host core, timed per system and summed
-    over a seeded sample of the ensemble (every 32nd system, the ensemble's
-    own seeds and pulses): sum of N^2 x steps over the summed step seconds."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    import pc_weights
+def cpu_c0():
+    """Config 0 as the reference runs it: N = 64, all 20000 steps from the
+    ground state through the pi/2 pulse, reference method, one core."""
     from quest_amd import systems
-    oracle.build()
+    oracle, p, (ps, cs, fc), c = _oracle_setup()
     oracle.set_threads(1)
-    c = systems.consts()
-    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
-    ps, cs, fc = pc_weights.weights()
+    sysd, steps = systems.workload("n64")
+    r = oracle.solve(p, sysd, _qo_pulse(oracle, systems.pulse(math.pi / 2)), ps, cs, fc, steps, mode=0)
+    n = sysd["pos"].shape[0]
+    return {"value": n * n * steps / r["seconds"], "unit": "pair-interactions/s", "cores": 1, "kind": "port",
+            "seconds": r["seconds"],
+            "sample": (f"FP64 oracle, reference method, N={n} (0.5 um cube), the whole {steps}-step run "
+                       f"from the ground state in {r['seconds']:.1f} s on 1 core (table build included)")}
+
+
+def cpu_baseline_ensemble(ens, steps=3, every=32):
+    """The reference method on one host core at steady state (see
+    cpu_baseline), timed per system and summed over a seeded sample of the
+    ensemble (every 32nd system, the ensemble's own seeds and pulses): sum of
+    N^2 x steps over the summed step seconds."""
+    oracle, p, (ps, cs, fc), c = _oracle_setup()
+    oracle.set_threads(1)
     pairs = secs = setup = 0.0
+    full = True
     sample = ens[::every]
     for sysd, pl in sample:
-        r = oracle.time_reference_steps(
-            p, sysd, oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
-                                  pl["wavevector"], pl["polarization"]), ps, cs, fc, steps)
         n = sysd["pos"].shape[0]
+        start = oracle.steady_start(sysd["pos"], c["c0"], c["dt"], 4)
+        r = oracle.time_steps(p, sysd, _qo_pulse(oracle, pl), ps, cs, fc, steps, start_step=start, mode=0)
+        full = full and r["taps"] == 11 * steps * (n * (n - 1) // 2) * 5 * 2
         pairs += float(n * n * steps)
         secs += r["step_seconds"]
         setup += r["setup_seconds"]
@@ -279,35 +317,54 @@ def cpu_baseline_ensemble(ens, steps=8, every=32):
     return {"value": pairs / secs, "unit": "pair-interactions/s", "cores": 1, "kind": "port",
             "sample": (f"FP64 oracle, reference method (pair table + 11 scatter sums/step), "
                        f"{len(sample)} of the 256 systems (every {every}nd, N {min(sizes)}-{max(sizes)}), "
-                       f"{steps} steps each, {secs:.2f} s of steps on 1 core (table builds {setup:.1f} s "
-                       f"excluded), timed per system and summed")}
+                       f"{steps} steps each past its largest delay after a synthetic prehistory "
+                       f"({'every' if full else 'NOT every'} tap summed), {secs:.2f} s of steps on 1 core "
+                       f"(table builds {setup:.1f} s excluded), timed per system and summed")}
 
 
-def cpu_best(steps=20):
+def cpu_best(steps=10):
     """The oracle's split gather (one pair sum per step plus the zero-delay
     pairs per evaluation, the arithmetic the GPU path uses) on up to 16 host
-    threads (the host cores available per GPU): the best CPU figure beside the
-    serial reference method."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    import pc_weights
+    threads (the host cores available per GPU), at steady state like
+    cpu_baseline: the best CPU figure beside the serial reference method."""
     from quest_amd import systems
-    oracle.build()
+    oracle, p, (ps, cs, fc), c = _oracle_setup()
     threads = max(1, min(16, os.cpu_count() or 1))
     oracle.set_threads(threads)
     sysd, _ = systems.workload("n4096")
-    c = systems.consts()
-    pl = systems.pulse(math.pi / 2)
-    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
-    ps, cs, fc = pc_weights.weights()
-    r = oracle.solve(p, sysd, oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
-                                           pl["wavevector"], pl["polarization"]), ps, cs, fc, steps, mode=1)
+    start = oracle.steady_start(sysd["pos"], c["c0"], c["dt"], 4)
+    r = oracle.time_steps(p, sysd, _qo_pulse(oracle, systems.pulse(math.pi / 2)), ps, cs, fc, steps,
+                          start_step=start, mode=1)
     oracle.set_threads(1)
     n = sysd["pos"].shape[0]
-    return {"value": n * n * steps / r["seconds"], "unit": "pair-interactions/s", "cores": threads,
+    return {"value": n * n * steps / r["step_seconds"], "unit": "pair-interactions/s", "cores": threads,
             "kind": "port",
-            "sample": (f"FP64 oracle, split gather (OpenMP), N={n} system of the same generator, "
-                       f"{steps} steps in {r['seconds']:.2f} s on {threads} threads (setup excluded)")}
+            "sample": (f"FP64 oracle, split gather (OpenMP), N={n} system of the same generator, steps "
+                       f"{start}-{start + steps - 1} after a synthetic prehistory, {r['step_seconds']:.2f} s "
+                       f"on {threads} threads (setup excluded)")}
+
+
+def driven_error(sysd, consts, weights, steps=100):
+    """Mixed against FP64 in a driven state: the same system under a short,
+    strong pulse (pi/2 area, sigma 15 steps, centred at step 50), so that
+    |rho01| is large within the 100 steps; rho01 error relative to the FP64
+    run's max |rho01|, rho00 error absolute."""
+    from quest_amd import Context
+    amp = (math.pi / 2) * 0.65821193 / (5.2917721e-4 * 0.0015 * math.sqrt(2 * math.pi))
+    pl = {"amplitude": amp, "delay": 0.005, "width": consts["laser_omega"] * 0.0015,
+          "freq": consts["laser_omega"], "wavevector": [consts["laser_omega"] / consts["c0"], 0.0, 0.0],
+          "polarization": [0.0, 0.0, 1.0]}
+    rho = {}
+    for precision in (0, 1):
+        with Context(sysd, consts, precision=precision, pulse=pl) as ctx:
+            ctx.set_pc_weights(*weights)
+            ctx.run(steps)
+            rho[precision], _ = ctx.state()
+    err01 = rho01_rel_err(rho[1], rho[0])
+    err00 = float(np.abs(rho[1][:, 0] - rho[0][:, 0]).max())
+    return {"steps": steps, "pulse": "pi/2 area, sigma 0.0015 ps, delay 0.005 ps",
+            "max_abs_rho01": float(np.abs(rho[0][:, 1]).max()), "rel_rho01_err_vs_fp64": err01,
+            "max_abs_drho00_vs_fp64": err00, "bound": 1e-6, "within_bound": bool(err01 <= 1e-6 and err00 <= 1e-6)}
 
 
 def main(argv=None) -> int:
@@ -318,6 +375,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="n65536")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-driven", action="store_true",
+                    help="skip the driven-state (short strong pulse) mixed-vs-FP64 error runs")
     ap.add_argument("--pair-method", choices=("on-the-fly", "table"), default="on-the-fly",
                     help="table: the FP64 coefficient-table method beside FP64 on the fly")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -369,7 +428,8 @@ def main(argv=None) -> int:
                      "traffic": (profiled_traffic("pair_past_f32<4, false, false, false, {}>".format(kpf))
                                  if args.workload == "n65536" else None),
                      "kernel": "pair_past_f32<4,false,false,false,{}>".format(kpf), "kernel_ms": mixed["pair_ms"],
-                     "flop_per_pair": F_ALG},
+                     "flop_per_pair": F_ALG, "fma_issue_ceiling": FMA_CEILING,
+                     "frac_of_fma_issue_ceiling": achieved / FMA_CEILING},
     }
     if fp64 is not None:
         v64 = pairs * args.steps / fp64["seconds"]
@@ -388,9 +448,12 @@ def main(argv=None) -> int:
             out["subsample_4096"] = {
                 "seed": 8, "max_abs_drho_vs_fp64": float(np.abs(mixed["rho"][sub] - fp64["rho"][sub]).max()),
                 "rel_rho01_err_vs_fp64": rho01_rel_err(mixed["rho"][sub], fp64["rho"][sub])}
+        if not args.no_driven:
+            out["driven"] = driven_error(sysd, consts, weights)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_best"] = cpu_best()
+        out["cpu_c0"] = cpu_c0()
     print(json.dumps(out))
     return 0
 

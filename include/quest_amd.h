@@ -60,7 +60,7 @@ typedef struct {
   double laser_omega;  /* config.omega, rad/ps (rotating frame)             */
   double dt;           /* config.dt, ps                                     */
   int interp_order;    /* config.interpolation_order, 1..6                  */
-  int history_window;  /* PC window; main.cpp:30 uses 22 (1..64)            */
+  int history_window;  /* PC window; main.cpp:30 uses 22 (2..64)            */
   int precision;       /* QUEST_PREC_*                                      */
   int dyadic;          /* QUEST_DYADIC_*                                    */
   int device;          /* HIP device ordinal                                */
@@ -140,8 +140,9 @@ const char *quest_last_error(const quest_ctx *ctx);
  * `step` -- the only history slot the integrator changes between calls
  * (integrator.cpp:142-185).  Values at earlier steps are the ones passed with
  * the last call for that step; steps before 0 are the ground-state prehistory
- * (rho01 = 0, main.cpp:32-36).  `step` must not decrease and may grow by at
- * most one per call. */
+ * (rho01 = 0, main.cpp:32-36).  The first call on a context is at step 0;
+ * `step` must not decrease and may grow by at most one per call (else
+ * QUEST_ERR_STATE).  rho01 and out hold n complex values each. */
 int quest_hist_evaluate(quest_ctx *ctx, int step, const double *rho01,
                         double *out);
 
@@ -231,8 +232,16 @@ int quest_get_state(quest_ctx *ctx, double *rho, double *drho);
 
 /* Integer delay floor(|r_obs - r_src| / (c0 dt)) of every unordered pair in
  * the reference's pair order (history_interaction.cpp:25-34, idx2coord
- * :80-86), computed by the device's FP64 delay split.  delays: [n(n-1)/2]. */
+ * :80-86; the pair index as a 64-bit integer), computed on the device by the
+ * delay split of the context's pair kernel: the FP64 split (QUEST_PREC_FP64)
+ * or the mixed path's FP32 estimate with the FP64 split near integers
+ * (QUEST_PREC_MIXED), so the two precisions' classifications can be
+ * compared pair by pair.  delays: [n(n-1)/2]. */
 int quest_pair_delays(quest_ctx *ctx, int32_t *delays);
+
+/* The same for the pairs [first, first + count) of that order (delays:
+ * [count]); QUEST_ERR_ARG outside [0, n(n-1)/2). */
+int quest_pair_delays_range(quest_ctx *ctx, int64_t first, int64_t count, int32_t *delays);
 
 /* Enables HIP-event timing of the pair kernel (on the context's stream). */
 int quest_set_timing(quest_ctx *ctx, int enable);

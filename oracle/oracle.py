@@ -71,10 +71,10 @@ def lib():
                                       C.POINTER(QoPulse), C.c_int, dp, dp, C.c_double, C.c_int,
                                       C.c_int, C.c_int, dp, dp, dp, C.POINTER(C.c_int), dp]
         L.qo_solve_pulses.restype = C.c_int
-        L.qo_time_reference_steps.argtypes = [C.POINTER(QoParams), C.c_int, dp, dp, dp, dp,
-                                              dp, C.POINTER(QoPulse), dp, dp, C.c_double,
-                                              C.c_int, dp, dp]
-        L.qo_time_reference_steps.restype = C.c_double
+        L.qo_time_steps.argtypes = [C.POINTER(QoParams), C.c_int, dp, dp, dp, dp, dp,
+                                    C.POINTER(QoPulse), dp, dp, C.c_double, C.c_int, C.c_int,
+                                    C.c_int, dp, dp, C.POINTER(C.c_int64)]
+        L.qo_time_steps.restype = C.c_double
         L.qo_set_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
@@ -233,13 +233,27 @@ def solve(p: QoParams, system, pl, ps, cs, fc, n_steps: int, mode: int = 1,
             "status": status, "bad_step": bad.value, "seconds": secs.value}
 
 
-def time_reference_steps(p: QoParams, system, pl: QoPulse, ps, cs, fc, n_steps: int):
+def time_steps(p: QoParams, system, pl: QoPulse, ps, cs, fc, n_steps: int, start_step: int = 0,
+               mode: int = 0):
+    """Times steps [start_step, start_step + n_steps) of the reference method
+    (mode 0, serial) or the split gather (mode 1, OpenMP) after a synthetic
+    non-zero prehistory (qo_time_steps).  Returns checksum, setup_seconds,
+    step_seconds and taps (the pair-tap terms the timed evaluations added)."""
     pos, dip = _f64(system["pos"]), _f64(system["dip"])
     freq, t1, t2 = _f64(system["freq"]), _f64(system["t1"]), _f64(system["t2"])
     ps, cs = _f64(ps).ravel(), _f64(cs).ravel()
-    setup, steps = C.c_double(0), C.c_double(0)
-    chk = lib().qo_time_reference_steps(C.byref(p), pos.shape[0], _dp(pos), _dp(dip),
-                                        _dp(freq), _dp(t1), _dp(t2), C.byref(pl), _dp(ps),
-                                        _dp(cs), float(fc), n_steps, C.byref(setup),
-                                        C.byref(steps))
-    return {"checksum": chk, "setup_seconds": setup.value, "step_seconds": steps.value}
+    setup, steps, taps = C.c_double(0), C.c_double(0), C.c_int64(0)
+    chk = lib().qo_time_steps(C.byref(p), pos.shape[0], _dp(pos), _dp(dip), _dp(freq), _dp(t1),
+                              _dp(t2), C.byref(pl), _dp(ps), _dp(cs), float(fc), int(mode),
+                              int(start_step), int(n_steps), C.byref(setup), C.byref(steps),
+                              C.byref(taps))
+    return {"checksum": chk, "setup_seconds": setup.value, "step_seconds": steps.value,
+            "taps": taps.value}
+
+
+def steady_start(pos, c0: float, dt: float, order: int) -> int:
+    """First step whose evaluations skip no tap: the bounding-box delay bound
+    plus the order plus one."""
+    pos = np.asarray(pos, dtype=np.float64)
+    diag = float(np.linalg.norm(pos.max(axis=0) - pos.min(axis=0)))
+    return int(diag / (c0 * dt)) + order + 1

@@ -680,39 +680,93 @@ int qo_solve_pulses(const qo_params *p, int kind, int n, const double *pos,
   return status;
 }
 
-double qo_time_reference_steps(const qo_params *p, int n, const double *pos,
-                               const double *dip, const double *freq,
-                               const double *t1, const double *t2,
-                               const qo_pulse *pulse, const double *ps,
-                               const double *cs, double future_coef,
-                               int n_steps, double *setup_seconds,
-                               double *step_seconds)
+/* Terms (pair, tap, direction) the evaluations of steps [start, start + count)
+ * add: mode 0, 11 reference evaluations per step, each adding both directions
+ * of every (pair, tap) that history_interaction.cpp:57 does not skip; mode 1,
+ * the past sum once per step over the pairs of delay >= 1 plus the zero-delay
+ * pairs at each of the 11 evaluations (pair_sum's skip).  The same condition
+ * as the evaluation loops, counted in a separate pass. */
+static int64_t count_taps(const qo_run *R, int mode, int start, int count)
 {
+  const int M = R->p->order, W = R->p->window;
+  const int64_t npairs = (int64_t)R->n * (R->n - 1) / 2;
+  int64_t taps = 0;
+  for(int step = start; step < start + count; ++step)
+    for(int64_t pair = 0; pair < npairs; ++pair) {
+      const int s = step - R->delays[pair];
+      int64_t live = 0;
+      for(int i = 0; i <= M; ++i)
+        if(s - i >= -W) ++live;
+      const int evals = (mode == 0 || R->delays[pair] == 0) ? 11 : 1;
+      taps += 2 * live * evals;
+    }
+  return taps;
+}
+
+/* A smooth synthetic history in every slot t in [-window, start): rho01 =
+ * A(t) e^{i phi_i}, A(t) = 0.2 (1 - e^{-(t + window + 1) / 60}), phi_i = 0.7 i,
+ * rho00 = 1 - |rho01|^2, and the time derivatives (per ps) of the same
+ * formulas.  No slot that steps >= start read is zero, so with start beyond
+ * the largest delay plus the order the reference's skip of taps before the
+ * prehistory (history_interaction.cpp:57) never applies. */
+static void fill_synthetic_history(qo_run *R, int start)
+{
+  const int W = R->p->window;
+  const double dt = R->p->dt;
+  for(int i = 0; i < R->n; ++i) {
+    const cplx ph = cexp(I * (0.7 * i));
+    for(int t = -W; t < start; ++t) {
+      const double e = exp(-(t + W + 1) / 60.0);
+      const double a = 0.2 * (1.0 - e), da = 0.2 * e / (60.0 * dt);
+      *Y(R, i, t, 0) = 1.0 - a * a;
+      *Y(R, i, t, 1) = a * ph;
+      *YP(R, i, t, 0) = -2.0 * a * da;
+      *YP(R, i, t, 1) = da * ph;
+    }
+  }
+}
+
+double qo_time_steps(const qo_params *p, int n, const double *pos, const double *dip,
+                     const double *freq, const double *t1, const double *t2,
+                     const qo_pulse *pulse, const double *ps, const double *cs,
+                     double future_coef, int mode, int start_step, int n_steps,
+                     double *setup_seconds, double *step_seconds, int64_t *taps)
+{
+  if(start_step < 0 || n_steps < 1) return NAN;
   qo_run R;
   if(run_alloc(&R, p, 0, n, pos, dip, freq, t1, t2, pulse, pulse ? 1 : 0, ps,
-               cs, future_coef, n_steps) != 0) {
+               cs, future_coef, start_step + n_steps) != 0) {
     run_free(&R);
     return NAN;
   }
+  if(start_step > 0) fill_synthetic_history(&R, start_step);
   const double t0 = now_seconds();
   qo_build_table(p, 0, n, pos, dip, R.delays, R.coef);
+  if(mode == 1) build_near_lists(&R);
   const double t1s = now_seconds();
   cplx *field = (cplx *)calloc(n, sizeof(cplx));
-  for(int step = 0; step < n_steps; ++step) {
+  for(int step = start_step; step < start_step + n_steps; ++step) {
+    if(mode == 1) {
+      past_sums(&R, step);
+      for(int i = 0; i < n; ++i) R.pulse_now[i] = pulses_at(&R, i, step * p->dt);
+    }
     predictor(&R, step);
-    evaluator_reference(&R, step, field);
+    if(mode == 0) evaluator_reference(&R, step, field);
+    else evaluator_split(&R, step, field);
     apply_rhs(&R, step, field);
     for(int m = 0; m < 10; ++m) {
       corrector(&R, step);
-      evaluator_reference(&R, step, field);
+      if(mode == 0) evaluator_reference(&R, step, field);
+      else evaluator_split(&R, step, field);
       apply_rhs(&R, step, field);
     }
   }
   const double t2s = now_seconds();
   double checksum = 0;
-  for(int i = 0; i < n; ++i) checksum += creal(*Y(&R, i, n_steps - 1, 0));
+  for(int i = 0; i < n; ++i) checksum += creal(*Y(&R, i, start_step + n_steps - 1, 0));
   if(setup_seconds) *setup_seconds = t1s - t0;
   if(step_seconds) *step_seconds = t2s - t1s;
+  if(taps) *taps = count_taps(&R, mode, start_step, n_steps);
   free(field);
   run_free(&R);
   return checksum;

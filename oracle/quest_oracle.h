@@ -124,16 +124,19 @@ int qo_solve_pulses(const qo_params *p, int kind, int n, const double *pos,
                     double *final_y, double *final_yp, int *bad_step,
                     double *seconds);
 
-/* The reference method's step loop only, for timing: builds the table (its
- * time goes to *setup_seconds), then runs n_steps of predictor + 11 scatter
- * sums on a ground-state system with the pulse on.  Returns a checksum. */
-double qo_time_reference_steps(const qo_params *p, int n, const double *pos,
-                               const double *dip, const double *freq,
-                               const double *t1, const double *t2,
-                               const qo_pulse *pulse, const double *ps,
-                               const double *cs, double future_coef,
-                               int n_steps, double *setup_seconds,
-                               double *step_seconds);
+/* The step loop only, for timing: builds the table (and, mode 1, the
+ * zero-delay lists; their time goes to *setup_seconds), fills every slot
+ * before start_step with a smooth non-zero synthetic history (start_step > 0),
+ * then runs steps [start_step, start_step + n_steps) with the pulse on:
+ * mode 0 the reference method (predictor + 11 scatter sums, serial), mode 1
+ * the split gather (OpenMP).  With start_step beyond the largest delay plus
+ * the order, no tap is skipped.  *taps: the (pair, tap, direction) terms the
+ * timed evaluations added.  Returns a checksum (NAN on failure). */
+double qo_time_steps(const qo_params *p, int n, const double *pos, const double *dip,
+                     const double *freq, const double *t1, const double *t2,
+                     const qo_pulse *pulse, const double *ps, const double *cs,
+                     double future_coef, int mode, int start_step, int n_steps,
+                     double *setup_seconds, double *step_seconds, int64_t *taps);
 
 /* Number of OpenMP threads the oracle uses (1 = the reference's serial run). */
 void qo_set_threads(int n);

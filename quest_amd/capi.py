@@ -39,7 +39,7 @@ EXPORTS = [
     "quest_run", "quest_current_step", "quest_bad_step", "quest_get_state",
     "quest_pair_delays", "quest_set_timing", "quest_pair_kernel_time", "quest_ctx_info",
     "quest_sync", "quest_batch_create", "quest_set_pulses", "quest_autotune",
-    "quest_set_pulse_list", "quest_set_pair_method",
+    "quest_set_pulse_list", "quest_set_pair_method", "quest_pair_delays_range",
 ]
 PAIR_ON_THE_FLY, PAIR_TABLE = 0, 1  # QUEST_PAIR_*
 CLAIM_AUTO, CLAIM_STATIC, CLAIM_DYNAMIC = 0, 1, 2  # QUEST_CLAIM_*
@@ -101,6 +101,7 @@ def load():
     L.quest_bad_step.argtypes = [vp, ip]
     L.quest_get_state.argtypes = [vp, dp, dp]
     L.quest_pair_delays.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.quest_pair_delays_range.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int32)]
     L.quest_set_timing.argtypes = [vp, C.c_int]
     L.quest_pair_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int]
     L.quest_ctx_info.argtypes = [vp, C.POINTER(C.c_int64)]
@@ -221,6 +222,8 @@ class Context:
 
     def set_pulses(self, pulses):
         """One pulse per system of a batch."""
+        if len(pulses) != len(self.sizes):
+            raise QuestError(f"set_pulses: {len(pulses)} pulses for {len(self.sizes)} systems")
         arr = (Pulse * len(pulses))(*[_pulse_struct(pl) for pl in pulses])
         self._check(self._L.quest_set_pulses(self.ptr, arr))
 
@@ -248,6 +251,8 @@ class Context:
 
     def hist_evaluate(self, step: int, rho01) -> np.ndarray:
         r = np.ascontiguousarray(rho01, dtype=np.complex128)
+        if r.shape != (self.n,):
+            raise QuestError(f"hist_evaluate: rho01 has shape {r.shape}, expected ({self.n},)")
         out = np.zeros(self.n, dtype=np.complex128)
         self._check(self._L.quest_hist_evaluate(self.ptr, step, _dp(r.view(np.float64)),
                                                 _dp(out.view(np.float64))))
@@ -275,10 +280,21 @@ class Context:
         self._check(self._L.quest_current_step(self.ptr, C.byref(s)))
         return s.value
 
-    def pair_delays(self) -> np.ndarray:
-        out = np.zeros(max(self.n * (self.n - 1) // 2, 1), dtype=np.int32)
-        self._check(self._L.quest_pair_delays(self.ptr, out.ctypes.data_as(C.POINTER(C.c_int32))))
-        return out[: self.n * (self.n - 1) // 2]
+    def pair_delays(self, first: int | None = None, count: int | None = None) -> np.ndarray:
+        """Integer delays of the unordered pairs in the reference's order, from
+        the delay split of this context's precision; all pairs, or
+        [first, first + count)."""
+        total = self.n * (self.n - 1) // 2
+        if first is None and count is None:
+            out = np.zeros(max(total, 1), dtype=np.int32)
+            self._check(self._L.quest_pair_delays(self.ptr, out.ctypes.data_as(C.POINTER(C.c_int32))))
+            return out[:total]
+        first = 0 if first is None else int(first)
+        count = total - first if count is None else int(count)
+        out = np.zeros(max(count, 1), dtype=np.int32)
+        self._check(self._L.quest_pair_delays_range(self.ptr, first, count,
+                                                    out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out[:max(count, 0)]
 
     def set_timing(self, enable: bool) -> None:
         self._check(self._L.quest_set_timing(self.ptr, int(bool(enable))))

@@ -36,11 +36,9 @@ struct quest_ctx {
   size_t ring_bytes = 0;
   int dmax_bound = 0;
   int near_total = 0;
-  int phase_len = 0;
   int mode = 0;           // 0 unset, 1 per-call, 2 whole-run
   int cur_step = 0;       // whole-run: next step; per-call: step of the current values (-1 = none)
   int bad = -1;
-  bool pc_uploaded = false;
   std::vector<double> pcw;  // ps[2W], cs[2W]
   double fc = 0;
   std::vector<PulseParams> pulses;
@@ -340,24 +338,27 @@ int default_chunk(const quest_ctx *c)
   return len;
 }
 
-int ensure_phase(quest_ctx *c, int upto)
+// The three factor tables of e^{i theta s} (phase_at on the device): each
+// entry from a long-double argument reduction, rounded once to FP64.  Fixed
+// size, uploaded once at create: no later call allocates for the phase.
+int upload_phase(quest_ctx *c)
 {
-  if(upto <= c->phase_len) return QUEST_OK;
-  int len = std::max(1024, next_pow2(upto));
-  std::vector<double2> h(len);
+  std::vector<double2> h(kPhaseEntries);
   const long double theta = static_cast<long double>(c->k.omega) * static_cast<long double>(c->k.dt);
   const long double two_pi = 6.283185307179586476925286766559005768L;
-  for(int s = 0; s < len; ++s) {
-    const long double a = fmodl(theta * s, two_pi);
-    h[s] = make_double2(static_cast<double>(cosl(a)), static_cast<double>(sinl(a)));
+  auto put = [&](int at, long double mult) {
+    const long double a = fmodl(theta * mult, two_pi);
+    h[at] = make_double2(static_cast<double>(cosl(a)), static_cast<double>(sinl(a)));
+  };
+  for(int j = 0; j < kPhaseLevel; ++j) {
+    put(j, static_cast<long double>(j));
+    put(kPhaseLevel + j, static_cast<long double>(j) * kPhaseLevel);
   }
-  drop_graph(c);  // the graph's kernels hold the old table pointer
-  if(c->d.phase) HIPC(c, hipFree(c->d.phase));
-  c->d.phase = nullptr;
-  HIPC(c, hipMalloc(&c->d.phase, sizeof(double2) * len));
-  HIPC(c, hipMemcpyAsync(c->d.phase, h.data(), sizeof(double2) * len, hipMemcpyHostToDevice, c->stream));
+  for(int j = 0; j < 2 * kPhaseLevel; ++j)
+    put(2 * kPhaseLevel + j, static_cast<long double>(j) * kPhaseLevel * kPhaseLevel);
+  HIPC(c, hipMemcpyAsync(c->d.phase, h.data(), sizeof(double2) * kPhaseEntries, hipMemcpyHostToDevice,
+                         c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  c->phase_len = len;
   return QUEST_OK;
 }
 
@@ -369,19 +370,14 @@ int start_state(quest_ctx *c)
   return QUEST_OK;
 }
 
-int ensure_pc(quest_ctx *c)
+// The predictor-corrector weights on the device: main.cpp:49's (18, window,
+// 3.15) from create on, or the caller's from quest_set_pc_weights.
+int upload_pc(quest_ctx *c)
 {
-  if(c->pc_uploaded) return QUEST_OK;
   const int W = c->P.history_window;
-  if(c->pcw.empty()) {
-    c->pcw.assign(4 * W, 0.0);
-    if(pc_weights(18, W, 3.15, c->pcw.data(), c->pcw.data() + 2 * W, &c->fc) != 0)
-      return fail(c, QUEST_ERR_ARG, "pc weight solve failed");
-  }
   HIPC(c, hipMemcpyAsync(c->d.pcw, c->pcw.data(), sizeof(double) * 4 * W, hipMemcpyHostToDevice,
                          c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  c->pc_uploaded = true;
   drop_graph(c);  // future_coef is a kernel argument of the graph
   return QUEST_OK;
 }
@@ -416,39 +412,44 @@ int collect_timing(quest_ctx *c)
   return QUEST_OK;
 }
 
-// The pair kernel, bracketed by HIP events on the context's stream when timing.
-int pair_sum(quest_ctx *c)
+// The pair kernel, bracketed by HIP events on the context's stream when
+// timing (`timed`).  `d` is the context's device state, or a copy of it whose
+// partial sums live elsewhere.
+int pair_sum(quest_ctx *c, DeviceState &d, bool timed)
 {
   if(c->k.n_work == 0) return QUEST_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if(c->timing) {
+  timed = timed && c->timing;
+  if(timed) {
     e0 = pool_event(c);
     e1 = pool_event(c);
     if(!e0 || !e1) return fail(c, QUEST_ERR_HIP, "hipEventCreate failed");
     HIPC(c, hipEventRecord(e0, c->stream));
   }
-  HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));
-  if(c->timing) HIPC(c, hipEventRecord(e1, c->stream));
+  HIPC(c, launch_pair_past(c->k, d, c->P.precision, c->stream));
+  if(timed) HIPC(c, hipEventRecord(e1, c->stream));
   return QUEST_OK;
 }
+int pair_sum(quest_ctx *c) { return pair_sum(c, c->d, true); }
 
 // One whole step: pair sum, predictor, 10 x (evaluate, correct), final
 // evaluation and commit, step counter advance.
-int enqueue_step(quest_ctx *c)
+int enqueue_step(quest_ctx *c, DeviceState &d, bool timed)
 {
-  int rc = pair_sum(c);
+  int rc = pair_sum(c, d, timed);
   if(rc != QUEST_OK) return rc;
   if(c->step_block > 0) {
-    HIPC(c, launch_pc_step(c->k, c->d, c->P.precision, c->fc, c->P.history_window, c->step_block,
+    HIPC(c, launch_pc_step(c->k, d, c->P.precision, c->fc, c->P.history_window, c->step_block,
                            c->step_blocks, c->step_sync_blocks, c->stream));
     return QUEST_OK;  // the step kernel advanced the step counter
   }
-  HIPC(c, launch_pc_begin(c->k, c->d, c->P.history_window, c->stream));
+  HIPC(c, launch_pc_begin(c->k, d, c->P.history_window, c->stream));
   for(int it = 0; it <= kNumCorrectorSteps; ++it)
-    HIPC(c, launch_pc_iter(c->k, c->d, c->P.precision, c->fc, it, c->stream));
-  HIPC(c, launch_advance_step(c->d, c->stream));
+    HIPC(c, launch_pc_iter(c->k, d, c->P.precision, c->fc, it, c->stream));
+  HIPC(c, launch_advance_step(d, c->stream));
   return QUEST_OK;
 }
+int enqueue_step(quest_ctx *c) { return enqueue_step(c, c->d, true); }
 
 // One step captured as a graph (kernel launches only; no memsets), replayed
 // for every step: the kernels read the step number from the device counter.
@@ -820,8 +821,17 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, hipStreamSynchronize(c->stream));
   rc = pack_step_blocks(c, start, near_src);
   if(rc != QUEST_OK) return rc;
-  rc = ensure_phase(c, 1024);
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.phase), sizeof(double2) * kPhaseEntries));
+  rc = upload_phase(c);
   if(rc != QUEST_OK) return rc;
+  {
+    const int W = p->history_window;
+    c->pcw.assign(4 * W, 0.0);
+    if(pc_weights(18, W, 3.15, c->pcw.data(), c->pcw.data() + 2 * W, &c->fc) != 0)
+      return fail(c, QUEST_ERR_ARG, "pc weight solve failed");
+    rc = upload_pc(c);
+    if(rc != QUEST_OK) return rc;
+  }
   return start_state(c);
 }
 
@@ -831,8 +841,8 @@ int check_params(const quest_params *p, int n)
   if(n < 1) return fail(nullptr, QUEST_ERR_ARG, "n_dots must be >= 1");
   if(p->interp_order < 1 || p->interp_order > kMaxOrder)
     return fail(nullptr, QUEST_ERR_ARG, "interp_order must be in 1..6");
-  if(p->history_window < 1 || p->history_window > kMaxWindow)
-    return fail(nullptr, QUEST_ERR_ARG, "history_window must be in 1..64");
+  if(p->history_window < 2 || p->history_window > kMaxWindow)
+    return fail(nullptr, QUEST_ERR_ARG, "history_window must be in 2..64");
   if(p->precision != QUEST_PREC_FP64 && p->precision != QUEST_PREC_MIXED)
     return fail(nullptr, QUEST_ERR_ARG, "unknown precision");
   if(p->dyadic != QUEST_DYADIC_ROTATING && p->dyadic != QUEST_DYADIC_LAB)
@@ -963,6 +973,39 @@ int create_checked(const quest_params *params, int n_sys, const int *sizes, cons
   return QUEST_OK;
 }
 
+// Entry checks of quest_run; the first call starts the
+// whole-run sequence at step 0.
+int run_begin(quest_ctx *c, int n_steps)
+{
+  if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
+  if(n_steps < 0) return fail(c, QUEST_ERR_ARG, "n_steps must be >= 0");
+  if(c->mode == 1) return fail(c, QUEST_ERR_STATE, "context is driven by quest_hist_evaluate");
+  HIPC(c, hipSetDevice(c->P.device));
+  if(c->mode == 0) {
+    c->mode = 2;
+    c->cur_step = 0;
+    HIPC(c, launch_set_step(c->d, 0, c->stream));
+  }
+  if(c->bad >= 0) return fail(c, QUEST_ERR_DOMAIN, "unbounded history values at or before step " + std::to_string(c->bad));
+  if(n_steps > INT_MAX - 1 - c->cur_step) return fail(c, QUEST_ERR_ARG, "steps beyond 2^31 - 1");
+  return QUEST_OK;
+}
+
+// After the steps are queued: waits for them, advances the host step and
+// reports the first non-finite step (integrator.cpp:187-207).
+int run_end(quest_ctx *c, int n_steps)
+{
+  int bad = INT_MAX;
+  HIPC(c, hipMemcpyAsync(&bad, c->d.bad_step, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  c->cur_step += n_steps;
+  if(bad != INT_MAX) {
+    c->bad = bad;
+    return fail(c, QUEST_ERR_DOMAIN, "unbounded history values at or before step " + std::to_string(bad));
+  }
+  return QUEST_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1008,16 +1051,16 @@ int quest_hist_evaluate(quest_ctx *c, int step, const double *rho01, double *out
     c->mode = 1;
     c->cur_step = -1;
   }
+  if(c->cur_step < 0 && step != 0)
+    return fail(c, QUEST_ERR_STATE, "the first evaluation must be at step 0 (every step is evaluated in order)");
   if(c->cur_step >= 0 && step < c->cur_step)
     return fail(c, QUEST_ERR_STATE, "step went backwards");
   if(c->cur_step >= 0 && step > c->cur_step + 1)
     return fail(c, QUEST_ERR_STATE, "step skipped: every step must be evaluated");
-  int rc = ensure_phase(c, step + 2);
-  if(rc != QUEST_OK) return rc;
   if(step != c->cur_step) {
     if(c->cur_step >= 0) HIPC(c, launch_hist_commit(c->k, c->d, c->P.precision, c->cur_step, c->stream));
     HIPC(c, launch_set_step(c->d, step, c->stream));
-    rc = pair_sum(c);
+    const int rc = pair_sum(c);
     if(rc != QUEST_OK) return rc;
     HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
     c->cur_step = step;
@@ -1126,25 +1169,13 @@ int quest_set_pc_weights(quest_ctx *c, int n_time, const double *ps, const doubl
   c->pcw.assign(ps, ps + 2 * W);
   c->pcw.insert(c->pcw.end(), cs, cs + 2 * W);
   c->fc = future_coef;
-  c->pc_uploaded = false;
-  return QUEST_OK;
+  HIPC(c, hipSetDevice(c->P.device));
+  return upload_pc(c);
 }
 
 int quest_run(quest_ctx *c, int n_steps)
 {
-  if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
-  if(n_steps < 0) return fail(c, QUEST_ERR_ARG, "n_steps must be >= 0");
-  if(c->mode == 1) return fail(c, QUEST_ERR_STATE, "context is driven by quest_hist_evaluate");
-  HIPC(c, hipSetDevice(c->P.device));
-  if(c->mode == 0) {
-    c->mode = 2;
-    c->cur_step = 0;
-    HIPC(c, launch_set_step(c->d, 0, c->stream));
-  }
-  if(c->bad >= 0) return fail(c, QUEST_ERR_DOMAIN, "unbounded history values at or before step " + std::to_string(c->bad));
-  int rc = ensure_pc(c);
-  if(rc != QUEST_OK) return rc;
-  rc = ensure_phase(c, c->cur_step + n_steps + 1);
+  int rc = run_begin(c, n_steps);
   if(rc != QUEST_OK) return rc;
   if(c->timing) {
     // kernel by kernel, so that HIP events can bracket every pair-kernel launch
@@ -1157,20 +1188,14 @@ int quest_run(quest_ctx *c, int n_steps)
     if(rc != QUEST_OK) return rc;
     for(int i = 0; i < n_steps; ++i) HIPC(c, hipGraphLaunch(c->graph_exec, c->stream));
   }
-  int bad = INT_MAX;
-  HIPC(c, hipMemcpyAsync(&bad, c->d.bad_step, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
-  c->cur_step += n_steps;
   if(c->timing) {
+    HIPC(c, hipStreamSynchronize(c->stream));
     rc = collect_timing(c);
     if(rc != QUEST_OK) return rc;
   }
-  if(bad != INT_MAX) {
-    c->bad = bad;
-    return fail(c, QUEST_ERR_DOMAIN, "unbounded history values at or before step " + std::to_string(bad));
-  }
-  return QUEST_OK;
+  return run_end(c, n_steps);
 }
+
 
 int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
 {
@@ -1296,21 +1321,24 @@ int quest_get_state(quest_ctx *c, double *rho, double *drho)
   return QUEST_OK;
 }
 
-int quest_pair_delays(quest_ctx *c, int32_t *delays)
+int quest_pair_delays_range(quest_ctx *c, int64_t first, int64_t count, int32_t *delays)
 {
   if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
+  const int64_t total = static_cast<int64_t>(c->n) * (c->n - 1) / 2;
+  if(first < 0 || count < 0 || first > total || count > total - first)
+    return fail(c, QUEST_ERR_ARG, "pair range [" + std::to_string(first) + ", +" + std::to_string(count) +
+                                      ") outside [0, " + std::to_string(total) + ")");
+  if(count == 0) return QUEST_OK;
   if(!delays) return fail(c, QUEST_ERR_ARG, "delays is required");
   HIPC(c, hipSetDevice(c->P.device));
-  const int64_t total = static_cast<int64_t>(c->n) * (c->n - 1) / 2;
-  if(total == 0) return QUEST_OK;
-  const int64_t chunk = std::min<int64_t>(total, int64_t(1) << 26);
+  const int64_t chunk = std::min<int64_t>(count, int64_t(1) << 26);
   int32_t *buf = nullptr;
   HIPC(c, hipMalloc(&buf, sizeof(int32_t) * chunk));
-  for(int64_t first = 0; first < total; first += chunk) {
-    const int64_t cnt = std::min(chunk, total - first);
-    hipError_t e = launch_pair_delays(c->k, c->d, buf, first, cnt, c->stream);
+  for(int64_t done = 0; done < count; done += chunk) {
+    const int64_t cnt = std::min(chunk, count - done);
+    hipError_t e = launch_pair_delays(c->k, c->d, c->P.precision, buf, first + done, cnt, c->stream);
     if(e == hipSuccess)
-      e = hipMemcpyAsync(delays + first, buf, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost, c->stream);
+      e = hipMemcpyAsync(delays + done, buf, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost, c->stream);
     if(e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if(e != hipSuccess) {
       (void)hipFree(buf);
@@ -1319,6 +1347,12 @@ int quest_pair_delays(quest_ctx *c, int32_t *delays)
   }
   HIPC(c, hipFree(buf));
   return QUEST_OK;
+}
+
+int quest_pair_delays(quest_ctx *c, int32_t *delays)
+{
+  if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
+  return quest_pair_delays_range(c, 0, static_cast<int64_t>(c->n) * (c->n - 1) / 2, delays);
 }
 
 int quest_set_timing(quest_ctx *c, int enable)

@@ -17,6 +17,10 @@ constexpr int kMaxWindow = 64;     // PC history window limit
 constexpr int kBlock = 256;        // threads per block of the per-dot and pair kernels
 constexpr int kWave = 64;          // wavefront size; a pair-kernel work item is one wave of targets
 constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
+// e^{i theta s} tables: 2^10 entries each for the low and middle ten bits of
+// the step, 2^11 for the high bits (steps below 2^31)
+constexpr int kPhaseLevel = 1024;
+constexpr int kPhaseEntries = 4 * kPhaseLevel;
 
 int pc_weights(int n_lambda, int n_time, double radius, double *ps, double *cs,
                double *future_coef);
@@ -99,7 +103,8 @@ struct DeviceState {
   double2 *yc1 = nullptr;              // corrector base [n] (rho01)
   double2 *field = nullptr;            // fixed part of the field at this step [n]
   double2 *partial = nullptr;          // [max_chunks][n]
-  double2 *phase = nullptr;            // e^{i theta s}, s in [0, phase_len)
+  double2 *phase = nullptr;            // [kPhaseEntries]: e^{i theta l}, l < 2^10; e^{i theta 2^10 m},
+                                       // m < 2^10; e^{i theta 2^20 h}, h < 2^11 (phase_at)
   double2 *out = nullptr;              // [n] staging of per-call results
   double2 *io = nullptr;               // [2n] per-call mode: rho01 in caller order, then results in it
   int *near_start = nullptr;           // [n + 1]
@@ -168,7 +173,7 @@ int pair_blocks_per_cu(const Consts &k, int precision);
 hipError_t launch_near_count(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s);
-hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int32_t *out, int64_t first,
+hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int precision, int32_t *out, int64_t first,
                               int64_t count, hipStream_t s);
 
 }  // namespace quest

@@ -21,6 +21,7 @@
 #include "quest_internal.hpp"
 
 namespace quest {
+
 namespace {
 
 // ------------------------------------------------------------------ helpers
@@ -625,11 +626,23 @@ __global__ __launch_bounds__(kBlock) void pair_table_f64(
 
 // ------------------------------------------------------- per-dot kernels
 
+// e^{i theta s} for 0 <= s < 2^31 from the three factor tables of
+// DeviceState::phase (s = 2^20 h + 2^10 m + l): e^{i theta 2^20 h} e^{i theta 2^10 m}
+// e^{i theta l}, each entry rounded once from a long-double argument reduction
+// on the host, so the product is within a few FP64 units of e^{i theta s}.
+__device__ __forceinline__ double2 phase_at(const double2 *phase, int s)
+{
+  const double2 hi = phase[2 * kPhaseLevel + (s >> 20)];
+  const double2 mid = phase[kPhaseLevel + ((s >> 10) & (kPhaseLevel - 1))];
+  const double2 lo = phase[s & (kPhaseLevel - 1)];
+  return cmul(cmul(hi, mid), lo);
+}
+
 // Field scale of the pair sum at `step`: -K e^{-i theta step} (rotating) or K (lab).
 __device__ __forceinline__ double2 hist_scale(const Consts &k, const double2 *phase, int step)
 {
   if(k.dyadic == QUEST_DYADIC_LAB) return make_double2(k.k_green, 0.0);
-  const double2 ph = phase[step];
+  const double2 ph = phase_at(phase, step);
   return make_double2(-k.k_green * ph.x, k.k_green * ph.y);
 }
 
@@ -787,7 +800,8 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
   constexpr bool f32 = sizeof(R2) == sizeof(float2);
   const double *basis = f32 ? k.vinv_c : k.vinv;
   const double2 ph = lab ? make_double2(1.0, 0.0)
-                         : (f32 ? cmul(phase[s], make_double2(k.half_phase_re, k.half_phase_im)) : phase[s]);
+                         : (f32 ? cmul(phase_at(phase, s), make_double2(k.half_phase_re, k.half_phase_im))
+                                : phase_at(phase, s));
   const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
   const size_t e = static_cast<size_t>(a) * k.ring_slots + (s & k.ring_mask);
   for(int kk = 0; kk <= k.order; ++kk) {
@@ -1235,11 +1249,15 @@ __global__ __launch_bounds__(kBlock) void table_build_kernel(Consts k, DeviceSta
   }
 }
 
-// Diagnostic: FP64 integer delays of unordered pairs [first, first + count) in
-// the reference's lower-triangle order (history_interaction.cpp:80-86).
-__global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_t first,
-                                   int64_t count)
+// Diagnostic: integer delays of unordered pairs [first, first + count) in the
+// reference's lower-triangle order (history_interaction.cpp:80-86; row = the
+// pair's source, col < row its observer), from the delay split the pair
+// kernel of the precision uses: split_f64 (F64) or split_mixed, the FP32
+// estimate with the FP64 split where it lies near an integer.
+template <bool F64>
+__global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_t first, int64_t count)
 {
+  const PairK<float> K(k);
   for(int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < count;
       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t idx = first + i;
@@ -1248,9 +1266,17 @@ __global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_
     while((row + 1) * row / 2 <= idx) ++row;
     const int64_t col = idx - row * (row - 1) / 2;
     const int rs = d.perm_dev[row], cs = d.perm_dev[col];
-    const double4 pr = d.pos64[rs], pc = d.pos64[cs];
-    double x, ir, r2;
-    out[i] = split_f64(pc.x - pr.x, pc.y - pr.y, pc.z - pr.z, k.cdt, k.icdt, x, ir, r2);
+    if constexpr(F64) {
+      const double4 pr = d.pos64[rs], pc = d.pos64[cs];
+      double x, ir, r2;
+      out[i] = split_f64(pc.x - pr.x, pc.y - pr.y, pc.z - pr.z, k.cdt, k.icdt, x, ir, r2);
+    } else {
+      // observer cs as the target a, source rs as b, as in the pair kernel
+      const float4 pa = d.pd32[2 * cs], pb = d.pd32[2 * rs];
+      float x, ir;
+      out[i] = kRoundMagicBits - split_mixed(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, K.eps_rel, K.lim0,
+                                             kRoundMagic, d.pos64[cs], d.pos64, rs, K.cdt, K.icdt64, x, ir);
+    }
   }
 }
 
@@ -1521,12 +1547,16 @@ hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hip
   return hipGetLastError();
 }
 
-hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int32_t *out, int64_t first,
+hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int precision, int32_t *out, int64_t first,
                               int64_t count, hipStream_t s)
 {
   const int blocks = static_cast<int>(std::min<int64_t>((count + kBlock - 1) / kBlock, 16384));
-  hipLaunchKernelGGL(pair_delays_kernel, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s, k, d,
-                     out, first, count);
+  if(precision == QUEST_PREC_FP64)
+    hipLaunchKernelGGL(pair_delays_kernel<true>, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s, k, d, out,
+                       first, count);
+  else
+    hipLaunchKernelGGL(pair_delays_kernel<false>, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s, k, d, out,
+                       first, count);
   return hipGetLastError();
 }
 

@@ -1,6 +1,17 @@
 """Writes the oracle fixtures the GPU parity tests compare against.
 
-    python tests/golden/make_golden.py [n4096] [n131072]
+    python tests/golden/make_golden.py [n4096] [n131072] [n64] [ensemble]
+
+n64_traj.npz: the N = 64 system (quest_amd.systems.workload("n64"): 0.5 um
+cube, dipoles along z, pi/2 pulse at 0.5 ps, dt = 1e-4 ps, order 4) over all
+20000 steps by the FP64 oracle's reference method (mode 0: the pair table and
+11 scatter sums per step, history_interaction.cpp:47-70, serial), every 2000
+steps.  About 16 s on one core.
+
+ensemble_members.npz: three systems of the 256-system ensemble
+(quest_amd.systems.ensemble(): the smallest, the median and the largest N),
+each with its own pulse, 5000 steps by the FP64 oracle (split-gather mode),
+every 1000 steps.
 
 n4096_traj.npz: the N = 4096 system (quest_amd.systems.workload("n4096"):
 512 um^-3, dipoles along z, pi/2 pulse, dt = 1e-4 ps, order 4) integrated for
@@ -49,6 +60,68 @@ def make_n4096(path):
     print(f"wrote {path} in {time.time() - t:.0f} s; max|rho01| {np.abs(r['y'][:, 1]).max():.4f}")
 
 
+def _weights_and_params():
+    import oracle
+    import pc_weights
+    from quest_amd import systems
+    c = systems.consts()
+    ps, cs, fc = pc_weights.weights(18, 22, 3.15)
+    p = oracle.params(c["c0"], c["hbar"], c["mu0"], c["laser_omega"], c["dt"], 4, 22)
+    return p, ps, cs, fc
+
+
+def _qo_pulse(pl):
+    import oracle
+    return oracle.pulse(pl["amplitude"], pl["delay"], pl["width"], pl["freq"], pl["wavevector"],
+                        pl["polarization"])
+
+
+def make_n64(path):
+    import oracle
+    from quest_amd import systems
+    oracle.build()
+    oracle.set_threads(1)
+    sysd, steps = systems.workload("n64")
+    p, ps, cs, fc = _weights_and_params()
+    t = time.time()
+    r = oracle.solve(p, sysd, _qo_pulse(systems.pulse(math.pi / 2)), ps, cs, fc, steps, mode=0,
+                     record_every=2000)
+    assert r["status"] == 0
+    np.savez_compressed(path, rec=r["rec"], y=r["y"], yp=r["yp"], steps=steps, record_every=2000,
+                        ps=ps, cs=cs, fc=fc)
+    print(f"wrote {path} in {time.time() - t:.0f} s; max|rho01| {np.abs(r['rec'][:, :, 1]).max():.4f}")
+
+
+def ensemble_member_index():
+    """Indices of the smallest, the median and the largest system of
+    quest_amd.systems.ensemble() (ties: the first)."""
+    from quest_amd import systems
+    ns = np.array([s["pos"].shape[0] for s, _ in systems.ensemble()])
+    order = np.argsort(ns, kind="stable")
+    return np.array([order[0], order[len(ns) // 2], order[-1]])
+
+
+def make_ensemble_members(path):
+    import oracle
+    from quest_amd import systems
+    oracle.build()
+    oracle.set_threads(os.cpu_count() or 1)
+    ens = systems.ensemble()
+    idx = ensemble_member_index()
+    p, ps, cs, fc = _weights_and_params()
+    out = {"index": idx, "steps": 5000, "record_every": 1000, "ps": ps, "cs": cs, "fc": fc}
+    t = time.time()
+    for j, i in enumerate(idx):
+        sysd, pl = ens[i]
+        r = oracle.solve(p, sysd, _qo_pulse(pl), ps, cs, fc, 5000, mode=1, record_every=1000)
+        assert r["status"] == 0
+        out[f"rec{j}"] = r["rec"]
+        out[f"yp{j}"] = r["yp"]
+        print(f"system {i}: n={sysd['pos'].shape[0]} max|rho01| {np.abs(r['rec'][:, :, 1]).max():.4f}")
+    np.savez_compressed(path, **out)
+    print(f"wrote {path} in {time.time() - t:.0f} s")
+
+
 def make_n131072_observers(path):
     import oracle
     from quest_amd import systems
@@ -73,3 +146,7 @@ if __name__ == "__main__":
         make_n4096(os.path.join(HERE, "n4096_traj.npz"))
     if "n131072" in which:
         make_n131072_observers(os.path.join(HERE, "n131072_observers.npz"))
+    if "n64" in which:
+        make_n64(os.path.join(HERE, "n64_traj.npz"))
+    if "ensemble" in which:
+        make_ensemble_members(os.path.join(HERE, "ensemble_members.npz"))

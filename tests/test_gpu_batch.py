@@ -135,20 +135,6 @@ def test_autotune_changes_no_state(ref_weights):
     assert rel_err(ra, rb) < 1e-12
 
 
-def test_ensemble_generator_and_batch_smoke(ref_weights):
-    """The 256-system ensemble of the batched workload builds and steps."""
-    from quest_amd import Context, systems
-    ens = systems.ensemble(count=256)
-    ns = np.array([len(s["pos"]) for s, _ in ens])
-    assert ns.min() >= 200 and ns.max() <= 2000
-    with Context([s for s, _ in ens], systems.consts(), precision=1, pulse=[p for _, p in ens]) as ctx:
-        ctx.set_pc_weights(*ref_weights)
-        ctx.run(5)
-        rho, _ = ctx.state()
-        assert np.all(np.isfinite(rho))
-        assert ctx.info()["systems"] == 256
-
-
 @pytest.mark.parametrize("precision", [0, 1])
 def test_dynamic_item_claiming_is_bitwise_static(ref_weights, precision):
     """Waves that claim items from the device counter compute the same items

@@ -450,3 +450,34 @@ def test_rho00_stays_real(oracle_lib, ref_weights):
     assert np.abs(r["y"][:, 1]).max() > 0.1
     assert np.all(r["rec"][:, :, 0].imag == 0)
     assert np.all(r["y"][:, 0].imag == 0) and np.all(r["yp"][:, 0].imag == 0)
+
+
+def test_timed_steps_at_steady_state_sum_every_tap(oracle_lib, ref_weights):
+    """The CPU baseline's timed steps (qo_time_steps) start past the largest
+    delay plus the order on a non-zero synthetic prehistory, so the reference
+    method's skip of taps before the prehistory (history_interaction.cpp:57)
+    never applies: 11 evaluations x N(N-1)/2 pairs x (M+1) taps x 2
+    directions per step.  From the ground state at step 0 most taps are
+    skipped; the split gather adds each far pair's taps once per step."""
+    import math
+    from quest_amd import systems
+    ps, cs, fc = ref_weights
+    # 64 dots in a 2 um cube: delays up to ~115 steps, as at N = 4096
+    sysd = dense_system(64, 2.0, seed=3)
+    c = systems.consts()
+    p = oracle_params(oracle_lib, c)
+    pl = oracle_pulse(oracle_lib, systems.pulse(math.pi / 2))
+    n, M, K = 64, 4, 3
+    start = oracle_lib.steady_start(sysd["pos"], c["c0"], c["dt"], M)
+    pairs = n * (n - 1) // 2
+    full = 11 * K * pairs * (M + 1) * 2
+    r = oracle_lib.time_steps(p, sysd, pl, ps, cs, fc, K, start_step=start, mode=0)
+    assert r["taps"] == full and math.isfinite(r["checksum"])
+    cold = oracle_lib.time_steps(p, sysd, pl, ps, cs, fc, K, start_step=0, mode=0)
+    assert 0 < cold["taps"] < full / 2
+    delays, _ = oracle_lib.build_table(p, sysd["pos"], sysd["dip"])
+    near = int((delays == 0).sum())
+    split = oracle_lib.time_steps(p, sysd, pl, ps, cs, fc, K, start_step=start, mode=1)
+    assert split["taps"] == K * (pairs - near) * (M + 1) * 2 + 11 * K * near * (M + 1) * 2
+    # the prehistory is what the timed steps integrate from: both modes agree
+    assert abs(split["checksum"] - r["checksum"]) < 1e-12 * abs(r["checksum"])

@@ -213,6 +213,24 @@ int quest_set_pc_weights(quest_ctx *ctx, int n_time, const double *ps,
  * first bad step (integrator.cpp:187-207). */
 int quest_run(quest_ctx *ctx, int n_steps);
 
+/* Allocation-free variant of quest_run.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory once a run is under way
+ * (tight serving loops, harnesses that forbid hipMalloc between steps).  It
+ * calls no hipMalloc, hipFree or other allocating routine: the same kernels as
+ * quest_run are launched one by one (no graph is instantiated), and the pair
+ * kernel's partial sums -- (source chunks + 1) x n x 16 B, e.g. 9 x 65536 x
+ * 16 B = 9.4 MB at the default 8 chunks -- go to a static device array of
+ * QUEST_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends, instead of the context's buffer.  Results are those of
+ * quest_run, bit for bit.  QUEST_ERR_ARG if the partial sums do not fit (at
+ * 8 chunks: above ~233k dots; fewer source chunks via quest_params.
+ * source_chunk shrink them).  Every call on one GPU shares that GPU's array:
+ * calls that overlap in time on one GPU (from several host threads) race and
+ * must be serialised by the caller; a call has finished with the array when
+ * it returns.  The pair kernel is not timed (quest_set_timing). */
+#define QUEST_SCRATCH_BYTES (32u << 20)
+int quest_run_noalloc(quest_ctx *ctx, int n_steps);
+
 /* Step the next quest_run/quest_hist_evaluate call works on. */
 int quest_current_step(const quest_ctx *ctx, int *step);
 

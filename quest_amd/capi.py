@@ -40,6 +40,7 @@ EXPORTS = [
     "quest_pair_delays", "quest_set_timing", "quest_pair_kernel_time", "quest_ctx_info",
     "quest_sync", "quest_batch_create", "quest_set_pulses", "quest_autotune",
     "quest_set_pulse_list", "quest_set_pair_method", "quest_pair_delays_range",
+    "quest_run_noalloc",
 ]
 PAIR_ON_THE_FLY, PAIR_TABLE = 0, 1  # QUEST_PAIR_*
 CLAIM_AUTO, CLAIM_STATIC, CLAIM_DYNAMIC = 0, 1, 2  # QUEST_CLAIM_*
@@ -97,6 +98,7 @@ def load():
     L.quest_pc_weights.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp, dp]
     L.quest_set_pc_weights.argtypes = [vp, C.c_int, dp, dp, C.c_double]
     L.quest_run.argtypes = [vp, C.c_int]
+    L.quest_run_noalloc.argtypes = [vp, C.c_int]
     L.quest_current_step.argtypes = [vp, ip]
     L.quest_bad_step.argtypes = [vp, ip]
     L.quest_get_state.argtypes = [vp, dp, dp]
@@ -265,6 +267,11 @@ class Context:
 
     def run(self, n_steps: int) -> None:
         self._check(self._L.quest_run(self.ptr, int(n_steps)))
+
+    def run_noalloc(self, n_steps: int) -> None:
+        """quest_run_noalloc: the same steps with no device allocation (the
+        pair sums in the library's static per-GPU scratch array)."""
+        self._check(self._L.quest_run_noalloc(self.ptr, int(n_steps)))
 
     def state(self):
         """(rho [n][2], drho [n][2]) complex at the last finished step."""

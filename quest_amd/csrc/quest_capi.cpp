@@ -60,6 +60,7 @@ struct quest_ctx {
   int max_group = 0;           // largest connected group of zero-delay partners
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  double2 *scratch = nullptr;  // this device's static scratch array (quest_run_noalloc)
 };
 
 namespace {
@@ -414,7 +415,7 @@ int collect_timing(quest_ctx *c)
 
 // The pair kernel, bracketed by HIP events on the context's stream when
 // timing (`timed`).  `d` is the context's device state, or a copy of it whose
-// partial sums live elsewhere.
+// partial sums live elsewhere (quest_run_noalloc).
 int pair_sum(quest_ctx *c, DeviceState &d, bool timed)
 {
   if(c->k.n_work == 0) return QUEST_OK;
@@ -586,6 +587,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   const int n = c->sys_begin[n_sys];
   c->n = n;
   HIPC(c, hipSetDevice(p->device));
+  HIPC(c, scratch_address(p->device, &c->scratch));  // once per device, with it current
   HIPC(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPC(c, hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, p->device));
 
@@ -973,7 +975,7 @@ int create_checked(const quest_params *params, int n_sys, const int *sizes, cons
   return QUEST_OK;
 }
 
-// Entry checks of quest_run; the first call starts the
+// Entry checks of quest_run and quest_run_noalloc; the first call starts the
 // whole-run sequence at step 0.
 int run_begin(quest_ctx *c, int n_steps)
 {
@@ -1196,6 +1198,25 @@ int quest_run(quest_ctx *c, int n_steps)
   return run_end(c, n_steps);
 }
 
+int quest_run_noalloc(quest_ctx *c, int n_steps)
+{
+  if(c && !c->scratch) return fail(c, QUEST_ERR_HIP, "the static scratch array has no device address");
+  // the pair kernel's partial rows (source chunks, plus the spare row of empty items)
+  if(c && sizeof(double2) * static_cast<size_t>(c->n) * (c->k.max_chunks + 1) > kScratchBytes)
+    return fail(c, QUEST_ERR_ARG,
+                "quest_run_noalloc: the pair sums need " + std::to_string(c->k.max_chunks + 1) + " x " +
+                    std::to_string(c->n) + " x 16 B, more than the static scratch of " +
+                    std::to_string(kScratchBytes >> 20) + " MiB; use quest_run or fewer source chunks");
+  int rc = run_begin(c, n_steps);
+  if(rc != QUEST_OK) return rc;
+  DeviceState d = c->d;
+  d.partial = c->scratch;
+  for(int i = 0; i < n_steps; ++i) {
+    rc = enqueue_step(c, d, false);
+    if(rc != QUEST_OK) return rc;
+  }
+  return run_end(c, n_steps);
+}
 
 int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
 {

@@ -21,6 +21,9 @@ constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
 // the step, 2^11 for the high bits (steps below 2^31)
 constexpr int kPhaseLevel = 1024;
 constexpr int kPhaseEntries = 4 * kPhaseLevel;
+// static device scratch of quest_run_noalloc (QUEST_SCRATCH_BYTES)
+constexpr size_t kScratchBytes = QUEST_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
 
 int pc_weights(int n_lambda, int n_time, double radius, double *ps, double *cs,
                double *future_coef);
@@ -138,6 +141,10 @@ struct DeviceState {
   // step (groups of partners lie whole in one block), for the LDS exchange
   uint8_t *near_lane = nullptr;
 };
+
+// Device address of the static scratch array on `device`, which must be the
+// current device; looked up once per device (hipGetSymbolAddress) and cached.
+hipError_t scratch_address(int device, double2 **ptr);
 
 // ---------------------------------------------------------------- launchers
 // All launch on `s` and return the hipError_t of the launch.

@@ -17,10 +17,37 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <mutex>
 
 #include "quest_internal.hpp"
 
 namespace quest {
+
+// Partial sums of the pair kernel for quest_run_noalloc: one static array per
+// GPU, loaded with the code object, so that entry point allocates nothing.
+// Host code never passes `scratch` itself to a kernel (on the host the name
+// is only a stand-in for the symbol); it uses the device address
+// scratch_address() looked up per device.  Nothing in it has to start at
+// zero: every step's pair kernel writes every partial row it reads.
+static __device__ __align__(256) double2 scratch[kScratchBytes / sizeof(double2)];
+
+hipError_t scratch_address(int device, double2 **ptr)
+{
+  static std::mutex mu;
+  static double2 *addr[kMaxDevices] = {};
+  *ptr = nullptr;
+  if(device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if(!addr[device]) {
+    // the symbol's address on the current device (the caller made `device` current)
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if(e != hipSuccess) return e;
+    addr[device] = static_cast<double2 *>(p);
+  }
+  *ptr = addr[device];
+  return hipSuccess;
+}
 
 namespace {
 

@@ -218,13 +218,14 @@ int quest_run(quest_ctx *ctx, int n_steps);
  * (tight serving loops, harnesses that forbid hipMalloc between steps).  It
  * calls no hipMalloc, hipFree or other allocating routine: the same kernels as
  * quest_run are launched one by one (no graph is instantiated), and the pair
- * kernel's partial sums -- (source chunks + 1) x n x 16 B, e.g. 9 x 65536 x
- * 16 B = 9.4 MB at the default 8 chunks -- go to a static device array of
- * QUEST_SCRATCH_BYTES that each GPU the library runs on holds until the
- * process ends, instead of the context's buffer.  Results are those of
+ * kernel's partial sums -- (source chunks + 1) x n x 16 B, e.g. 17 x 65536 x
+ * 16 B = 17.8 MB for mixed precision at the default chunk length (4096
+ * sources), 9 x 65536 x 16 B for FP64 (8 chunks) -- go to a static device
+ * array of QUEST_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends, instead of the context's buffer.  Results are those of
  * quest_run, bit for bit.  QUEST_ERR_ARG if the partial sums do not fit (at
- * 8 chunks: above ~233k dots; fewer source chunks via quest_params.
- * source_chunk shrink them).  Every call on one GPU shares that GPU's array:
+ * the default chunk lengths: mixed above ~92k dots, FP64 above ~233k; longer
+ * chunks via quest_params.source_chunk shrink them).  Every call on one GPU shares that GPU's array:
  * calls that overlap in time on one GPU (from several host threads) race and
  * must be serialised by the caller; a call has finished with the array when
  * it returns.  The pair kernel is not timed (quest_set_timing). */

@@ -184,26 +184,24 @@ int count_work(const quest_ctx *c, int chunk_len)
   return static_cast<int>(std::min<int64_t>(items, INT_MAX));
 }
 
-// Work list of the pair kernel for a given source-chunk length: for every
-// system, every target tile of one wave (kWave dots) and every chunk of
-// chunk_len sources.  Items are ordered so that the kBlock / kWave items a
-// block takes together are consecutive wave tiles with the same chunk, and
-// the chunks of a group of tiles consecutive (see pair_past_*): for systems
-// of at least kBlock dots a block is a kBlock-target tile, as with one item
-// per block, while small systems leave at most a wave's worth of lanes idle.
-// Work list of the pair kernel for a chunk length.  Items come in groups of
-// up to four wave tiles of one system (256 targets) with one source chunk:
-// the waves of one block.  One system, or dynamic claiming: tile groups in
-// order, the chunks of a tile group consecutive (with 8 chunks, round-robin
-// block dispatch gives each XCD one source range).  A batch under the static
-// stride (block i's waves take items 4i..4i+3, then 4(i + grid)..): each
-// system is placed on one XCD (systems longest first, each to the XCD with
-// the least pair work), its groups chunk-major on that XCD's blocks (blocks
-// i = x mod 8), so the source rows of a chunk are fetched into one L2 and read
-// there by all its tile groups together; unused slots hold empty items (one
-// target, dot 0, no sources) whose partial row is a spare row past the real
-// ones.  (Spread over all XCDs instead, the 256-system ensemble hit L2 21 % of
-// the time and fetched 3.4 GB per launch.)
+// Work list of the pair kernel for a source-chunk length.  Items are one wave
+// of targets (kWave dots of one system) against one chunk of chunk_len of the
+// system's sources; they come in groups of up to four wave tiles with the same
+// chunk (the waves of one block; for systems of >= 256 dots a 256-target
+// tile).  The partial row an item writes is its chunk's index.
+//   Dynamic claiming: tile groups in order, the chunks of a tile group
+// consecutive; waves claim items in list order.
+//   Static stride (block i's waves take items 4i..4i+3, then 4(i + grid)..,
+// and block i runs on XCD i mod 8 under round-robin dispatch): the unit of
+// placement is a (system, chunk) pair -- all tile groups of the system against
+// that chunk's sources.  Units go to XCDs longest first, each to the XCD with
+// the least work so far (for one system: chunk q to XCD q mod 8), and each
+// XCD's blocks run its units one after another, chunk-major, so that all of
+// an XCD's resident blocks sweep the same source chunk at about the same time
+// and read its ring rows from that XCD's L2 together.  Unused slots hold
+// empty items (one target, dot 0, no sources) whose partial row is a spare row
+// past the real ones.  (A batch spread over all XCDs instead hit L2 21 % of the
+// time and fetched 3.4 GB per launch.)
 int build_work(quest_ctx *c, int chunk_len)
 {
   constexpr int kPer = kBlock / kWave;
@@ -229,7 +227,7 @@ int build_work(quest_ctx *c, int chunk_len)
   };
   const int resident = std::max(kXcds, c->cu_count * c->blocks_per_cu / kXcds * kXcds);
   int grid = 0;
-  if(c->n_sys == 1 || c->k.dyn_items) {
+  if(c->k.dyn_items) {
     for(int s = 0; s < c->n_sys; ++s) {
       const int ns = c->sys_begin[s + 1] - c->sys_begin[s];
       if(ns < 2) continue;
@@ -237,45 +235,47 @@ int build_work(quest_ctx *c, int chunk_len)
       for(int g = 0; g < wtiles; g += kPer)
         for(int q = 0; q < chunks[s]; ++q) emit(work, row, s, g, q);
     }
-    // persistent grid: one full wave of resident blocks (a multiple of 8, see
-    // pair_past_*), at most the blocks the items need, rounded up to 8
+    // persistent grid: one full wave of resident blocks (a multiple of 8), at
+    // most the blocks the items need, rounded up to 8
     const int blocks8 = ((static_cast<int>(work.size()) + kPer - 1) / kPer + kXcds - 1) / kXcds * kXcds;
     grid = std::max(kXcds, std::min(resident, blocks8));
   } else {
-    // systems to XCDs, longest first (pair work ~ wave tiles x sources)
-    std::vector<int> order;
-    std::vector<long long> cost(c->n_sys, 0);
+    // (system, chunk) units to XCDs, longest first (pair work ~ wave tiles x sources)
+    struct Unit {
+      int s, q;
+      long long cost;
+    };
+    std::vector<Unit> units;
     int n_groups = 0;
     for(int s = 0; s < c->n_sys; ++s) {
-      const int ns = c->sys_begin[s + 1] - c->sys_begin[s];
+      const int b = c->sys_begin[s], e = c->sys_begin[s + 1], ns = e - b;
       if(ns < 2) continue;
       const int wtiles = (ns + kWave - 1) / kWave;
-      cost[s] = static_cast<long long>(wtiles) * ns;
       n_groups += (wtiles + kPer - 1) / kPer * chunks[s];
-      order.push_back(s);
+      for(int q = 0; q < chunks[s]; ++q)
+        units.push_back({s, q, static_cast<long long>(wtiles) * (std::min(e, b + (q + 1) * chunk_len) - (b + q * chunk_len))});
     }
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::stable_sort(units.begin(), units.end(), [](const Unit &x, const Unit &y) { return x.cost > y.cost; });
     long long load[kXcds] = {0};
-    std::vector<std::vector<int>> on_xcd(kXcds);
-    for(int s : order) {
+    std::vector<std::vector<Unit>> on_xcd(kXcds);
+    for(const Unit &u : units) {
       const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
-      on_xcd[x].push_back(s);
-      load[x] += cost[s];
+      on_xcd[x].push_back(u);
+      load[x] += u.cost;
     }
     const int blocks8 = (n_groups + kXcds - 1) / kXcds * kXcds;
     grid = std::max(kXcds, std::min(resident, blocks8));
     const int per_xcd = grid / kXcds;  // blocks of one XCD: x, x + 8, x + 16, ...
-    // each XCD's groups, chunk-major per system
+    // each XCD's tile groups, unit after unit (chunk-major)
     struct Group {
       int s, g, q;  // system, first wave tile, chunk
     };
     std::vector<std::vector<Group>> seq(kXcds);
     size_t rounds = 0;
     for(int x = 0; x < kXcds; ++x) {
-      for(int s : on_xcd[x]) {
-        const int wtiles = (c->sys_begin[s + 1] - c->sys_begin[s] + kWave - 1) / kWave;
-        for(int q = 0; q < chunks[s]; ++q)
-          for(int g = 0; g < wtiles; g += kPer) seq[x].push_back({s, g, q});
+      for(const Unit &u : on_xcd[x]) {
+        const int wtiles = (c->sys_begin[u.s + 1] - c->sys_begin[u.s] + kWave - 1) / kWave;
+        for(int g = 0; g < wtiles; g += kPer) seq[x].push_back({u.s, g, u.q});
       }
       rounds = std::max(rounds, (seq[x].size() + per_xcd - 1) / per_xcd);
     }
@@ -328,12 +328,15 @@ int build_work(quest_ctx *c, int chunk_len)
 }
 
 // Default chunk length: 8 chunks of the largest system (one source range per
-// XCD), halved while the work list has fewer items than resident blocks.
+// XCD), for mixed precision at most 4096 sources (measured at N = 65536 and
+// 131072: 16-32 chunks 1-4 % faster than 8; FP64 fastest at 8), halved while
+// the work list has fewer items than resident blocks.
 int default_chunk(const quest_ctx *c)
 {
   int nmax = 2;
   for(int s = 0; s < c->n_sys; ++s) nmax = std::max(nmax, c->sys_begin[s + 1] - c->sys_begin[s]);
   int len = std::max(64, (nmax + 7) / 8);
+  if(c->P.precision == QUEST_PREC_MIXED) len = std::min(len, 4096);
   const int resident = c->cu_count * c->blocks_per_cu * (kBlock / kWave);  // waves
   while(len > 64 && count_work(c, len) < resident) len = std::max(64, (len + 1) / 2);
   return len;
@@ -478,7 +481,7 @@ void free_device(quest_ctx *c)
 {
   drop_graph(c);
   DeviceState &d = c->d;
-  void *ptrs[] = {d.pos64, d.dip64, d.pd32, d.freq, d.t1, d.t2, d.ring, d.y0, d.y1,
+  void *ptrs[] = {d.pos64, d.dip64, d.pd32, d.pd64, d.freq, d.t1, d.t2, d.ring, d.y0, d.y1,
                   d.d0, d.d1, d.cur0, d.cur1, d.yc0, d.yc1, d.field, d.partial, d.phase, d.out,
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.io, d.sys_of, d.sys_range, d.sys_chunks,
@@ -644,40 +647,35 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   k.k_green = p->mu0 / (4 * M_PI * p->hbar);
   k.hbar = p->hbar;
   k.laser_omega = p->laser_omega;
-  k.cc = p->c0 * p->c0;
-  k.oo = p->laser_omega * p->laser_omega;
-  k.oc = p->laser_omega * p->c0;
-  k.m_cdt = -p->c0 / p->dt;
-  k.m_o2dt = -2 * p->laser_omega / p->dt;
   k.two_idt2 = 2 / (p->dt * p->dt);
-  k.f32_scale = k.two_idt2 / cdt;
-  // In these units (lengths in cdt, the term divided by f32_scale) the pair
+  k.unit_scale = k.two_idt2 / cdt;
+  // In these units (lengths in cdt, the term divided by unit_scale) the pair
   // constants are c^2 -> 1/2, w c -> theta/2, c/dt -> 1/2, 2w/dt -> theta and
-  // w^2 -> theta^2/2 (oo_l); the FP32 pair term is written with those values.
-  k.oo_l = k.oo / cdt / k.f32_scale;
+  // w^2 -> theta^2/2 (oo_l); the pair term is written with those values.
+  k.oo_l = 0.5 * k.theta * k.theta;
   {
     // FP32 rotation e^{-i theta xc}, |xc| <= 1/2, u = theta xc, |u| <= U = theta/2:
     // sin u = u - u^3/6 + u^5/120 - ... with u^5 economized over [-U, U]
     // (T5: t^5 ~ (20 t^3 - 5 t)/16, t = u/U), error <= U^5/1920 + U^7/5040;
     // cos u = 1 - u^2/2 + u^4/24, error <= U^6/720.  Both stay below 3e-8 (half
     // an FP32 unit of a unit term) for theta <= 0.28.  The FP64 series
-    // (sincos_small, u = theta x in [0, theta), terms to u^11 and u^12) stays
-    // below 1e-16 for theta <= 0.33.  Beyond these the pair kernels call the
-    // library sincos.
+    // (sincos_centred64, the same u, terms to u^9 and u^10) has errors <=
+    // U^11/11! and U^12/12!, below 1e-16 for theta <= 0.33.  Beyond these the
+    // pair kernels call the library sincos.
     const double t = k.theta, t2 = t * t, U = 0.5 * std::fabs(t), U2 = U * U;
     k.sin_c[0] = t * (1.0 - 5.0 * U2 * U2 / 1920.0);
     k.sin_c[1] = t * t2 * (-1.0 / 6.0 + U2 / 96.0);
     k.cos_c[0] = -t2 / 2.0;
     k.cos_c[1] = t2 * t2 / 24.0;
-    // the FP64 series' coefficients in x: sin: (-1)^k t^(2k+1)/(2k+1)!, k = 5..0;
-    // cos: (-1)^k t^(2k)/(2k)!, k = 6..1
-    for(int j = 0; j < 6; ++j) {
-      const int ks = 5 - j, kc = 6 - j;
+    // the FP64 series' coefficients in x: sin: (-1)^k t^(2k+1)/(2k+1)!, k = 4..0;
+    // cos: (-1)^k t^(2k)/(2k)!, k = 5..1
+    for(int j = 0; j < 5; ++j) {
+      const int ks = 4 - j, kc = 5 - j;
       double a = t, b = 1.0;  // t^(2 ks + 1) / (2 ks + 1)!, t^(2 kc) / (2 kc)!
       for(int i = 2; i <= 2 * ks + 1; ++i) a = a * t / i;
       for(int i = 1; i <= 2 * kc; ++i) b = b * t / i;
       k.trig64[j] = (ks % 2 ? -a : a);
-      k.trig64[6 + j] = (kc % 2 ? -b : b);
+      k.trig64[5 + j] = (kc % 2 ? -b : b);
     }
     k.trig_poly = std::fabs(t) <= (p->precision == QUEST_PREC_FP64 ? 0.33 : 0.28) ? 1 : 0;
   }
@@ -707,6 +705,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   // sorted host copies
   std::vector<double4> p64(n), d64(n);
   std::vector<float4> pd32(2 * static_cast<size_t>(n));  // (position, dipole) per dot
+  std::vector<double4> pd64(p->precision == QUEST_PREC_FP64 ? 2 * static_cast<size_t>(n) : 0);
   std::vector<double> f(n, 0.0), a1(n, 1.0), a2(n, 1.0);
   std::vector<int> sys_of(n);
   std::vector<int2> ranges(n_sys);
@@ -723,6 +722,10 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
                               static_cast<float>((p64[s].z - ctr[2]) / cdt), 0.0f);
     pd32[2 * s + 1] = make_float4(static_cast<float>(d64[s].x), static_cast<float>(d64[s].y),
                                   static_cast<float>(d64[s].z), 0.0f);
+    if(!pd64.empty()) {
+      pd64[2 * s] = make_double4((p64[s].x - ctr[0]) / cdt, (p64[s].y - ctr[1]) / cdt, (p64[s].z - ctr[2]) / cdt, 0.0);
+      pd64[2 * s + 1] = d64[s];
+    }
     if(freq) f[s] = freq[i];
     if(t1) a1[s] = t1[i];
     if(t2) a2[s] = t2[i];
@@ -742,6 +745,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, alloc(reinterpret_cast<void **>(&d.pos64), sizeof(double4) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.dip64), sizeof(double4) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.pd32), sizeof(float4) * 2 * nn));
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.pd64), sizeof(double4) * pd64.size()));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.freq), sizeof(double) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.t1), sizeof(double) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.t2), sizeof(double) * nn));
@@ -777,6 +781,8 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, hipMemcpyAsync(d.pos64, p64.data(), sizeof(double4) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.dip64, d64.data(), sizeof(double4) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.pd32, pd32.data(), sizeof(float4) * 2 * nn, hipMemcpyHostToDevice, c->stream));
+  if(!pd64.empty())
+    HIPC(c, hipMemcpyAsync(d.pd64, pd64.data(), sizeof(double4) * pd64.size(), hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.freq, f.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.t1, a1.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   HIPC(c, hipMemcpyAsync(d.t2, a2.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));

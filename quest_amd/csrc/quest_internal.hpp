@@ -42,21 +42,22 @@ struct Consts {
   double c, dt, cdt, icdt, omega, theta;   // theta = omega dt
   double k_green;     // mu0 / (4 pi hbar)
   double hbar, laser_omega;
-  // combination constants of the pair term, see pair_term()
-  double cc, oo, oc, m_cdt, m_o2dt, two_idt2;
-  // the FP32 pair sum's positions are in units of L = c0 dt and its terms are
-  // divided by f32_scale = two_idt2 / L (applied once per target, in FP64);
-  // there w^2 becomes oo_l = oo / L / f32_scale = theta^2 / 2
-  double oo_l, f32_scale;
+  double two_idt2;    // 2 / dt^2
+  // Both pair sums work in units of L = c0 dt (positions relative to the
+  // centre of all systems' boxes, divided by L, so a distance is the delay
+  // tau) and divide their terms by unit_scale = two_idt2 / L (applied once per
+  // target, in FP64); there w^2 dt^2 / 2 becomes oo_l = theta^2 / 2
+  double oo_l, unit_scale;
   // FP32 sin/cos of theta x, |x| <= 1/2 (sincos_theta): sin_c[0..1] the
   // Chebyshev-economized u - u^3/6 + u^5/120 over |u| <= theta/2 (two terms),
   // cos_c = -theta^2/2, theta^4/24
   double sin_c[2], cos_c[2];
-  // FP64 sin/cos of theta x, x in [0, 1), theta <= 0.33 (sincos_small): Taylor
-  // series in x^2 with theta folded in, highest first: trig64[0..5] =
-  // (-1)^k theta^(2k+1) / (2k+1)!, k = 5..0; trig64[6..11] = (-1)^k theta^(2k) / (2k)!,
-  // k = 6..1 (the constant 1 is the last addend)
-  double trig64[12];
+  // FP64 sin/cos of theta x, |x| <= 1/2, theta <= 0.33 (sincos_centred64):
+  // Taylor series in x^2 with theta folded in, highest first: trig64[0..4] =
+  // (-1)^k theta^(2k+1) / (2k+1)!, k = 4..0; trig64[5..9] = (-1)^k theta^(2k) / (2k)!,
+  // k = 5..1 (the constant 1 is the last addend); truncation below 1e-16 for
+  // |theta x| <= 0.165
+  double trig64[10];
   // 1: the rotation e^{-i theta x} of the pair term uses those short series
   // (theta small enough for rounding-level accuracy); 0: library sincos
   int trig_poly;
@@ -64,9 +65,9 @@ struct Consts {
   double eps_rel, eps_abs;
   // monomial coefficients of the Lagrange basis: vinv[k][m] = [x^k] L_m(x)
   double vinv[(kMaxOrder + 1) * (kMaxOrder + 1)];
-  // the same about x = 1/2 (FP32 ring): vinv_c[j][m] = [(x - 1/2)^j] L_m(x)
+  // the same about x = 1/2 (the ring's basis): vinv_c[j][m] = [(x - 1/2)^j] L_m(x)
   double vinv_c[(kMaxOrder + 1) * (kMaxOrder + 1)];
-  double half_phase_re, half_phase_im;  // e^{-i theta / 2} (FP32 ring, rotating frame)
+  double half_phase_re, half_phase_im;  // e^{-i theta / 2} (the ring's phase, rotating frame)
   int dyadic;         // QUEST_DYADIC_*
   int n_systems;      // independent dot systems (pairs only within a system)
   int n_work;         // pair-kernel work items (system, target tile, source chunk)
@@ -90,6 +91,7 @@ struct PulseParams {
 struct DeviceState {
   double4 *pos64 = nullptr, *dip64 = nullptr;   // (x, y, z, 0)
   float4 *pd32 = nullptr;              // [n][2]: position (pos - centre) / (c0 dt), dipole (FP32)
+  double4 *pd64 = nullptr;             // the same in FP64 (FP64 contexts)
   double *freq = nullptr, *t1 = nullptr, *t2 = nullptr;
   void *ring = nullptr;                // coefficient planes, kstride x [n][ring_slots] complex
   // rho00 is held real: from the real ground state its imaginary part stays an

@@ -126,6 +126,48 @@ __device__ __forceinline__ int split_f64(double dx, double dy, double dz, double
   return static_cast<int>(__builtin_fmax(fl, 0.0));
 }
 
+// Delay split of the FP64 pair sum, whose positions are in units of L = c0 dt
+// (dx, dy, dz: a pair's separation in those units, so its length is the
+// delay tau).  The estimate tau = r2 rsq(r2) (hardware reciprocal square root,
+// two Newton steps; within ~1e-13 of r / (c0 dt) including the rounding of
+// the scaled positions) decides the integer part unless it lies within 1e-9
+// of an integer; then the reference's own split of the um positions
+// (split_f64's exact branch: correctly rounded sqrt of the unfused sum, then
+// the division) decides.  So n is the reference's trunc(r / (c0 dt))
+// (history_interaction.cpp:31-32).  Returns n, the centred fraction
+// xc = x - 1/2 and 1/r in units of 1/L.  r2 >= 0, so the floor is never
+// negative; r2 is clamped to 1e-20 (pairs that close have delay 0: the self
+// pair) so that their geometry stays finite.
+__device__ __forceinline__ int split_l64(double dx, double dy, double dz, const double4 &pa64,
+                                         const double4 *__restrict__ pos64, int b, double cdt, double &xc,
+                                         double &ir)
+{
+  const double r2 = __builtin_fmax(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), 1e-20);
+  double y = __builtin_amdgcn_rsq(r2);
+  {
+    const double e = __builtin_fma(-(r2 * y), y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+  }
+  {
+    const double e = __builtin_fma(-(r2 * y), y, 1.0);
+    y = __builtin_fma(0.5 * y, e, y);
+  }
+  ir = y;
+  const double tau = r2 * y;
+  double fl = __builtin_floor(tau);
+  double fr = tau - fl;
+  if(__builtin_expect(fr < 1e-9 || fr > 1.0 - 1e-9, 0)) {
+#pragma clang fp contract(off)
+    const double4 pb = pos64[b];
+    const double ex = pa64.x - pb.x, ey = pa64.y - pb.y, ez = pa64.z - pb.z;
+    const double te = __builtin_sqrt((ex * ex + ey * ey) + ez * ez) / cdt;
+    fl = __builtin_floor(te);
+    fr = te - fl;
+  }
+  xc = fr - 0.5;
+  return static_cast<int>(fl);
+}
+
 // Delay split of the mixed-precision path: the FP32 estimate decides the
 // integer part unless its fraction lies within the estimate's error bound
 // eps = eps_rel tau + eps_abs (position rounding plus five roundings, with a
@@ -181,20 +223,21 @@ __device__ __forceinline__ void sincos_theta(const float *sn, const float *cs, f
   s = x * __builtin_fmaf(x2, sn[1], sn[0]);
   c = __builtin_fmaf(x2, __builtin_fmaf(x2, cs[1], cs[0]), 1.0f);
 }
-// sin and cos of theta x for x in [0, 1), used while theta <= 0.33: Taylor
+// sin and cos of theta x for |x| <= 1/2, used while theta <= 0.33: Taylor
 // series in x^2 with theta folded into the coefficients (Consts::trig64,
 // kernel arguments, so each FMA takes its coefficient from a scalar register),
-// truncated where the next term is below FP64 rounding.
-__device__ __forceinline__ void sincos_small(const double *tg, double x, double &s, double &c)
+// truncated where the next term is below FP64 rounding (|theta x| <= 0.165:
+// sine to u^9, cosine to u^10).
+__device__ __forceinline__ void sincos_centred64(const double *tg, double x, double &s, double &c)
 {
   const double x2 = x * x;
   double ps = tg[0];
 #pragma unroll
-  for(int k = 1; k < 6; ++k) ps = __builtin_fma(ps, x2, tg[k]);
+  for(int k = 1; k < 5; ++k) ps = __builtin_fma(ps, x2, tg[k]);
   s = x * ps;
-  double pc = tg[6];
+  double pc = tg[5];
 #pragma unroll
-  for(int k = 7; k < 12; ++k) pc = __builtin_fma(pc, x2, tg[k]);
+  for(int k = 6; k < 10; ++k) pc = __builtin_fma(pc, x2, tg[k]);
   c = __builtin_fma(pc, x2, 1.0);
 }
 
@@ -270,29 +313,20 @@ __device__ __forceinline__ void horner3(const typename V2<R>::type *c, R x, R &q
   }
 }
 
-// Pair constants converted to the pair arithmetic's type.  FP64 geometry is
-// in um; FP32 positions are stored in units of L = c0 dt, so there 1/r, g3
-// and g1 are L, L^2 and L times their um values and the constants that
-// multiply them carry the inverse factors (the *_l constants), all divided by
-// f32_scale = two_idt2 / L so that the g1 q2 term needs no multiply.
+// Pair constants converted to the pair arithmetic's type.  Both pair sums
+// work in units of L = c0 dt (1/r, g3 and g1 are L, L^2 and L times their um
+// values) and divide every term by unit_scale = two_idt2 / L, so that the
+// constants of the pair term reduce to theta and theta^2 / 2 (pair_term).
 template <typename R> struct PairK {
   R oo, theta, eps_rel, lim0;
   R sn[2], cs[2];  // sincos_theta coefficients (FP32)
-  // FP64 pair_term only: the constants divided by two_idt2 (the kernel
-  // multiplies each target's sum by two_idt2 once), with m_cdt factored out of
-  // G3: tc = cc / m_cdt, ocm = oc / m_cdt, g3m = m_cdt / two_idt2, mo = m_o2dt / two_idt2
-  R tc, ocm, g3m, mo;
-  double tg[12];  // sincos_small coefficients (FP64)
+  double tg[10];   // sincos_centred64 coefficients (FP64)
   double cdt, icdt64;
-  // FP32: oo = theta^2 / 2 (oo_l); FP64: oo = w^2 / two_idt2
   __host__ __device__ explicit PairK(const Consts &k)
-      : oo((R)(sizeof(R) == sizeof(float) ? k.oo_l : k.oo / k.two_idt2)), theta((R)k.theta),
-        eps_rel((R)k.eps_rel), lim0((R)(0.5 - k.eps_abs - 0.5 * k.eps_rel)),
-        sn{(R)k.sin_c[0], (R)k.sin_c[1]}, cs{(R)k.cos_c[0], (R)k.cos_c[1]},
-        tc((R)(k.cc / k.m_cdt)), ocm((R)(k.oc / k.m_cdt)), g3m((R)(k.m_cdt / k.two_idt2)),
-        mo((R)(k.m_o2dt / k.two_idt2)), cdt(k.cdt), icdt64(k.icdt)
+      : oo((R)k.oo_l), theta((R)k.theta), eps_rel((R)k.eps_rel), lim0((R)(0.5 - k.eps_abs - 0.5 * k.eps_rel)),
+        sn{(R)k.sin_c[0], (R)k.sin_c[1]}, cs{(R)k.cos_c[0], (R)k.cos_c[1]}, cdt(k.cdt), icdt64(k.icdt)
   {
-    for(int i = 0; i < 12; ++i) tg[i] = k.trig64[i];
+    for(int i = 0; i < 10; ++i) tg[i] = k.trig64[i];
   }
 };
 
@@ -308,91 +342,56 @@ template <typename R> struct PairK {
 //   G3 = (c^2/r) q0 + i (w c) q0 - (c/dt) q1,
 //   G1 = (2/dt^2) q2 - w^2 q0 - i (2w/dt) q1,
 // and the rotated term is added to (accr, acci) in the same fused operations.
-// This FP64 form carries the whole term divided by 2/dt^2 (the kernel scales
-// each target's sum back once) and m_cdt = -c/dt factored out of G3:
-//   g3 G3 / (2/dt^2) = g3m' [tc (1/r) q0 + i ocm q0 + q1],  g3m' = g3 m_cdt dt^2/2,
-//   G1 / (2/dt^2) = q2 - oo q0 - i mo q1,
-// with tc = c^2 / m_cdt, ocm = w c / m_cdt, oo = w^2 dt^2/2, mo = -w dt
-// (PairK).  GEN: e^{-i theta x} from the library sincos (theta beyond the
-// short series' range); a separate kernel variant, so the common one carries
-// none of it.
-template <bool LAB, bool GEN = false>
-__device__ __forceinline__ void pair_term_f64(const PairK<double> &K, double ir, double dx, double dy,
-                                              double dz, const double4 &da, const double4 &db, double x,
-                                              double q0r, double q0i, double q1r, double q1i, double q2r,
-                                              double q2i, double &accr, double &acci)
-{
-  const double ir2 = ir * ir;
-  const double ba = da.x * dx + da.y * dy + da.z * dz;
-  const double bb = db.x * dx + db.y * dy + db.z * dz;
-  const double aab = da.x * db.x + da.y * db.y + da.z * db.z;
-  const double bq = ba * bb * ir2;
-  const double g3 = (aab - 3 * bq) * (ir2 * K.g3m);
-  const double g1 = (aab - bq) * ir;
-  const double t = K.tc * ir;
-  if constexpr(LAB) {
-    const double g3r = fmaT(t, q0r, q1r);
-    const double g3i = fmaT(t, q0i, q1i);
-    accr = fmaT(g3, g3r, fmaT(g1, q2r, accr));
-    acci = fmaT(g3, g3i, fmaT(g1, q2i, acci));
-  } else {
-    const double g3r = fmaT(t, q0r, fmaT(-K.ocm, q0i, q1r));
-    const double g3i = fmaT(t, q0i, fmaT(K.ocm, q0r, q1i));
-    const double g1r = fmaT(-K.oo, q0r, fmaT(-K.mo, q1i, q2r));
-    const double g1i = fmaT(-K.oo, q0i, fmaT(K.mo, q1r, q2i));
-    const double cr = g3 * g3r + g1 * g1r;
-    const double ci = g3 * g3i + g1 * g1i;
-    double sn, cs;
-    if constexpr(GEN)
-      sincos(K.theta * x, &sn, &cs);
-    else
-      sincos_small(K.tg, x, sn, cs);
-    accr = fmaT(cr, cs, fmaT(ci, sn, accr));
-    acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
-  }
-}
-
-// The same term in the FP32 pair arithmetic's units (lengths in L = c0 dt,
-// the whole term divided by f32_scale = 2 / (dt^2 L)).  There the constants
-// of G3 and G1 reduce to
+// In the pair sums' units (lengths in L = c0 dt, the whole term divided by
+// unit_scale = 2 / (dt^2 L)) the constants of G3 and G1 reduce to
 //   c^2/r -> (1/2) ir,  w c -> theta / 2,  c/dt -> 1/2,
 //   w^2 -> theta^2 / 2,  2w/dt -> theta,  2/dt^2 -> 1      (theta = w dt),
 // so with g3h = g3 / 2
 //   g3 G3 = g3h [ir q0 - q1 + i theta q0],  G1 = q2 - (theta^2/2) q0 - i theta q1.
 // g3h and g1 come from the halved target dipole dah = d_a / 2 (an exact
 // power-of-two scale): aab/2 = dah.d_b, g3h = (aab/2 - (3/2) B) / r^2 and
-// g1 = (2 (aab/2) - B) / r, the same value as (aab - B) / r.
-template <bool LAB, bool GEN = false>
-__device__ __forceinline__ void pair_term_f32(const PairK<float> &K, float ir, float dx, float dy, float dz,
-                                              const float4 &da, const float4 &dah, const float4 &db, float x,
-                                              float q0r, float q0i, float q1r, float q1i, float q2r, float q2i,
-                                              float &accr, float &acci)
+// g1 = (2 (aab/2) - B) / r, the same value as (aab - B) / r.  x is the
+// fraction centred on 1/2 (the ring's basis; e^{-i theta / 2} is folded into
+// the ring).  GEN: e^{-i theta x} from the library sincos (theta beyond the
+// short series' range); a separate kernel variant, so the common one carries
+// none of it.
+template <typename R, bool LAB, bool GEN = false>
+__device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R dz, const typename V4<R>::type &da,
+                                          const typename V4<R>::type &dah, const typename V4<R>::type &db, R x,
+                                          R q0r, R q0i, R q1r, R q1i, R q2r, R q2i, R &accr, R &acci)
 {
-  const float ir2 = ir * ir;
-  const float ba = da.x * dx + da.y * dy + da.z * dz;
-  const float bb = db.x * dx + db.y * dy + db.z * dz;
-  const float aabh = dah.x * db.x + dah.y * db.y + dah.z * db.z;
-  const float bq = ba * bb * ir2;
-  const float g3h = fmaT(-1.5f, bq, aabh) * ir2;
-  const float g1 = fmaT(2.0f, aabh, -bq) * ir;
+  const R ir2 = ir * ir;
+  const R ba = da.x * dx + da.y * dy + da.z * dz;
+  const R bb = db.x * dx + db.y * dy + db.z * dz;
+  const R aabh = dah.x * db.x + dah.y * db.y + dah.z * db.z;
+  const R bq = ba * bb * ir2;
+  const R g3h = fmaT(R(-1.5), bq, aabh) * ir2;
+  const R g1 = fmaT(R(2), aabh, -bq) * ir;
   if constexpr(LAB) {
-    const float g3r = fmaT(ir, q0r, -q1r);
-    const float g3i = fmaT(ir, q0i, -q1i);
+    const R g3r = fmaT(ir, q0r, -q1r);
+    const R g3i = fmaT(ir, q0i, -q1i);
     accr = fmaT(g3h, g3r, fmaT(g1, q2r, accr));
     acci = fmaT(g3h, g3i, fmaT(g1, q2i, acci));
   } else {
-    const float th = K.theta;
-    const float g3r = fmaT(ir, q0r, fmaT(-th, q0i, -q1r));
-    const float g3i = fmaT(ir, q0i, fmaT(th, q0r, -q1i));
-    const float g1r = fmaT(-K.oo, q0r, fmaT(th, q1i, q2r));
-    const float g1i = fmaT(-K.oo, q0i, fmaT(-th, q1r, q2i));
-    const float cr = g3h * g3r + g1 * g1r;
-    const float ci = g3h * g3i + g1 * g1i;
-    float sn, cs;
-    if constexpr(GEN)
-      __sincosf(K.theta * x, &sn, &cs);
-    else
-      sincos_theta(K.sn, K.cs, x, sn, cs);
+    const R th = K.theta;
+    const R g3r = fmaT(ir, q0r, fmaT(-th, q0i, -q1r));
+    const R g3i = fmaT(ir, q0i, fmaT(th, q0r, -q1i));
+    const R g1r = fmaT(-K.oo, q0r, fmaT(th, q1i, q2r));
+    const R g1i = fmaT(-K.oo, q0i, fmaT(-th, q1r, q2i));
+    const R cr = g3h * g3r + g1 * g1r;
+    const R ci = g3h * g3i + g1 * g1i;
+    R sn, cs;
+    if constexpr(sizeof(R) == sizeof(double)) {
+      if constexpr(GEN)
+        sincos(K.theta * x, &sn, &cs);
+      else
+        sincos_centred64(K.tg, x, sn, cs);
+    } else {
+      if constexpr(GEN)
+        __sincosf(K.theta * x, &sn, &cs);
+      else
+        sincos_theta(K.sn, K.cs, x, sn, cs);
+    }
     accr = fmaT(cr, cs, fmaT(ci, sn, accr));
     acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
   }
@@ -465,9 +464,9 @@ constexpr int kChunk = 64;  // sources summed in the pair type before the FP64 f
 
 template <int M, bool LAB, bool DYN, bool GEN>
 __global__ __launch_bounds__(kBlock) void pair_past_f64(
-    const double4 *__restrict__ pos, const double4 *__restrict__ dip,
-    const double2 *__restrict__ ring, const int4 *__restrict__ work, const int *__restrict__ work_row,
-    int *step_ptr, double2 *__restrict__ partial, Consts k)
+    const double4 *__restrict__ pd, const double4 *__restrict__ pos64, const double2 *__restrict__ ring,
+    const int4 *__restrict__ work, const int *__restrict__ work_row, int *step_ptr, double2 *__restrict__ partial,
+    Consts k)
 {
   const PairK<double> K(k);
   const int step = *step_ptr;
@@ -476,27 +475,30 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
     const int a = tgt < wi.y ? tgt : wi.y - 1;
-    const double4 pa = pos[a], da = dip[a];
+    const double4 pa = pd[2 * a], da = pd[2 * a + 1];
+    const double4 dah = make_double4(0.5 * da.x, 0.5 * da.y, 0.5 * da.z, 0.0);
+    const double4 pa64 = pos64[a];
     double accr = 0, acci = 0;
     for(int b = wi.z; b < wi.w; ++b) {
-      const double4 pb = pos[b], db = dip[b];
+      const double4 pb = pd[2 * b], db = pd[2 * b + 1];
       const double dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-      double x, ir, r2;
+      double xc, ir;
       // delay 0 (the self pair, zero-delay partners) reads the current step's
       // slot, which holds zeros until the step commits: its term is 0
-      const int n = split_f64(dx, dy, dz, K.cdt, K.icdt64, x, ir, r2);
+      const int n = split_l64(dx, dy, dz, pa64, pos64, b, K.cdt, xc, ir);
       const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
       const double2 *c = ring + static_cast<size_t>(b) * k.ring_slots * (M + 1);
-      const unsigned off = slot * static_cast<unsigned>((M + 1) * sizeof(double2));
+      // slot < 2^22: a 24-bit multiply (full rate) gives the record's byte offset
+      const unsigned off = __umul24(slot, static_cast<unsigned>((M + 1) * sizeof(double2)));
       double2 cv[M + 1];
 #pragma unroll
       for(int i = 0; i <= M; ++i) cv[i] = load_off(c + i, off);
       double q0r, q0i, q1r, q1i, q2r, q2i;
-      horner3<double, M>(cv, x, q0r, q0i, q1r, q1i, q2r, q2i);
-      pair_term_f64<LAB, GEN>(K, ir, dx, dy, dz, da, db, x, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
+      horner3<double, M>(cv, xc, q0r, q0i, q1r, q1i, q2r, q2i);
+      pair_term<double, LAB, GEN>(K, ir, dx, dy, dz, da, dah, db, xc, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
     }
     if(tgt < wi.y)
-      partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.two_idt2, acci * k.two_idt2);
+      partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
   }
   items.finish();
 }
@@ -579,8 +581,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
       auto evaluate = [&](const StagedF32<M> &st) {
         float q0r, q0i, q1r, q1i, q2r, q2i;
         horner3<float, M>(st.cv, st.x, q0r, q0i, q1r, q1i, q2r, q2i);
-        pair_term_f32<LAB, GEN>(K, st.ir, st.dx, st.dy, st.dz, da, dah, st.db, st.x, q0r, q0i, q1r, q1i, q2r,
-                                q2i, sr, si);
+        pair_term<float, LAB, GEN>(K, st.ir, st.dx, st.dy, st.dz, da, dah, st.db, st.x, q0r, q0i, q1r, q1i, q2r,
+                                   q2i, sr, si);
       };
       StagedF32<M> cur;
       if constexpr(PF) {
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
       acci += static_cast<double>(si);
     }
     if(tgt < wi.y)
-      partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.f32_scale, acci * k.f32_scale);
+      partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
   }
   items.finish();
 }
@@ -822,13 +824,11 @@ __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase
   if(lab)
 #pragma unroll
     for(int m = 0; m <= kMaxOrder; ++m) smp[m] = make_double2(2.0 * smp[m].x, 0.0);
-  // FP32 ring: coefficients about x = 1/2, so the rotation e^{-i theta x} of
-  // the pair kernel splits into e^{-i theta / 2} (here) and e^{-i theta xc}
-  constexpr bool f32 = sizeof(R2) == sizeof(float2);
-  const double *basis = f32 ? k.vinv_c : k.vinv;
-  const double2 ph = lab ? make_double2(1.0, 0.0)
-                         : (f32 ? cmul(phase_at(phase, s), make_double2(k.half_phase_re, k.half_phase_im))
-                                : phase_at(phase, s));
+  // coefficients about x = 1/2, so the rotation e^{-i theta x} of the pair
+  // kernels splits into e^{-i theta / 2} (here) and e^{-i theta xc}
+  const double *basis = k.vinv_c;
+  const double2 ph =
+      lab ? make_double2(1.0, 0.0) : cmul(phase_at(phase, s), make_double2(k.half_phase_re, k.half_phase_im));
   const size_t E = static_cast<size_t>(k.n) * k.ring_slots;
   const size_t e = static_cast<size_t>(a) * k.ring_slots + (s & k.ring_mask);
   for(int kk = 0; kk <= k.order; ++kk) {
@@ -1279,7 +1279,7 @@ __global__ __launch_bounds__(kBlock) void table_build_kernel(Consts k, DeviceSta
 // Diagnostic: integer delays of unordered pairs [first, first + count) in the
 // reference's lower-triangle order (history_interaction.cpp:80-86; row = the
 // pair's source, col < row its observer), from the delay split the pair
-// kernel of the precision uses: split_f64 (F64) or split_mixed, the FP32
+// kernel of the precision uses: split_l64 (F64) or split_mixed, the FP32
 // estimate with the FP64 split where it lies near an integer.
 template <bool F64>
 __global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_t first, int64_t count)
@@ -1294,9 +1294,10 @@ __global__ void pair_delays_kernel(Consts k, DeviceState d, int32_t *out, int64_
     const int64_t col = idx - row * (row - 1) / 2;
     const int rs = d.perm_dev[row], cs = d.perm_dev[col];
     if constexpr(F64) {
-      const double4 pr = d.pos64[rs], pc = d.pos64[cs];
-      double x, ir, r2;
-      out[i] = split_f64(pc.x - pr.x, pc.y - pr.y, pc.z - pr.z, k.cdt, k.icdt, x, ir, r2);
+      // observer cs as the target a, source rs as b, as in the pair kernel
+      const double4 pa = d.pd64[2 * cs], pb = d.pd64[2 * rs];
+      double xc, ir;
+      out[i] = split_l64(pa.x - pb.x, pa.y - pb.y, pa.z - pb.z, d.pos64[cs], d.pos64, rs, k.cdt, xc, ir);
     } else {
       // observer cs as the target a, source rs as b, as in the pair kernel
       const float4 pa = d.pd32[2 * cs], pb = d.pd32[2 * rs];
@@ -1314,7 +1315,7 @@ void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t
 {
   const dim3 grid(k.grid), block(kBlock);
   if(precision == QUEST_PREC_FP64)
-    hipLaunchKernelGGL((pair_past_f64<M, LAB, DYN, GEN>), grid, block, 0, s, d.pos64, d.dip64,
+    hipLaunchKernelGGL((pair_past_f64<M, LAB, DYN, GEN>), grid, block, 0, s, d.pd64, d.pos64,
                        static_cast<const double2 *>(d.ring), d.work, d.work_row, d.step, d.partial, k);
   else {
     // FP32 ring layout (see ring_index): (M + 1) / 2 float4 planes, then the c_M plane

@@ -1,12 +1,15 @@
 // quest_kernels.hip -- gfx950 kernels of the retarded-interaction step.
 //
 // Data layout in HBM (all in the library's Morton-sorted dot order):
-//   pos/dip        (x, y, z, 0) per dot, FP64 and FP32 copies
+//   pos/dip        (x, y, z, 0) per dot in um (FP64), and interleaved
+//                  (position relative to the centre in units of c0 dt,
+//                  dipole) copies in FP64 and FP32 for the pair sums
 //   ring           for every source dot and every finished step s (slot
-//                  s & ring_mask) the monomial coefficients c_k of its
-//                  Lagrange interpolant of rho01 through steps s, s-1, ...,
-//                  s-M, multiplied by e^{i theta s} (rotating frame), in FP64
-//                  or FP32, in coefficient planes (see ring_index)
+//                  s & ring_mask) the monomial coefficients c_k (in x - 1/2)
+//                  of its Lagrange interpolant of rho01 through steps s, s-1,
+//                  ..., s-M, multiplied by e^{i theta s} e^{-i theta / 2}
+//                  (rotating frame), in FP64 or FP32, in coefficient planes
+//                  (see ring_index)
 //   state ring     double2 [state_slots][n] rho00, rho01 and derivatives, for
 //                  the predictor-corrector window
 //
@@ -245,20 +248,20 @@ __device__ __forceinline__ float fmaT(float a, float b, float c) { return __buil
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Interpolant ring.  Entry e = a * ring_slots + slot holds the M + 1
-// coefficients of source a for one slot; E = n * ring_slots entries.
-//   FP64: one record per entry, monomials in x     double2 [E][M + 1]
-//   FP32: monomials in xc = x - 1/2, in (M + 1) / 2 planes of two coefficients
-//         float4 [(M + 1) / 2][E], then, for even M, one plane of c_M
-//         float2 [E]
+// coefficients (monomials in xc = x - 1/2) of source a for one slot;
+// E = n * ring_slots entries, in coefficient planes:
+//   FP64: double2 [M + 1][E]
+//   FP32: (M + 1) / 2 planes of two coefficients float4 [(M + 1) / 2][E],
+//         then, for even M, one plane of c_M float2 [E]
 // A wave's lanes read one source at the ~30 consecutive slots its targets'
-// delays span.  In the FP32 planes each load instruction touches a short
-// contiguous run of 16-B aligned entries (measured 3.5% faster than 40-B
-// records at N = 65536); the FP64 planes were no faster than records and
-// need five more SGPRs, which costs residency.
+// delays span, so each load instruction touches a short contiguous run of
+// 16-B entries (~4 cache lines) instead of ~20 lines of whole records.
+// Measured against 80-B records: FP64 -18 % at N = 131072 and within 1 % at
+// N = 65536; FP32 (40-B records) -3.5 % at N = 65536.
 // Index of coefficient kk of entry e, in units of the complex type.
-__device__ __forceinline__ size_t ring_index(const double2 *, int order, size_t /*E*/, size_t e, int kk)
+__device__ __forceinline__ size_t ring_index(const double2 *, int /*order*/, size_t E, size_t e, int kk)
 {
-  return e * static_cast<size_t>(order + 1) + kk;
+  return static_cast<size_t>(kk) * E + e;
 }
 __device__ __forceinline__ size_t ring_index(const float2 *, int order, size_t E, size_t e, int kk)
 {
@@ -470,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
 {
   const PairK<double> K(k);
   const int step = *step_ptr;
+  const size_t E = static_cast<size_t>(k.n) * k.ring_slots;  // entries per coefficient plane
   ItemCursor<DYN> items(step_ptr + 1);
   for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
@@ -486,13 +490,11 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
       // delay 0 (the self pair, zero-delay partners) reads the current step's
       // slot, which holds zeros until the step commits: its term is 0
       const int n = split_l64(dx, dy, dz, pa64, pos64, b, K.cdt, xc, ir);
-      const unsigned slot = static_cast<unsigned>(step - n) & k.ring_mask;
-      const double2 *c = ring + static_cast<size_t>(b) * k.ring_slots * (M + 1);
-      // slot < 2^22: a 24-bit multiply (full rate) gives the record's byte offset
-      const unsigned off = __umul24(slot, static_cast<unsigned>((M + 1) * sizeof(double2)));
+      const unsigned off = (static_cast<unsigned>(step - n) & k.ring_mask) << 4;  // slot * 16 B
+      const double2 *row = ring + static_cast<size_t>(b) * k.ring_slots;       // source b in plane 0
       double2 cv[M + 1];
 #pragma unroll
-      for(int i = 0; i <= M; ++i) cv[i] = load_off(c + i, off);
+      for(int i = 0; i <= M; ++i) cv[i] = load_off(row + static_cast<size_t>(i) * E, off);
       double q0r, q0i, q1r, q1i, q2r, q2i;
       horner3<double, M>(cv, xc, q0r, q0i, q1r, q1i, q2r, q2i);
       pair_term<double, LAB, GEN>(K, ir, dx, dy, dz, da, dah, db, xc, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);

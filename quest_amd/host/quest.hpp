@@ -20,8 +20,11 @@
 // DeviceIntegrator runs the whole predictor-corrector loop on the device.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <complex>
 #include <fstream>
 #include <functional>
@@ -300,6 +303,116 @@ inline void write_history(const std::shared_ptr<const HistoryArray> &history,
     output << std::endl;
   }
 }
+// Output of long or large runs without the O(N T) text history (an addition;
+// the reference writes every dot at every step, history.cpp:17-34): every
+// `every`-th step's (rho00, rho01) of a set of dots, as text in output.dat's
+// row format (one line per recorded step, the listed dots only) or as binary
+// records.  Binary layout, little-endian:
+//   char[8] "QUESTOUT"; uint32 version (1), n_dots (dots in the system),
+//   n_sel (dots recorded), every, n_records, 0; double dt;
+//   int32 dots[n_sel] (row indices in dots.cfg, ascending);
+//   n_records x { int32 step, int32 0, double [n_sel][4] (re, im rho00;
+//   re, im rho01) }
+// step t is output.dat's line t: the state after step t (t = every - 1,
+// 2 every - 1, ...).
+struct SampledOutput {
+  std::string path;
+  bool binary = false;
+  int every = 1;
+  std::vector<int> dots;  // ascending; all dots when the subsample is not asked for
+};
+
+// `count` distinct dots of n chosen by a seeded partial Fisher-Yates shuffle
+// (splitmix64 stream), returned in ascending order: the same set on every
+// platform for a given (n, count, seed).
+inline std::vector<int> subsample_dots(int n, int count, uint64_t seed)
+{
+  if(count < 1 || count > n) throw std::runtime_error("subsample count must be in 1..num_particles");
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  uint64_t x = seed;
+  auto next = [&x]() {
+    uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+  };
+  for(int i = 0; i < count; ++i) {
+    const uint64_t span = static_cast<uint64_t>(n - i);
+    const int j = i + static_cast<int>(next() % span);
+    std::swap(idx[i], idx[j]);
+  }
+  std::vector<int> out(idx.begin(), idx.begin() + count);
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+class SampledWriter {
+ public:
+  SampledWriter(const SampledOutput &spec, int n_dots, int num_timesteps, double dt) : spec_(spec)
+  {
+    if(spec_.every < 1) throw std::runtime_error("--every must be >= 1");
+    if(spec_.dots.empty()) {
+      spec_.dots.resize(n_dots);
+      std::iota(spec_.dots.begin(), spec_.dots.end(), 0);
+    }
+    for(int d : spec_.dots)
+      if(d < 0 || d >= n_dots) throw std::runtime_error("sampled dot index out of range");
+    out_.open(spec_.path, spec_.binary ? std::ios::binary : std::ios::out);
+    if(!out_) throw std::runtime_error("Could not open " + spec_.path);
+    records_ = num_timesteps / spec_.every;
+    if(spec_.binary) {
+      const uint32_t head[6] = {1u, static_cast<uint32_t>(n_dots), static_cast<uint32_t>(spec_.dots.size()),
+                                static_cast<uint32_t>(spec_.every), static_cast<uint32_t>(records_), 0u};
+      out_.write("QUESTOUT", 8);
+      out_.write(reinterpret_cast<const char *>(head), sizeof(head));
+      out_.write(reinterpret_cast<const char *>(&dt), sizeof(dt));
+      out_.write(reinterpret_cast<const char *>(spec_.dots.data()),
+                 static_cast<std::streamsize>(sizeof(int) * spec_.dots.size()));
+    } else {
+      out_ << std::setprecision(12) << std::scientific;
+    }
+  }
+  int every() const { return spec_.every; }
+  int records() const { return records_; }
+  // The state after step `step` of every dot: rho [n_dots][4] (re, im rho00;
+  // re, im rho01), as quest_get_state returns it.
+  void write(int step, const double *rho)
+  {
+    if(spec_.binary) {
+      const int32_t tag[2] = {step, 0};
+      out_.write(reinterpret_cast<const char *>(tag), sizeof(tag));
+      for(int d : spec_.dots) out_.write(reinterpret_cast<const char *>(rho + 4 * static_cast<size_t>(d)), 32);
+    } else {
+      for(int d : spec_.dots) {
+        const double *r = rho + 4 * static_cast<size_t>(d);
+        out_ << cmplx(r[0], r[1]) << " " << cmplx(r[2], r[3]) << " ";
+      }
+      out_ << std::endl;
+    }
+    if(!out_) throw std::runtime_error("write to " + spec_.path + " failed");
+  }
+  // From a history array (the host loop): the recorded steps of `history`.
+  void write_history(const HistoryArray &history, int num_timesteps)
+  {
+    std::vector<double> rho(4 * static_cast<size_t>(history.num_dots()));
+    for(int t = spec_.every - 1; t < num_timesteps; t += spec_.every) {
+      for(int i = 0; i < history.num_dots(); ++i) {
+        const soltype &s = history[i][t][0];
+        rho[4 * i] = s[0].real();
+        rho[4 * i + 1] = s[0].imag();
+        rho[4 * i + 2] = s[1].real();
+        rho[4 * i + 3] = s[1].imag();
+      }
+      write(t, rho.data());
+    }
+  }
+
+ private:
+  SampledOutput spec_;
+  std::ofstream out_;
+  int records_ = 0;
+};
 }  // namespace History
 
 // -------------------------------------------------------- green functions
@@ -638,6 +751,23 @@ class DeviceIntegrator {
           (*history)[static_cast<int>(i)][done - 1][1] = {cmplx(drho[4 * i], drho[4 * i + 1]),
                                                           cmplx(drho[4 * i + 2], drho[4 * i + 3])};
         }
+      }
+    }
+  }
+  // Runs n_steps steps and streams every writer.every()-th step's state to
+  // `out` (no host history is filled).
+  void solve_sampled(History::SampledWriter &out, int n_steps)
+  {
+    const size_t n = dots->size();
+    std::vector<double> rho(4 * n);
+    int done = 0;
+    while(done < n_steps) {
+      const int chunk = std::min(out.every(), n_steps - done);
+      quest_detail::check(quest_run(ctx.get(), chunk), ctx.get());
+      done += chunk;
+      if(chunk == out.every()) {
+        quest_detail::check(quest_get_state(ctx.get(), rho.data(), nullptr), ctx.get());
+        out.write(done - 1, rho.data());
       }
     }
   }

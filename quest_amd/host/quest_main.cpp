@@ -4,7 +4,8 @@
 //
 //   quest_hip [--config input.cfg] [--precision fp64|mixed] [--host-loop]
 //             [--all-pulses] [--pair-method on-the-fly|table]
-//             [--output output.dat] [--device N] [--help] [--version]
+//             [--output output.dat] [--format text|binary] [--every K]
+//             [--subsample COUNT:SEED] [--device N] [--help] [--version]
 //
 // input.cfg is the reference's INI file (configuration.cpp:17-42):
 //   [files]      qd_path (dots.cfg), pulse_path (pulse.cfg)
@@ -20,11 +21,17 @@
 // only, which is what the reference reads (pulse.cpp:37-46).
 // --pair-method table streams the reference's per-pair coefficient table
 // (FP64 only) instead of evaluating the Green's function per pair.
+// --format binary, --every K and --subsample COUNT:SEED write only every K-th
+// step and/or a seeded subsample of the dots, as text rows in output.dat's
+// format or as binary records (History::SampledWriter); with the device loop
+// the states stream from the device every K steps and no O(N T) host history
+// is held (at N = 65536 x 2000 steps the full text output is about 5 GB).
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <string>
 
@@ -32,8 +39,9 @@
 
 namespace {
 
+// --help, --version or a missing configuration file: nothing to run.
 struct CommandLineException : public std::exception {
-  const char *what() const noexcept override { return "Have a nice day! :)"; }
+  const char *what() const noexcept override { return "command line handled"; }
 };
 
 std::string trim(const std::string &s)
@@ -85,6 +93,11 @@ struct Options {
   std::string output = "output.dat";
   bool host_loop = false;
   bool all_pulses = false;
+  bool binary = false;
+  int every = 1;
+  int sub_count = 0;  // 0: every dot
+  uint64_t sub_seed = 0;
+  bool sampled() const { return binary || every != 1 || sub_count > 0; }
 };
 
 // configuration.cpp:7-81
@@ -107,6 +120,9 @@ Options parse_configs(int argc, char *argv[])
                    "  --all-pulses           every row of pulse.cfg (default: the first)\n"
                    "  --pair-method arg      on-the-fly (default) or table (FP64 only)\n"
                    "  --output arg           output file (default output.dat)\n"
+                   "  --format arg           text (default, output.dat's format) or binary\n"
+                   "  --every arg            record every arg-th step (default 1)\n"
+                   "  --subsample arg        COUNT:SEED, record a seeded subsample of the dots\n"
                    "  --device arg           HIP device ordinal (default 0)\n";
       throw CommandLineException();
     } else if(a == "--version" || a == "-v") {
@@ -130,6 +146,21 @@ Options parse_configs(int argc, char *argv[])
       else throw std::runtime_error("--pair-method must be on-the-fly or table");
     } else if(a == "--output") {
       o.output = next();
+    } else if(a == "--format") {
+      const std::string v = next();
+      if(v == "text") o.binary = false;
+      else if(v == "binary") o.binary = true;
+      else throw std::runtime_error("--format must be text or binary");
+    } else if(a == "--every") {
+      o.every = std::stoi(next());
+      if(o.every < 1) throw std::runtime_error("--every must be >= 1");
+    } else if(a == "--subsample") {
+      const std::string v = next();
+      const size_t colon = v.find(':');
+      if(colon == std::string::npos) throw std::runtime_error("--subsample takes COUNT:SEED");
+      o.sub_count = std::stoi(v.substr(0, colon));
+      o.sub_seed = std::stoull(v.substr(colon + 1));
+      if(o.sub_count < 1) throw std::runtime_error("--subsample COUNT must be >= 1");
     } else if(a == "--device") {
       config.device = std::stoi(next());
     } else {
@@ -170,10 +201,21 @@ int main(int argc, char *argv[])
     qds->resize(config.num_particles);
     auto rhs_funs = rhs_functions(*qds, config.omega);
 
-    auto history = History::make_shared_history(config.num_particles, 22, config.num_timesteps);
+    // the device loop with sampled output streams its records and needs no host history
+    const bool stream = opt.sampled() && !opt.host_loop;
+    auto history = History::make_shared_history(config.num_particles, 22, stream ? 1 : config.num_timesteps);
     for(int t = -22; t <= 0; ++t)
       for(int sol_idx = 0; sol_idx < config.num_particles; ++sol_idx)
         (*history)[sol_idx][t][0] = {cmplx(1, 0), cmplx(0, 0)};  // ground state
+    std::unique_ptr<History::SampledWriter> sampled;
+    if(opt.sampled()) {
+      History::SampledOutput spec;
+      spec.path = opt.output;
+      spec.binary = opt.binary;
+      spec.every = opt.every;
+      if(opt.sub_count > 0) spec.dots = History::subsample_dots(config.num_particles, opt.sub_count, opt.sub_seed);
+      sampled = std::make_unique<History::SampledWriter>(spec, config.num_particles, config.num_timesteps, config.dt);
+    }
 
     std::vector<std::shared_ptr<const Pulse>> pulses;
     if(opt.all_pulses) {
@@ -199,11 +241,13 @@ int main(int argc, char *argv[])
       PredictorCorrector::DeviceIntegrator integrator(qds, pulses, rotating_dyadic,
                                                       config.interpolation_order, history);
       integrator.set_weights(PredictorCorrector::Weights(18, 22, 3.15));
-      integrator.solve(1);
+      if(stream) integrator.solve_sampled(*sampled, config.num_timesteps);
+      else integrator.solve(1);
     }
 
     std::cout << "Writing output..." << std::endl;
-    History::write_history(history, opt.output);
+    if(!opt.sampled()) History::write_history(history, opt.output);
+    else if(opt.host_loop) sampled->write_history(*history, config.num_timesteps);
   } catch(CommandLineException &e) {
     // --help / --version / missing configuration file: nothing more to do
   } catch(std::domain_error &e) {

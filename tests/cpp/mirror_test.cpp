@@ -151,6 +151,51 @@ static void output_format()
   std::remove("mirror_test_output.dat");
 }
 
+// History::subsample_dots: COUNT distinct ascending indices, the same set for
+// the same seed, a different one for another seed; SampledWriter: a binary
+// and a text file of known values (checked by tests/test_cpp_mirror.py with
+// quest_amd.outputs).
+static void sampled_output()
+{
+  const std::vector<int> a = History::subsample_dots(1000, 64, 11), b = History::subsample_dots(1000, 64, 11);
+  const std::vector<int> c = History::subsample_dots(1000, 64, 12);
+  CHECK(a == b);
+  CHECK(a != c);
+  CHECK(a.size() == 64);
+  for(size_t i = 1; i < a.size(); ++i) CHECK(a[i - 1] < a[i]);
+  CHECK(a.front() >= 0 && a.back() < 1000);
+  const std::vector<int> all = History::subsample_dots(5, 5, 3);
+  CHECK((all == std::vector<int>{0, 1, 2, 3, 4}));
+  bool threw = false;
+  try {
+    History::subsample_dots(5, 6, 1);
+  } catch(std::runtime_error &) {
+    threw = true;
+  }
+  CHECK(threw);
+  // 6 dots, 7 steps, every 3rd step of dots {1, 4}: records at steps 2 and 5;
+  // the state of dot i after step t is (t + i/10, -t) and (i, t/2)
+  for(int binary : {1, 0}) {
+    History::SampledOutput spec;
+    spec.path = binary ? "sampled_test.bin" : "sampled_test.txt";
+    spec.binary = binary;
+    spec.every = 3;
+    spec.dots = {1, 4};
+    History::SampledWriter w(spec, 6, 7, 1e-4);
+    CHECK(w.records() == 2);
+    std::vector<double> rho(4 * 6);
+    for(int t = 0; t < 7; ++t) {
+      for(int i = 0; i < 6; ++i) {
+        rho[4 * i] = t + i / 10.0;
+        rho[4 * i + 1] = -t;
+        rho[4 * i + 2] = i;
+        rho[4 * i + 3] = t / 2.0;
+      }
+      if(t % 3 == 2) w.write(t, rho.data());
+    }
+  }
+}
+
 int main()
 {
   resonant_rhs();
@@ -159,6 +204,7 @@ int main()
   file_formats();
   weights();
   output_format();
+  sampled_output();
   if(failures) std::fprintf(stderr, "%d check(s) failed\n", failures);
   else std::printf("mirror_test: all checks passed\n");
   return failures ? 1 : 0;

@@ -145,3 +145,72 @@ def test_all_pulses(oracle_lib, tmp_path, loop):
         print(f"{loop} loop, {len(used)} pulse(s): rel err {err:.3e}")
         assert np.abs(want[:, :, 1]).max() > 0.1
         assert err < 1e-10
+
+
+def _write_inputs(tmp_path, sysd, pl, steps, dt, omega, order=4):
+    n = len(sysd["pos"])
+    with open(tmp_path / "dots.cfg", "w") as f:
+        for i in range(n):
+            f.write(" ".join(repr(float(v)) for v in (*sysd["pos"][i], sysd["freq"][i], sysd["t1"][i],
+                                                       sysd["t2"][i], *sysd["dip"][i])) + "\n")
+    with open(tmp_path / "pulse.cfg", "w") as f:
+        f.write(" ".join(repr(float(v)) for v in (pl["amplitude"], pl["delay"], pl["width"], pl["freq"],
+                                                   *pl["wavevector"], *pl["polarization"])) + "\n")
+    with open(tmp_path / "input.cfg", "w") as f:
+        f.write(f"[parameters]\n num_particles = {n}\n total_time = {steps * dt!r}\n timestep = {dt!r}\n"
+                f" interpolation_order = {order}\n[constants]\n c0 = 299.792458\n hbar = 0.65821193\n"
+                f" mu0 = 2.0133545e-04\n laser_frequency = {omega!r}\n")
+
+
+@pytest.mark.timeout(300)
+def test_binary_subsample_n4096_matches_oracle_fixture(tmp_path):
+    """--format binary --every 500 --subsample 512:11 on the N = 4096 system
+    (dipoles along z, pi/2 pulse, all 5000 steps, device loop): the records
+    against the oracle trajectory of tests/golden/n4096_traj.npz at the same
+    steps and dots, 1e-10 relative (the fixture's PC weights are numpy's,
+    quest_hip's the library's; valid weight sets agree to ~1e-13); the text
+    form of the same options holds the same values to 12 digits."""
+    from quest_amd import outputs, systems
+    g = np.load(os.path.join(ROOT, "tests", "golden", "n4096_traj.npz"))
+    sysd, steps = systems.workload("n4096")
+    _write_inputs(tmp_path, sysd, systems.pulse(math.pi / 2), steps, 1e-4, systems.OMEGA0)
+    opts = ["--every", "500", "--subsample", "512:11"]
+    run_cli(tmp_path, opts + ["--format", "binary", "--output", "out.bin"])
+    b = outputs.read_binary(str(tmp_path / "out.bin"))
+    assert b["n_dots"] == 4096 and b["every"] == 500 and len(b["dots"]) == 512
+    assert list(b["steps"]) == [500 * (r + 1) - 1 for r in range(10)]
+    assert len(set(b["dots"].tolist())) == 512 and np.all(np.diff(b["dots"]) > 0)
+    want = g["rec"][:, b["dots"], :]
+    err = rel_err(b["rho"], want)
+    print(f"N=4096 binary subsample: 10 records x 512 dots, rel err {err:.3e}, "
+          f"max|rho01| {np.abs(want[:, :, 1]).max():.3f}")
+    assert np.abs(want[:, :, 1]).max() > 0.1
+    assert err < 1e-10
+    run_cli(tmp_path, opts + ["--output", "out.txt"])
+    txt = outputs.read_text(str(tmp_path / "out.txt"), 512)
+    assert txt.shape == b["rho"].shape
+    assert np.abs(txt - b["rho"]).max() <= 1e-11 * np.abs(b["rho"]).max()
+
+
+def test_sampled_output_host_loop(oracle_lib, tmp_path):
+    """The host loop (reference integrator, GPU interactions) writes the same
+    sampled records from its history: --every 7 --subsample 10:5 --format
+    binary on a 40-dot system against the oracle at those steps and dots."""
+    from quest_amd import outputs
+    n, steps, omega = 40, 300, 2278.9
+    sysd = dense_system(n, 0.2, seed=79)
+    sigma = 0.005
+    amp = (math.pi / 2) * 0.65821193 / (5.2917721e-4 * sigma * math.sqrt(2 * math.pi))
+    pl = {"amplitude": amp, "delay": 0.015, "width": omega * sigma, "freq": omega,
+          "wavevector": [omega / 299.792458, 0.0, 0.0], "polarization": [0.0, 0.0, 1.0]}
+    _write_inputs(tmp_path, sysd, pl, steps, 1e-4, omega)
+    run_cli(tmp_path, ["--host-loop", "--every", "7", "--subsample", "10:5", "--format", "binary",
+                       "--output", "out.bin"])
+    b = outputs.read_binary(str(tmp_path / "out.bin"))
+    assert list(b["steps"]) == list(range(6, steps, 7)) and len(b["dots"]) == 10
+    consts = {"c0": 299.792458, "hbar": 0.65821193, "mu0": 2.0133545e-04, "laser_omega": omega, "dt": 1e-4}
+    want = oracle_traj(oracle_lib, sysd, consts, pl, steps, 4, mode=1)[b["steps"]][:, b["dots"], :]
+    err = rel_err(b["rho"], want)
+    print(f"host loop sampled: {len(b['steps'])} records, rel err {err:.3e}")
+    assert np.abs(want[:, :, 1]).max() > 0.1
+    assert err < 1e-10

@@ -84,11 +84,22 @@ typedef struct {
                           miss L2, e.g. many small systems; slower on large
                           ones).  Same arithmetic, same results.  FP64
                           contexts use the plain loop.                       */
+  int accumulation;    /* mixed precision's per-target sum over a block of
+                          64 sources before it is folded into the FP64 sum:
+                          QUEST_ACC_BLOCK (plain FP32 block sums; the
+                          default) or QUEST_ACC_COMPENSATED (Kahan-compensated
+                          FP32 block sums: each addition's rounding error is
+                          carried in a second FP32 sum and the pair is folded
+                          into FP64 per block, so the block sum's error no
+                          longer grows with the block length).  FP64
+                          contexts ignore it.                                */
 } quest_params;
 
 #define QUEST_CLAIM_AUTO 0
 #define QUEST_CLAIM_STATIC 1
 #define QUEST_CLAIM_DYNAMIC 2
+#define QUEST_ACC_BLOCK 0
+#define QUEST_ACC_COMPENSATED 1
 #define QUEST_LOOP_AUTO 0
 #define QUEST_LOOP_PLAIN 1
 #define QUEST_LOOP_PIPELINED 2

@@ -45,6 +45,7 @@ EXPORTS = [
 PAIR_ON_THE_FLY, PAIR_TABLE = 0, 1  # QUEST_PAIR_*
 CLAIM_AUTO, CLAIM_STATIC, CLAIM_DYNAMIC = 0, 1, 2  # QUEST_CLAIM_*
 LOOP_AUTO, LOOP_PLAIN, LOOP_PIPELINED = 0, 1, 2     # QUEST_LOOP_*
+ACC_BLOCK, ACC_COMPENSATED = 0, 1                   # QUEST_ACC_*
 MAX_PULSES = 8  # QUEST_MAX_PULSES
 
 
@@ -65,7 +66,7 @@ class Params(C.Structure):
                 ("laser_omega", C.c_double), ("dt", C.c_double), ("interp_order", C.c_int),
                 ("history_window", C.c_int), ("precision", C.c_int), ("dyadic", C.c_int),
                 ("device", C.c_int), ("source_chunk", C.c_int),
-                ("item_claim", C.c_int), ("source_loop", C.c_int)]
+                ("item_claim", C.c_int), ("source_loop", C.c_int), ("accumulation", C.c_int)]
 
 
 class Pulse(C.Structure):
@@ -148,7 +149,8 @@ class Context:
     of a batch are the systems' arrays concatenated in order."""
 
     def __init__(self, system, consts, precision=PREC_FP64, dyadic=DYADIC_ROTATING,
-                 order=4, window=22, device=0, source_chunk=0, pulse=None, item_claim=0, source_loop=0):
+                 order=4, window=22, device=0, source_chunk=0, pulse=None, item_claim=0, source_loop=0,
+                 accumulation=0):
         L = load()
         self._L = L
         self.ptr = None
@@ -170,7 +172,7 @@ class Context:
         self.sizes = sizes
         p = Params(consts["c0"], consts["hbar"], consts["mu0"], consts["laser_omega"],
                    consts["dt"], order, window, precision, dyadic, device, source_chunk, item_claim,
-                   source_loop)
+                   source_loop, accumulation)
         out = C.c_void_p()
         if batch:
             arr = (C.c_int * len(sizes))(*sizes)

@@ -685,6 +685,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   k.half_phase_im = std::sin(-0.5 * k.theta);
   k.dyadic = p->dyadic;
   k.n_systems = n_sys;
+  k.compensated = p->precision == QUEST_PREC_MIXED && p->accumulation == QUEST_ACC_COMPENSATED;
   // FP32 positions are stored relative to the centre of all systems' boxes,
   // in units of c0 dt (so a pair's FP32 distance is its delay tau); each
   // coordinate is off by at most half an ulp of the largest scaled centred
@@ -863,6 +864,8 @@ int check_params(const quest_params *p, int n)
     return fail(nullptr, QUEST_ERR_ARG, "item_claim must be QUEST_CLAIM_AUTO, _STATIC or _DYNAMIC");
   if(p->source_loop < QUEST_LOOP_AUTO || p->source_loop > QUEST_LOOP_PIPELINED)
     return fail(nullptr, QUEST_ERR_ARG, "source_loop must be QUEST_LOOP_AUTO, _PLAIN or _PIPELINED");
+  if(p->accumulation != QUEST_ACC_BLOCK && p->accumulation != QUEST_ACC_COMPENSATED)
+    return fail(nullptr, QUEST_ERR_ARG, "accumulation must be QUEST_ACC_BLOCK or QUEST_ACC_COMPENSATED");
   return QUEST_OK;
 }
 

@@ -73,6 +73,7 @@ struct Consts {
   int n_work;         // pair-kernel work items (system, target tile, source chunk)
   int dyn_items;      // 1: pair-kernel waves claim items from a device counter (else a static stride)
   int pipelined;      // 1: the FP32 pair loop stages the next source ahead of this one's arithmetic
+  int compensated;    // 1: the FP32 pair loop's block sums are Kahan-compensated
   int max_chunks;     // largest number of source chunks of one system (rows of `partial`)
   // coefficient-table mode (the reference's precomputed per-pair coefficients)
   int pair_table;     // 1: the pair sum streams `table` instead of evaluating on the fly

@@ -21,6 +21,7 @@
 
 #include <climits>
 #include <mutex>
+#include <type_traits>
 
 #include "quest_internal.hpp"
 
@@ -358,10 +359,24 @@ template <typename R> struct PairK {
 // the ring).  GEN: e^{-i theta x} from the library sincos (theta beyond the
 // short series' range); a separate kernel variant, so the common one carries
 // none of it.
-template <typename R, bool LAB, bool GEN = false>
+// Kahan's compensated addition: s + c carries the sum of the added terms
+// with the rounding error of every addition kept in c (-c is the error).
+template <typename R> __device__ __forceinline__ void kahan_add(R &s, R &c, R x)
+{
+  const R y = x - c;
+  const R t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// COMP: the term is formed on its own and added to (accr, acci) by
+// kahan_add with compensations (cmpr, cmpi); else it is added in the same
+// fused operations that rotate it.
+template <typename R, bool LAB, bool GEN = false, bool COMP = false>
 __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R dz, const typename V4<R>::type &da,
                                           const typename V4<R>::type &dah, const typename V4<R>::type &db, R x,
-                                          R q0r, R q0i, R q1r, R q1i, R q2r, R q2i, R &accr, R &acci)
+                                          R q0r, R q0i, R q1r, R q1i, R q2r, R q2i, R &accr, R &acci, R &cmpr,
+                                          R &cmpi)
 {
   const R ir2 = ir * ir;
   const R ba = da.x * dx + da.y * dy + da.z * dz;
@@ -373,8 +388,13 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
   if constexpr(LAB) {
     const R g3r = fmaT(ir, q0r, -q1r);
     const R g3i = fmaT(ir, q0i, -q1i);
-    accr = fmaT(g3h, g3r, fmaT(g1, q2r, accr));
-    acci = fmaT(g3h, g3i, fmaT(g1, q2i, acci));
+    if constexpr(COMP) {
+      kahan_add(accr, cmpr, fmaT(g3h, g3r, g1 * q2r));
+      kahan_add(acci, cmpi, fmaT(g3h, g3i, g1 * q2i));
+    } else {
+      accr = fmaT(g3h, g3r, fmaT(g1, q2r, accr));
+      acci = fmaT(g3h, g3i, fmaT(g1, q2i, acci));
+    }
   } else {
     const R th = K.theta;
     const R g3r = fmaT(ir, q0r, fmaT(-th, q0i, -q1r));
@@ -395,8 +415,13 @@ __device__ __forceinline__ void pair_term(const PairK<R> &K, R ir, R dx, R dy, R
       else
         sincos_theta(K.sn, K.cs, x, sn, cs);
     }
-    accr = fmaT(cr, cs, fmaT(ci, sn, accr));
-    acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
+    if constexpr(COMP) {
+      kahan_add(accr, cmpr, fmaT(cr, cs, ci * sn));
+      kahan_add(acci, cmpi, fmaT(ci, cs, -(cr * sn)));
+    } else {
+      accr = fmaT(cr, cs, fmaT(ci, sn, accr));
+      acci = fmaT(ci, cs, fmaT(-cr, sn, acci));
+    }
   }
 }
 
@@ -497,7 +522,8 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
       for(int i = 0; i <= M; ++i) cv[i] = load_off(row + static_cast<size_t>(i) * E, off);
       double q0r, q0i, q1r, q1i, q2r, q2i;
       horner3<double, M>(cv, xc, q0r, q0i, q1r, q1i, q2r, q2i);
-      pair_term<double, LAB, GEN>(K, ir, dx, dy, dz, da, dah, db, xc, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci);
+      pair_term<double, LAB, GEN>(K, ir, dx, dy, dz, da, dah, db, xc, q0r, q0i, q1r, q1i, q2r, q2i, accr, acci,
+                                  accr, acci);
     }
     if(tgt < wi.y)
       partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
@@ -527,7 +553,7 @@ struct RingPlanesF32 {
   const float2 *last;     // plane of c_M (even M)
 };
 
-template <int M, bool LAB, bool DYN, bool GEN, bool PF>
+template <int M, bool LAB, bool DYN, bool GEN, bool PF, bool COMP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void pair_past_f32(
     const float4 *__restrict__ pd, const double4 *__restrict__ pos64, const RingPlanesF32 planes,
     const int4 *__restrict__ work, const int *__restrict__ work_row, int *step_ptr,
@@ -554,6 +580,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
     for(int bt = wi.z; bt < wi.w; bt += kChunk) {
       const int be = min(wi.w, bt + kChunk);
       float sr = 0, si = 0;
+      float cr = 0, ci = 0;  // Kahan compensations (COMP)
       // A source is staged (position and dipole: one scalar load of 32 B;
       // delay split; coefficient gathers) and then evaluated.  PF: software-
       // pipelined by one source -- source b + 1 is staged before source b's
@@ -583,8 +610,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
       auto evaluate = [&](const StagedF32<M> &st) {
         float q0r, q0i, q1r, q1i, q2r, q2i;
         horner3<float, M>(st.cv, st.x, q0r, q0i, q1r, q1i, q2r, q2i);
-        pair_term<float, LAB, GEN>(K, st.ir, st.dx, st.dy, st.dz, da, dah, st.db, st.x, q0r, q0i, q1r, q1i, q2r,
-                                   q2i, sr, si);
+        pair_term<float, LAB, GEN, COMP>(K, st.ir, st.dx, st.dy, st.dz, da, dah, st.db, st.x, q0r, q0i, q1r, q1i,
+                                         q2r, q2i, sr, si, cr, ci);
       };
       StagedF32<M> cur;
       if constexpr(PF) {
@@ -601,8 +628,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
           evaluate(cur);
         }
       }
-      accr += static_cast<double>(sr);
-      acci += static_cast<double>(si);
+      if constexpr(COMP) {  // the compensated block sum is s - c
+        accr += static_cast<double>(sr) - static_cast<double>(cr);
+        acci += static_cast<double>(si) - static_cast<double>(ci);
+      } else {
+        accr += static_cast<double>(sr);
+        acci += static_cast<double>(si);
+      }
     }
     if(tgt < wi.y)
       partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
@@ -1326,12 +1358,19 @@ void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t
     RingPlanesF32 planes{};
     for(int i = 0; i < (M + 1) / 2; ++i) planes.pair[i] = r4 + i * E;
     planes.last = reinterpret_cast<const float2 *>(r4 + ((M + 1) / 2) * E);
-    if(k.pipelined)
-      hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, true>), grid, block, 0, s, d.pd32, d.pos64, planes,
-                         d.work, d.work_row, d.step, d.partial, k);
+    if(k.compensated) {
+      if(k.pipelined)
+        hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, true, true>), grid, block, 0, s, d.pd32, d.pos64,
+                           planes, d.work, d.work_row, d.step, d.partial, k);
+      else
+        hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, false, true>), grid, block, 0, s, d.pd32, d.pos64,
+                           planes, d.work, d.work_row, d.step, d.partial, k);
+    } else if(k.pipelined)
+      hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, true, false>), grid, block, 0, s, d.pd32, d.pos64,
+                         planes, d.work, d.work_row, d.step, d.partial, k);
     else
-      hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, false>), grid, block, 0, s, d.pd32, d.pos64, planes,
-                         d.work, d.work_row, d.step, d.partial, k);
+      hipLaunchKernelGGL((pair_past_f32<M, LAB, DYN, GEN, false, false>), grid, block, 0, s, d.pd32, d.pos64,
+                         planes, d.work, d.work_row, d.step, d.partial, k);
   }
 }
 
@@ -1365,24 +1404,29 @@ hipError_t launch_table_m(const Consts &k, DeviceState &d, hipStream_t s)
 }
 
 template <int M>
-int blocks_per_cu_m(int precision, bool table, bool gen, bool pf)
+int blocks_per_cu_m(int precision, bool table, bool gen, bool pf, bool comp)
 {
   int nb = 0;
   hipError_t e;
+  // the FP32 variant of the context's loop (plain / pipelined, block /
+  // compensated sums, short series / library sincos)
+  auto f32 = [&](auto pf_c, auto comp_c) {
+    constexpr bool P = decltype(pf_c)::value, C = decltype(comp_c)::value;
+    return gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, true, P, C>, kBlock, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, false, P, C>, kBlock,
+                                                              0);
+  };
+  using T = std::true_type;
+  using F = std::false_type;
   if(table)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_table_f64<M, false>, kBlock, 0);
   else if(precision == QUEST_PREC_FP64)
     e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false, true>, kBlock, 0)
             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f64<M, false, false, false>, kBlock, 0);
   else if(pf)
-    e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, true, true>, kBlock, 0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, false, true>, kBlock,
-                                                           0);
+    e = comp ? f32(T{}, T{}) : f32(T{}, F{});
   else
-    e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, true, false>, kBlock,
-                                                           0)
-            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_past_f32<M, false, false, false, false>, kBlock,
-                                                           0);
+    e = comp ? f32(F{}, T{}) : f32(F{}, F{});
   return e == hipSuccess && nb > 0 ? nb : 1;
 }
 
@@ -1469,14 +1513,14 @@ int pair_blocks_per_cu(const Consts &k, int precision)
 {
   const bool t = k.pair_table != 0;
   const bool g = k.dyadic != QUEST_DYADIC_LAB && !k.trig_poly;
-  const bool pf = k.pipelined != 0;
+  const bool pf = k.pipelined != 0, cp = k.compensated != 0;
   switch(k.order) {
-  case 1: return blocks_per_cu_m<1>(precision, t, g, pf);
-  case 2: return blocks_per_cu_m<2>(precision, t, g, pf);
-  case 3: return blocks_per_cu_m<3>(precision, t, g, pf);
-  case 4: return blocks_per_cu_m<4>(precision, t, g, pf);
-  case 5: return blocks_per_cu_m<5>(precision, t, g, pf);
-  default: return blocks_per_cu_m<6>(precision, t, g, pf);
+  case 1: return blocks_per_cu_m<1>(precision, t, g, pf, cp);
+  case 2: return blocks_per_cu_m<2>(precision, t, g, pf, cp);
+  case 3: return blocks_per_cu_m<3>(precision, t, g, pf, cp);
+  case 4: return blocks_per_cu_m<4>(precision, t, g, pf, cp);
+  case 5: return blocks_per_cu_m<5>(precision, t, g, pf, cp);
+  default: return blocks_per_cu_m<6>(precision, t, g, pf, cp);
   }
 }
 

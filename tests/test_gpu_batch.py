@@ -161,14 +161,15 @@ def test_dynamic_item_claiming_is_bitwise_static(ref_weights, precision):
     assert rel_err(second[CLAIM_DYNAMIC], second[CLAIM_STATIC]) < (1e-12 if precision == 0 else 1e-8)
 
 
-@pytest.mark.parametrize("batch,dyadic,dt", [(False, 0, 1e-4), (True, 0, 1e-4), (False, 1, 1e-4),
-                                              (False, 0, 1.3e-4)])
-def test_pipelined_source_loop_is_bitwise_plain(ref_weights, batch, dyadic, dt):
+@pytest.mark.parametrize("batch,dyadic,dt,acc", [(False, 0, 1e-4, 0), (True, 0, 1e-4, 0), (False, 1, 1e-4, 0),
+                                                  (False, 0, 1.3e-4, 0), (False, 0, 1e-4, 1), (True, 1, 1e-4, 1)])
+def test_pipelined_source_loop_is_bitwise_plain(ref_weights, batch, dyadic, dt, acc):
     """The mixed-precision pair kernel's two source loops (plain; staged one
     source ahead) do the same arithmetic in the same order: whole runs agree
     bit for bit, for one system and for a batch, in the rotating and the lab
-    frame, and with theta = omega dt beyond the short rotation series (the
-    library-sincos kernel variant, dt = 1.3e-4 ps)."""
+    frame, with theta = omega dt beyond the short rotation series (the
+    library-sincos kernel variant, dt = 1.3e-4 ps), and with Kahan-compensated
+    block sums (acc = 1)."""
     from quest_amd import LOOP_PIPELINED, LOOP_PLAIN, Context, systems
     consts = systems.consts(dt=dt)
     if batch:
@@ -177,7 +178,8 @@ def test_pipelined_source_loop_is_bitwise_plain(ref_weights, batch, dyadic, dt):
         sys_list, pulses = dense_system(300, 0.3, seed=71), short_pulse()
     rho = {}
     for loop in (LOOP_PLAIN, LOOP_PIPELINED):
-        with Context(sys_list, consts, precision=1, dyadic=dyadic, pulse=pulses, source_loop=loop) as ctx:
+        with Context(sys_list, consts, precision=1, dyadic=dyadic, pulse=pulses, source_loop=loop,
+                     accumulation=acc) as ctx:
             assert ctx.info()["pipelined"] == (loop == LOOP_PIPELINED)
             ctx.set_pc_weights(*ref_weights)
             ctx.run(150)

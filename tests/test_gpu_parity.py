@@ -76,6 +76,39 @@ def test_hist_evaluate_matches_oracle(oracle_lib, ref_weights, precision, dyadic
     assert worst < tol
 
 
+@pytest.mark.parametrize("dyadic", [0, 1])
+def test_compensated_block_sums_match_oracle(oracle_lib, ref_weights, dyadic):
+    """QUEST_ACC_COMPENSATED (Kahan-compensated FP32 block sums folded into
+    FP64): per-call fields of N = 1000 against the oracle within the mixed
+    tolerance 2e-6, and within 1e-6 of the plain block sums -- both errors are
+    set by the FP32 arithmetic of each pair's term, which compensation of the
+    sums does not touch; a whole driven run within 1e-6 of FP64."""
+    from quest_amd import ACC_COMPENSATED, Context, systems
+    oracle = oracle_lib
+    consts = systems.consts()
+    n, steps = 1000, 60
+    sysd = dense_system(n, 1.2, seed=1011)
+    rho01 = _physical_history(oracle, sysd, consts, steps, ref_weights)
+    p = oracle_params(oracle, consts)
+    delays, coef = oracle.build_table(p, sysd["pos"], sysd["dip"], kind=dyadic)
+    y = _oracle_y(rho01)
+    got = {}
+    for acc in (0, ACC_COMPENSATED):
+        with Context(sysd, consts, precision=1, dyadic=dyadic, accumulation=acc) as ctx:
+            for s in range(steps):
+                got[acc] = ctx.hist_evaluate(s, rho01[:, s])
+    want = oracle.evaluate_history(p, delays, coef, y, steps - 1, kind=dyadic)
+    e_c, e_b = rel_err(got[ACC_COMPENSATED], want), rel_err(got[0], want)
+    print(f"dyadic={dyadic}: compensated {e_c:.3e}, block {e_b:.3e} vs oracle")
+    assert e_c < 2e-6 and rel_err(got[ACC_COMPENSATED], got[0]) < 1e-6
+    pl = short_pulse()
+    r = {}
+    for prec, acc in ((0, 0), (1, ACC_COMPENSATED)):
+        r[prec] = _run_gpu(sysd, consts, pl, ref_weights, 150, precision=prec, accumulation=acc)[0]
+    assert np.abs(r[0][:, 1]).max() > 0.05
+    assert np.abs(r[1] - r[0]).max() < 1e-6
+
+
 def test_pulse_matches_oracle(oracle_lib):
     from quest_amd import Context, systems
     oracle = oracle_lib
@@ -145,10 +178,10 @@ def test_pulse_list_run_matches_oracle(oracle_lib, ref_weights, precision, tol):
     assert err < tol
 
 
-def _run_gpu(sysd, consts, pl, weights, steps, precision=0, chunks=1):
+def _run_gpu(sysd, consts, pl, weights, steps, precision=0, chunks=1, accumulation=0):
     from quest_amd import Context
     ps, cs, fc = weights
-    with Context(sysd, consts, precision=precision, pulse=pl) as ctx:
+    with Context(sysd, consts, precision=precision, pulse=pl, accumulation=accumulation) as ctx:
         ctx.set_pc_weights(ps, cs, fc)
         per = steps // chunks
         for _ in range(chunks):

@@ -77,14 +77,18 @@ def timed_steps(ctx, warmup, steps):
             "graph_ms_per_step": 1e3 * graph_secs / max(steps, 1)}
 
 
-def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0):
-    """One context on the workload; see timed_steps."""
+def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0, tune=False):
+    """One context on the workload with the library's default work split (or,
+    tune, the autotuner's: its few-launch timings can favour splits that lose
+    in sustained runs, so the default line does not use it); see timed_steps."""
     from quest_amd import Context
     ps, cs, fc = weights
     with Context(sysd, consts, precision=precision, pulse=pulse) as ctx:
         if method:
             ctx.set_pair_method(method)
         ctx.set_pc_weights(ps, cs, fc)
+        if tune and not method:
+            ctx.autotune(3)
         info = ctx.info()
         out = timed_steps(ctx, warmup, steps)
         rho, _ = ctx.state()
@@ -134,7 +138,7 @@ def ensemble_main(args):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK, "traffic": None, "fma_issue_ceiling": FMA_CEILING,
                      "frac_of_fma_issue_ceiling": achieved / FMA_CEILING,
-                     "kernel": "pair_past_f32<4,false,{},false,{}>".format(
+                     "kernel": "pair_past_f32<4, false, {}, false, {}, false>".format(
                          "true" if mixed["info"]["dynamic_items"] else "false",
                          "true" if mixed["info"]["pipelined"] else "false"),
                      "kernel_ms": mixed["pair_ms"], "flop_per_pair": F_ALG},
@@ -407,7 +411,8 @@ def main(argv=None) -> int:
     ms_per_step = 1e3 * mixed["seconds"] / args.steps
     ordered = float(n) * (n - 1)
     achieved = F_ALG * ordered / (mixed["pair_ms"] * 1e-3) / 1e12
-    kpf = "true" if mixed["info"]["pipelined"] else "false"
+    kname = "pair_past_f32<4, false, {}, false, {}, false>".format(
+        "true" if mixed["info"]["dynamic_items"] else "false", "true" if mixed["info"]["pipelined"] else "false")
     out = {
         "metric": f"pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N={n}, ρ err≤1e-6",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
@@ -420,14 +425,16 @@ def main(argv=None) -> int:
                    "source_chunk": mixed["info"]["source_chunk"],
                    "source_chunks_per_tile": mixed["info"]["splits"],
                    "persistent_grid": mixed["info"]["grid"],
+                   "pipelined_loop": mixed["info"]["pipelined"],
+                   "dynamic_items": mixed["info"]["dynamic_items"],
+                   "autotuned": False,
                    "ring_slots": mixed["info"]["ring_slots"],
                    "zero_delay_pairs": mixed["info"]["near_pairs"]},
         "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK,
-                     "traffic": (profiled_traffic("pair_past_f32<4, false, false, false, {}>".format(kpf))
-                                 if args.workload == "n65536" else None),
-                     "kernel": "pair_past_f32<4,false,false,false,{}>".format(kpf), "kernel_ms": mixed["pair_ms"],
+                     "traffic": profiled_traffic(kname) if args.workload == "n65536" else None,
+                     "kernel": kname, "kernel_ms": mixed["pair_ms"],
                      "flop_per_pair": F_ALG, "fma_issue_ceiling": FMA_CEILING,
                      "frac_of_fma_issue_ceiling": achieved / FMA_CEILING},
     }
@@ -437,7 +444,9 @@ def main(argv=None) -> int:
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
                        "graph_ms_per_step": fp64["graph_ms_per_step"],
                        "pair_kernel_ms": fp64["pair_ms"], "achieved_tflops": a64,
-                       "frac_of_fp64_peak": a64 / FP64_PEAK}
+                       "frac_of_fp64_peak": a64 / FP64_PEAK,
+                       "source_chunk": fp64["info"]["source_chunk"], "persistent_grid": fp64["info"]["grid"],
+                       "dynamic_items": fp64["info"]["dynamic_items"]}
         out["mixed_over_fp64"] = value / v64
         out["max_abs_drho_vs_fp64"] = float(np.abs(mixed["rho"] - fp64["rho"]).max())
         out["max_abs_rho01"] = float(np.abs(fp64["rho"][:, 1]).max())

@@ -65,13 +65,15 @@ typedef struct {
   int dyadic;          /* QUEST_DYADIC_*                                    */
   int device;          /* HIP device ordinal                                */
   int source_chunk;    /* sources per pair-kernel work item; 0 = default
-                          (8 chunks of the largest system, halved until the
+                          (8 chunks of the largest system, for mixed
+                          precision at most 4096 sources, halved until the
                           work list fills the GPU); see quest_autotune      */
   int item_claim;      /* how the pair kernel's waves take work items:
                           QUEST_CLAIM_AUTO (static until quest_autotune,
                           which times both and keeps the faster),
                           QUEST_CLAIM_STATIC (a fixed stride over an
-                          XCD-aware list: a batch's systems each on one XCD)
+                          XCD-aware list: each (system, source chunk) pair
+                          on one XCD)
                           or QUEST_CLAIM_DYNAMIC (a device counter; balances
                           items of uneven size, in list order)               */
   int source_loop;     /* mixed precision's pair loop over an item's sources:
@@ -285,7 +287,8 @@ int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
  * source-chunk lengths around the default, each with static and dynamic item
  * claiming (only the one item_claim names, unless QUEST_CLAIM_AUTO) and, in
  * mixed precision, the plain and the pipelined source loop (only the one
- * source_loop names, unless QUEST_LOOP_AUTO), and keeps the fastest; returns
+ * source_loop names, unless QUEST_LOOP_AUTO), each with a persistent grid of
+ * all resident blocks and of two thirds of them, and keeps the fastest; returns
  * its chunk length in *chunk_len.  Changes no simulation state.  No
  * counterpart in the reference, which has no tunable kernel. */
 int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);

@@ -1249,7 +1249,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   // every step recomputes from scratch, so timing it between steps changes no
   // state; the device step counter is restored below.
   HIPC(c, launch_set_step(c->d, std::max(c->cur_step, 0), c->stream));
-  int best = c->chunk_len, best_dyn = c->k.dyn_items, best_pf = c->k.pipelined;
+  int best = c->chunk_len, best_dyn = c->k.dyn_items, best_pf = c->k.pipelined, best_occ = 0;
   const bool mixed = c->P.precision == QUEST_PREC_MIXED;
   float best_ms = INFINITY;
   struct Events {  // destroyed on every return path
@@ -1263,15 +1263,23 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   HIPC(c, hipEventCreate(&ev.e0));
   HIPC(c, hipEventCreate(&ev.e1));
   hipEvent_t e0 = ev.e0, e1 = ev.e1;
+  // occ 1: a persistent grid of two thirds of the resident blocks (fewer waves
+  // per SIMD share the L1: measured faster for the FP64 kernel at N = 65536,
+  // slower at N = 131072)
   for(int len : cand) {
     for(int dyn : {0, 1})
-    for(int pf : {0, 1}) {
+    for(int pf : {0, 1})
+    for(int occ : {0, 1}) {
       if(c->P.item_claim != QUEST_CLAIM_AUTO && dyn != (c->P.item_claim == QUEST_CLAIM_DYNAMIC)) continue;
       if(pf && !mixed) continue;
       if(mixed && c->P.source_loop != QUEST_LOOP_AUTO && pf != (c->P.source_loop == QUEST_LOOP_PIPELINED)) continue;
       c->k.dyn_items = dyn;
       c->k.pipelined = pf;
       c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
+      if(occ) {
+        if(c->blocks_per_cu < 3) continue;
+        c->blocks_per_cu = c->blocks_per_cu * 2 / 3;
+      }
       int rc = build_work(c, len);  // the list's layout depends on the claiming mode
       if(rc != QUEST_OK) return rc;
       HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
@@ -1286,12 +1294,14 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
         best = len;
         best_dyn = dyn;
         best_pf = pf;
+        best_occ = occ;
       }
     }
   }
   c->k.dyn_items = best_dyn;
   c->k.pipelined = best_pf;
   c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
+  if(best_occ) c->blocks_per_cu = c->blocks_per_cu * 2 / 3;
   int rc = build_work(c, best);
   if(rc != QUEST_OK) return rc;
   if(c->mode == 1 && c->cur_step >= 0) {

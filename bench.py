@@ -50,12 +50,24 @@ FP64_PEAK = 78.6       # TFLOP/s, MI355X vector FP64
 FMA_CEILING = 78.6
 
 
+def fill_warmup(ctx, order=4):
+    """Warm-up steps that fill the ring's active window with computed history:
+    every pair then reads a slot written by the run, as in a long simulation.
+    The pair kernels take the same cycles either way, but with slots still at
+    their cleared zeros the GPU holds a higher clock under its power limit
+    (measured at N = 65536: 2.38 GHz with an all-zero ring, ~2.07 GHz once
+    it is populated; 6.5 against 7.5 ms per mixed launch)."""
+    return max(10, int(ctx.info()["max_delay_bound"]) + order + 2)
+
+
 def timed_steps(ctx, warmup, steps):
-    """Warm-up, then the timed region: `steps` steps launched kernel by kernel
-    with HIP events on the context's stream around every pair-kernel launch
-    (wall time and the pair kernel's mean duration over the same steps); then
-    `steps` steps replayed as one captured graph per step (captured outside
-    the timed loop)."""
+    """Warm-up (warmup < 0: fill_warmup), then the timed region: `steps` steps
+    launched kernel by kernel with HIP events on the context's stream around
+    every pair-kernel launch (wall time and the pair kernel's mean duration
+    over the same steps); then `steps` steps replayed as one captured graph
+    per step (captured outside the timed loop)."""
+    if warmup < 0:
+        warmup = fill_warmup(ctx)
     if warmup:
         ctx.run(warmup)
     ctx.sync()
@@ -74,13 +86,13 @@ def timed_steps(ctx, warmup, steps):
     ctx.sync()
     graph_secs = time.perf_counter() - t0
     return {"seconds": secs, "pair_ms": kms / max(launches, 1), "launches": launches,
-            "graph_ms_per_step": 1e3 * graph_secs / max(steps, 1)}
+            "graph_ms_per_step": 1e3 * graph_secs / max(steps, 1), "warmup": warmup}
 
 
 def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0, tune=False):
-    """One context on the workload with the library's default work split (or,
-    tune, the autotuner's: its few-launch timings can favour splits that lose
-    in sustained runs, so the default line does not use it); see timed_steps."""
+    """One context on the workload; with tune (the on-the-fly method only) the
+    work split is the autotuner's (quest_autotune, 3 launches per candidate,
+    before the warm-up), else the library's default; see timed_steps."""
     from quest_amd import Context
     ps, cs, fc = weights
     with Context(sysd, consts, precision=precision, pulse=pulse) as ctx:
@@ -124,7 +136,7 @@ def ensemble_main(args):
     out = {
         "metric": "pair-interactions/s (sum of N^2 * steps / s), 256 batched systems",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * mixed["seconds"] / args.steps,
+        "warmup": mixed["warmup"], "ms_per_step": 1e3 * mixed["seconds"] / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 pair arithmetic + f64 accumulation/state", "data": "synthetic",
         "config": {"workload": "ensemble: 256 systems, N log-uniform in [200, 2000] at 512 um^-3, "
@@ -181,7 +193,7 @@ def table_main(args):
     out = {
         "metric": "pair-interactions/s (N²·steps/s), FP64 coefficient-table method",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * tab["seconds"] / args.steps, "higher_is_better": True,
+        "warmup": tab["warmup"], "ms_per_step": 1e3 * tab["seconds"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.workload}: N={n} dots, table of {table_bytes / 1e9:.2f} GB, "
                                f"{full_steps}-step run", "n_dots": n},
@@ -375,10 +387,15 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=-1,
+                    help="untimed steps before the timed ones; -1 (default): enough to fill the "
+                         "interpolant ring's active window (max delay + order + 2 steps)")
     ap.add_argument("--workload", default="n65536")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--tune", action="store_true",
+                    help="quest_autotune the work split before the warm-up (default: the library's "
+                         "default split; measured within 1 %% of the tuned one in sustained runs at N = 65536)")
     ap.add_argument("--no-driven", action="store_true",
                     help="skip the driven-state (short strong pulse) mixed-vs-FP64 error runs")
     ap.add_argument("--pair-method", choices=("on-the-fly", "table"), default="on-the-fly",
@@ -403,8 +420,9 @@ def main(argv=None) -> int:
 
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
-    mixed = run_gpu(sysd, consts, pulse, 1, args.warmup, args.steps, weights)
-    fp64 = None if args.no_fp64 else run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights)
+    tune = args.tune
+    mixed = run_gpu(sysd, consts, pulse, 1, args.warmup, args.steps, weights, tune=tune)
+    fp64 = None if args.no_fp64 else run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights, tune=tune)
 
     pairs = float(n) * n
     value = pairs * args.steps / mixed["seconds"]
@@ -416,7 +434,7 @@ def main(argv=None) -> int:
     out = {
         "metric": f"pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N={n}, ρ err≤1e-6",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "warmup": mixed["warmup"], "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair arithmetic + f64 accumulation/state",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: N={n} dots at 512 um^-3, random dipoles, "
@@ -427,7 +445,7 @@ def main(argv=None) -> int:
                    "persistent_grid": mixed["info"]["grid"],
                    "pipelined_loop": mixed["info"]["pipelined"],
                    "dynamic_items": mixed["info"]["dynamic_items"],
-                   "autotuned": False,
+                   "autotuned": tune,
                    "ring_slots": mixed["info"]["ring_slots"],
                    "zero_delay_pairs": mixed["info"]["near_pairs"]},
         "graph_ms_per_step": mixed["graph_ms_per_step"],

@@ -65,9 +65,10 @@ typedef struct {
   int dyadic;          /* QUEST_DYADIC_*                                    */
   int device;          /* HIP device ordinal                                */
   int source_chunk;    /* sources per pair-kernel work item; 0 = default
-                          (8 chunks of the largest system, for mixed
-                          precision at most 4096 sources, halved until the
-                          work list fills the GPU); see quest_autotune      */
+                          (the largest system in 48 (mixed) or 24 (FP64)
+                          equal chunks of at least 128 / 256 sources,
+                          halved until the work list fills the GPU); see
+                          quest_autotune                                    */
   int item_claim;      /* how the pair kernel's waves take work items:
                           QUEST_CLAIM_AUTO (static until quest_autotune,
                           which times both and keeps the faster),
@@ -77,14 +78,12 @@ typedef struct {
                           or QUEST_CLAIM_DYNAMIC (a device counter; balances
                           items of uneven size, in list order)               */
   int source_loop;     /* mixed precision's pair loop over an item's sources:
-                          QUEST_LOOP_AUTO (pipelined when the largest system
-                          has at most 8192 dots, else plain, until
-                          quest_autotune times both), QUEST_LOOP_PLAIN, or
+                          QUEST_LOOP_AUTO (pipelined until quest_autotune
+                          times both), QUEST_LOOP_PLAIN, or
                           QUEST_LOOP_PIPELINED (the next source's delay split
                           and interpolant gathers issued before this
-                          source's arithmetic: faster where many ring reads
-                          miss L2, e.g. many small systems; slower on large
-                          ones).  Same arithmetic, same results.  FP64
+                          source's arithmetic, so their cache latency
+                          overlaps it).  Same arithmetic, same results.  FP64
                           contexts use the plain loop.                       */
   int accumulation;    /* mixed precision's per-target sum over a block of
                           64 sources before it is folded into the FP64 sum:
@@ -283,14 +282,15 @@ int quest_set_timing(quest_ctx *ctx, int enable);
 int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
                            int reset);
 
-/* Times the pair kernel (`launches` launches each, after a warm-up) for
- * source-chunk lengths around the default, each with static and dynamic item
- * claiming (only the one item_claim names, unless QUEST_CLAIM_AUTO) and, in
- * mixed precision, the plain and the pipelined source loop (only the one
- * source_loop names, unless QUEST_LOOP_AUTO), each with a persistent grid of
- * all resident blocks and of two thirds of them, and keeps the fastest; returns
- * its chunk length in *chunk_len.  Changes no simulation state.  No
- * counterpart in the reference, which has no tunable kernel. */
+/* Times the pair kernel (`launches` launches each, after a warm-up) and keeps
+ * the fastest configuration, in two stages: (1) source-chunk lengths (the
+ * default scaled by 1/4 to 4, and the largest system in 8, 16, 24, 32 or 48
+ * equal chunks) x a persistent grid of all resident blocks per CU, one fewer
+ * and two fewer, with static item claiming and the default source loop; (2)
+ * at the three fastest of those, dynamic claiming (unless item_claim fixes
+ * it) and, in mixed precision, the other source loop (unless source_loop
+ * fixes it).  Returns the chunk length in *chunk_len.  Changes no simulation
+ * state.  No counterpart in the reference, which has no tunable kernel. */
 int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);
 
 /* Integer facts about the context.  info: [16] int64 = n_dots, ring slots,

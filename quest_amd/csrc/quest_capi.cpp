@@ -327,19 +327,36 @@ int build_work(quest_ctx *c, int chunk_len)
   return QUEST_OK;
 }
 
-// Default chunk length: 8 chunks of the largest system (one source range per
-// XCD), for mixed precision at most 4096 sources (measured at N = 65536 and
-// 131072: 16-32 chunks 1-4 % faster than 8; FP64 fastest at 8), halved while
-// the work list has fewer items than resident blocks.
+// Default chunk length: the largest system cut into 48 (mixed) or 24 (FP64)
+// equal source ranges -- a multiple of the 8 XCDs, so that the static list
+// gives every XCD the same share of a large system -- of at least 128 (mixed)
+// or 256 (FP64) sources, halved while the work list has fewer items than
+// resident waves.  Measured on MI355X at N = 16384-131072 against 8 and 16
+// chunks: mixed 1-11 % faster at N >= 65536 and equal below; FP64 2-15 %
+// faster except at N = 32768 and 131072 (7 % slower there; quest_autotune
+// finds those).  A chunk count that is not a multiple of 8 (N = 49152 cut
+// into 4096-source chunks: 12) left half the XCDs with half the work: 1.5x.
 int default_chunk(const quest_ctx *c)
 {
   int nmax = 2;
   for(int s = 0; s < c->n_sys; ++s) nmax = std::max(nmax, c->sys_begin[s + 1] - c->sys_begin[s]);
-  int len = std::max(64, (nmax + 7) / 8);
-  if(c->P.precision == QUEST_PREC_MIXED) len = std::min(len, 4096);
+  const bool mixed = c->P.precision == QUEST_PREC_MIXED;
+  const int count = mixed ? 48 : 24, min_len = mixed ? 128 : 256;
+  int len = std::max(min_len, (nmax + count - 1) / count);
   const int resident = c->cu_count * c->blocks_per_cu * (kBlock / kWave);  // waves
   while(len > 64 && count_work(c, len) < resident) len = std::max(64, (len + 1) / 2);
   return len;
+}
+
+// Persistent grid of the pair kernels: the resident blocks per CU, one fewer
+// for the on-the-fly kernels (measured at N = 32768-131072: the FP64 kernel's
+// gathers and the pipelined FP32 loop share each CU's L1 with fewer waves and
+// run up to 11 % faster; quest_autotune also times the full grid and two
+// fewer).
+int default_blocks_per_cu(const quest_ctx *c)
+{
+  const int full = pair_blocks_per_cu(c->k, c->P.precision);
+  return c->pair_method == QUEST_PAIR_ON_THE_FLY && full >= 4 ? full - 1 : full;
 }
 
 // The three factor tables of e^{i theta s} (phase_at on the device): each
@@ -794,17 +811,13 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   int rc = upload_pulses(c);
   if(rc != QUEST_OK) return rc;
 
-  // FP32 source loop: pipelined for systems of at most 8192 dots, whose ring
-  // reads miss L2 often enough for the staging to pay (measured: N = 4096 and
-  // the 256-system ensemble faster, N = 16384 and 65536 slower), unless asked
-  // otherwise (quest_autotune times both)
-  int max_sys = 0;
-  for(int s = 0; s < n_sys; ++s) max_sys = std::max(max_sys, c->sys_begin[s + 1] - c->sys_begin[s]);
-  k.pipelined = p->precision == QUEST_PREC_MIXED &&
-                (p->source_loop == QUEST_LOOP_PIPELINED || (p->source_loop == QUEST_LOOP_AUTO && max_sys <= 8192));
+  // FP32 source loop: pipelined unless asked otherwise (quest_autotune times
+  // both; measured with the default grid and chunks: faster or equal at every
+  // size from N = 4096 and the 256-system ensemble to N = 131072)
+  k.pipelined = p->precision == QUEST_PREC_MIXED && p->source_loop != QUEST_LOOP_PLAIN;
   // pair-kernel work list sized for one wave of resident blocks; static
   // claiming unless asked otherwise (quest_autotune times both)
-  c->blocks_per_cu = pair_blocks_per_cu(k, p->precision);
+  c->blocks_per_cu = default_blocks_per_cu(c);
   k.dyn_items = p->item_claim == QUEST_CLAIM_DYNAMIC;
   rc = build_work(c, p->source_chunk > 0 ? p->source_chunk : default_chunk(c));
   if(rc != QUEST_OK) return rc;
@@ -1137,7 +1150,7 @@ int quest_set_pair_method(quest_ctx *c, int method)
     free_table(c);
   }
   c->pair_method = method;
-  c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
+  c->blocks_per_cu = default_blocks_per_cu(c);
   return build_work(c, c->chunk_len);
 }
 
@@ -1238,20 +1251,22 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   }
   int nmax = 2;
   for(int s = 0; s < c->n_sys; ++s) nmax = std::max(nmax, c->sys_begin[s + 1] - c->sys_begin[s]);
+  // chunk lengths: the default scaled by 1/4 .. 4, and the largest system cut
+  // into 8, 16, 24, 32 or 48 equal source ranges (multiples of the 8 XCDs)
   const int base = default_chunk(c);
   std::vector<int> cand;
-  for(int f : {-2, -1, 0, 1, 2}) {
-    int len = f >= 0 ? base << f : base >> -f;
+  auto add_len = [&](int len) {
     len = std::max(32, std::min(len, nmax));
     if(std::find(cand.begin(), cand.end(), len) == cand.end()) cand.push_back(len);
-  }
+  };
+  for(int f : {-2, -1, 0, 1, 2}) add_len(f >= 0 ? base << f : base >> -f);
+  for(int cnt : {8, 16, 24, 32, 48})
+    if(nmax / cnt >= 64) add_len((nmax + cnt - 1) / cnt);
   // The pair kernel reads the ring and writes only the partial sums, which
   // every step recomputes from scratch, so timing it between steps changes no
   // state; the device step counter is restored below.
   HIPC(c, launch_set_step(c->d, std::max(c->cur_step, 0), c->stream));
-  int best = c->chunk_len, best_dyn = c->k.dyn_items, best_pf = c->k.pipelined, best_occ = 0;
   const bool mixed = c->P.precision == QUEST_PREC_MIXED;
-  float best_ms = INFINITY;
   struct Events {  // destroyed on every return path
     hipEvent_t e0 = nullptr, e1 = nullptr;
     ~Events()
@@ -1262,47 +1277,66 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   } ev;
   HIPC(c, hipEventCreate(&ev.e0));
   HIPC(c, hipEventCreate(&ev.e1));
-  hipEvent_t e0 = ev.e0, e1 = ev.e1;
-  // occ 1: a persistent grid of two thirds of the resident blocks (fewer waves
-  // per SIMD share the L1: measured faster for the FP64 kernel at N = 65536,
-  // slower at N = 131072)
-  for(int len : cand) {
-    for(int dyn : {0, 1})
-    for(int pf : {0, 1})
-    for(int occ : {0, 1}) {
-      if(c->P.item_claim != QUEST_CLAIM_AUTO && dyn != (c->P.item_claim == QUEST_CLAIM_DYNAMIC)) continue;
-      if(pf && !mixed) continue;
-      if(mixed && c->P.source_loop != QUEST_LOOP_AUTO && pf != (c->P.source_loop == QUEST_LOOP_PIPELINED)) continue;
-      c->k.dyn_items = dyn;
-      c->k.pipelined = pf;
-      c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
-      if(occ) {
-        if(c->blocks_per_cu < 3) continue;
-        c->blocks_per_cu = c->blocks_per_cu * 2 / 3;
-      }
-      int rc = build_work(c, len);  // the list's layout depends on the claiming mode
+  struct Cfg {
+    int len, dyn, pf, occ;  // occ: resident blocks per CU left out of the persistent grid
+    float ms;
+  };
+  // occupancy is a candidate dimension: at fewer waves per SIMD the FP64
+  // kernel's gathers share the L1 with fewer waves (measured up to 11 % faster
+  // with 4 of 6 blocks per CU at N = 98304, slower at N = 131072), and the
+  // grid's last round of items fills differently
+  auto time_cfg = [&](Cfg &g) -> int {
+    c->k.dyn_items = g.dyn;
+    c->k.pipelined = g.pf;
+    c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision) - g.occ;
+    int rc = build_work(c, g.len);  // the list's layout depends on the claiming mode and the grid
+    if(rc != QUEST_OK) return rc;
+    HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
+    HIPC(c, hipEventRecord(ev.e0, c->stream));
+    for(int i = 0; i < launches; ++i) HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));
+    HIPC(c, hipEventRecord(ev.e1, c->stream));
+    HIPC(c, hipEventSynchronize(ev.e1));
+    HIPC(c, hipEventElapsedTime(&g.ms, ev.e0, ev.e1));
+    return QUEST_OK;
+  };
+  // claiming and source loop: the caller's choice, else static and (mixed) the
+  // default loop in the first stage
+  const bool dyn_fixed = c->P.item_claim != QUEST_CLAIM_AUTO;
+  const bool pf_fixed = !mixed || c->P.source_loop != QUEST_LOOP_AUTO;
+  const int dyn0 = c->P.item_claim == QUEST_CLAIM_DYNAMIC ? 1 : 0;
+  const int pf0 = mixed ? (c->P.source_loop == QUEST_LOOP_AUTO ? c->k.pipelined
+                                                               : c->P.source_loop == QUEST_LOOP_PIPELINED)
+                        : 0;
+  // stage 1: chunk length x occupancy (all resident blocks per CU, one and two fewer)
+  std::vector<Cfg> tried;
+  for(int len : cand)
+    for(int occ = 0; occ <= 2; ++occ) {
+      c->k.pipelined = pf0;
+      if(occ > 0 && pair_blocks_per_cu(c->k, c->P.precision) - occ < 2) continue;
+      Cfg g{len, dyn0, pf0, occ, 0.0f};
+      int rc = time_cfg(g);
       if(rc != QUEST_OK) return rc;
-      HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
-      HIPC(c, hipEventRecord(e0, c->stream));
-      for(int i = 0; i < launches; ++i) HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));
-      HIPC(c, hipEventRecord(e1, c->stream));
-      HIPC(c, hipEventSynchronize(e1));
-      float ms = 0;
-      HIPC(c, hipEventElapsedTime(&ms, e0, e1));
-      if(ms < best_ms) {
-        best_ms = ms;
-        best = len;
-        best_dyn = dyn;
-        best_pf = pf;
-        best_occ = occ;
-      }
+      tried.push_back(g);
     }
-  }
-  c->k.dyn_items = best_dyn;
-  c->k.pipelined = best_pf;
-  c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision);
-  if(best_occ) c->blocks_per_cu = c->blocks_per_cu * 2 / 3;
-  int rc = build_work(c, best);
+  std::stable_sort(tried.begin(), tried.end(), [](const Cfg &x, const Cfg &y) { return x.ms < y.ms; });
+  // stage 2: the other claiming mode and source loop at the three fastest
+  const size_t top = std::min<size_t>(3, tried.size());
+  Cfg best = tried[0];
+  for(size_t t = 0; t < top; ++t)
+    for(int dyn : {0, 1})
+      for(int pf : {0, 1}) {
+        if((dyn_fixed && dyn != dyn0) || (pf_fixed && pf != pf0) || (dyn == dyn0 && pf == pf0)) continue;
+        Cfg g{tried[t].len, dyn, pf, tried[t].occ, 0.0f};
+        c->k.pipelined = pf;
+        if(g.occ > 0 && pair_blocks_per_cu(c->k, c->P.precision) - g.occ < 2) continue;
+        int rc = time_cfg(g);
+        if(rc != QUEST_OK) return rc;
+        if(g.ms < best.ms) best = g;
+      }
+  c->k.dyn_items = best.dyn;
+  c->k.pipelined = best.pf;
+  c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision) - best.occ;
+  int rc = build_work(c, best.len);
   if(rc != QUEST_OK) return rc;
   if(c->mode == 1 && c->cur_step >= 0) {
     // per-call mode keeps the current step's pair sum in `field`: recompute it
@@ -1311,7 +1345,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
     HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
-  if(chunk_len) *chunk_len = best;
+  if(chunk_len) *chunk_len = best.len;
   return QUEST_OK;
 }
 

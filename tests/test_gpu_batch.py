@@ -189,9 +189,9 @@ def test_pipelined_source_loop_is_bitwise_plain(ref_weights, batch, dyadic, dt, 
 
 
 def test_source_loop_default_and_autotune(ref_weights):
-    """QUEST_LOOP_AUTO: pipelined in mixed precision for systems of at most
-    8192 dots, plain above and in FP64; the autotuner may switch it and
-    changes no state; a pinned loop stays pinned."""
+    """QUEST_LOOP_AUTO: pipelined in mixed precision at every size, plain in
+    FP64; the autotuner may switch it and changes no state; a pinned loop
+    stays pinned."""
     from quest_amd import LOOP_PLAIN, Context, systems
     consts = systems.consts()
     small = dense_system(400, 0.35, seed=72)
@@ -201,6 +201,8 @@ def test_source_loop_default_and_autotune(ref_weights):
     with Context(small, consts, precision=0) as ctx:
         assert ctx.info()["pipelined"] == 0
     with Context(large, consts, precision=1) as ctx:
+        assert ctx.info()["pipelined"] == 1
+    with Context(large, consts, precision=1, source_loop=LOOP_PLAIN) as ctx:
         assert ctx.info()["pipelined"] == 0
     with Context(small, consts, precision=1, source_loop=LOOP_PLAIN) as ctx:
         ctx.set_pc_weights(*ref_weights)

@@ -393,6 +393,10 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="n65536")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--zero-drive", action="store_true",
+                    help="diagnostic: the pulse amplitude set to 0, so rho01 and every interpolant-ring "
+                         "slot stay exactly 0 (the same kernels and cycles; shows the GPU clock's "
+                         "dependence on the data under its power limit)")
     ap.add_argument("--tune", action="store_true",
                     help="quest_autotune the work split before the warm-up (default: the library's "
                          "default split; measured within 1 %% of the tuned one in sustained runs at N = 65536)")
@@ -415,6 +419,8 @@ def main(argv=None) -> int:
     sysd, full_steps = systems.workload(args.workload)
     consts = systems.consts()
     pulse = systems.pulse(math.pi / 2)
+    if args.zero_drive:
+        pulse = dict(pulse, amplitude=0.0)
     weights = pc_weights(18, 22, 3.15)
     n = sysd["pos"].shape[0]
 
@@ -446,6 +452,8 @@ def main(argv=None) -> int:
                    "pipelined_loop": mixed["info"]["pipelined"],
                    "dynamic_items": mixed["info"]["dynamic_items"],
                    "autotuned": tune,
+                   "drive": "none: pulse amplitude 0, every ring slot holds zeros" if args.zero_drive
+                            else "pi/2 pulse",
                    "ring_slots": mixed["info"]["ring_slots"],
                    "zero_delay_pairs": mixed["info"]["near_pairs"]},
         "graph_ms_per_step": mixed["graph_ms_per_step"],

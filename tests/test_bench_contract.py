@@ -12,8 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_profiled_traffic_matches_bench_labels():
     sys.path.insert(0, ROOT)
     import bench
-    # the N = 65536 line's kernel (plain source loop: the default above 8192 dots)
-    t32 = bench.profiled_traffic("pair_past_f32<4, false, false, false, false, false>")
+    # the N = 65536 line's kernel (the default pipelined source loop)
+    t32 = bench.profiled_traffic("pair_past_f32<4, false, false, false, true, false>")
     assert t32 is not None and 5e8 < t32 < 1e10
     ttab = bench.profiled_traffic("pair_table_f64<4, false>", "pair_table_n16384.md")
     assert ttab is not None and ttab > 1e10

@@ -230,18 +230,18 @@ int quest_run(quest_ctx *ctx, int n_steps);
  * (tight serving loops, harnesses that forbid hipMalloc between steps).  It
  * calls no hipMalloc, hipFree or other allocating routine: the same kernels as
  * quest_run are launched one by one (no graph is instantiated), and the pair
- * kernel's partial sums -- (source chunks + 1) x n x 16 B, e.g. 17 x 65536 x
- * 16 B = 17.8 MB for mixed precision at the default chunk length (4096
- * sources), 9 x 65536 x 16 B for FP64 (8 chunks) -- go to a static device
- * array of QUEST_SCRATCH_BYTES that each GPU the library runs on holds until
- * the process ends, instead of the context's buffer.  Results are those of
- * quest_run, bit for bit.  QUEST_ERR_ARG if the partial sums do not fit (at
- * the default chunk lengths: mixed above ~92k dots, FP64 above ~233k; longer
- * chunks via quest_params.source_chunk shrink them).  Every call on one GPU shares that GPU's array:
+ * kernel's partial sums -- (source chunks + 1) x n x 16 B, e.g. 49 x 65536 x
+ * 16 B = 51.4 MB for mixed precision at the default split (48 chunks of 1366
+ * sources), 25 x 65536 x 16 B = 26.2 MB for FP64 (24 chunks) -- go to a
+ * static device array of QUEST_SCRATCH_BYTES that each GPU the library runs
+ * on holds until the process ends, instead of the context's buffer.  Results
+ * are those of quest_run, bit for bit.  QUEST_ERR_ARG if the partial sums do
+ * not fit (at the default split: mixed above ~85k dots, FP64 above ~167k;
+ * longer chunks via quest_params.source_chunk shrink them).  Every call on one GPU shares that GPU's array:
  * calls that overlap in time on one GPU (from several host threads) race and
  * must be serialised by the caller; a call has finished with the array when
  * it returns.  The pair kernel is not timed (quest_set_timing). */
-#define QUEST_SCRATCH_BYTES (32u << 20)
+#define QUEST_SCRATCH_BYTES (64u << 20)
 int quest_run_noalloc(quest_ctx *ctx, int n_steps);
 
 /* Step the next quest_run/quest_hist_evaluate call works on. */

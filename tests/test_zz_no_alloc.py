@@ -56,8 +56,8 @@ def test_noalloc_matches_run_on_a_batch():
 def test_noalloc_rejects_partial_sums_beyond_the_scratch():
     from quest_amd import Context, QuestError, systems
     sysd = dense_system(4096, 2.0, seed=5)
-    # 4-source chunks: 1025 partial rows x 4096 dots x 16 B = 67 MB > 32 MiB
-    with Context(sysd, systems.consts(), precision=1, source_chunk=4,
+    # 2-source chunks: 2049 partial rows x 4096 dots x 16 B = 134 MB > 64 MiB
+    with Context(sysd, systems.consts(), precision=1, source_chunk=2,
                  pulse=short_pulse(delay=0.004, sigma_t=0.002)) as ctx:
         with pytest.raises(QuestError, match="static scratch"):
             ctx.run_noalloc(5)

@@ -1288,7 +1288,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   auto time_cfg = [&](Cfg &g) -> int {
     c->k.dyn_items = g.dyn;
     c->k.pipelined = g.pf;
-    c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision) - g.occ;
+    c->blocks_per_cu = std::max(1, pair_blocks_per_cu(c->k, c->P.precision) - g.occ);
     int rc = build_work(c, g.len);  // the list's layout depends on the claiming mode and the grid
     if(rc != QUEST_OK) return rc;
     HIPC(c, launch_pair_past(c->k, c->d, c->P.precision, c->stream));  // warm-up
@@ -1311,6 +1311,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   std::vector<Cfg> tried;
   for(int len : cand)
     for(int occ = 0; occ <= 2; ++occ) {
+      c->k.dyn_items = dyn0;
       c->k.pipelined = pf0;
       if(occ > 0 && pair_blocks_per_cu(c->k, c->P.precision) - occ < 2) continue;
       Cfg g{len, dyn0, pf0, occ, 0.0f};
@@ -1327,6 +1328,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
       for(int pf : {0, 1}) {
         if((dyn_fixed && dyn != dyn0) || (pf_fixed && pf != pf0) || (dyn == dyn0 && pf == pf0)) continue;
         Cfg g{tried[t].len, dyn, pf, tried[t].occ, 0.0f};
+        c->k.dyn_items = dyn;
         c->k.pipelined = pf;
         if(g.occ > 0 && pair_blocks_per_cu(c->k, c->P.precision) - g.occ < 2) continue;
         int rc = time_cfg(g);
@@ -1335,7 +1337,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
       }
   c->k.dyn_items = best.dyn;
   c->k.pipelined = best.pf;
-  c->blocks_per_cu = pair_blocks_per_cu(c->k, c->P.precision) - best.occ;
+  c->blocks_per_cu = std::max(1, pair_blocks_per_cu(c->k, c->P.precision) - best.occ);
   int rc = build_work(c, best.len);
   if(rc != QUEST_OK) return rc;
   if(c->mode == 1 && c->cur_step >= 0) {

@@ -219,3 +219,24 @@ def test_source_loop_default_and_autotune(ref_weights):
         assert b.info()["pipelined"] in (0, 1)
         ra, rb = a.state()[0], b.state()[0]
     assert rel_err(rb, ra) < 1e-8
+
+
+@pytest.mark.parametrize("n", [24576, 49152])
+def test_default_split_gives_every_xcd_an_equal_share(n):
+    """The default work split cuts the largest system into 48 (mixed) or 24
+    (FP64) equal source chunks -- multiples of the 8 XCDs, so the static list
+    gives every XCD the same pair work (49152 dots in 4096-source chunks made
+    12, half the XCDs doing twice the work) -- and runs one block per CU
+    below the occupancy limit; the partial rows are the chunks."""
+    from quest_amd import Context, systems
+    sysd = dense_system(n, (n / 512) ** (1 / 3), seed=81, plant_near=False)
+    grids = {}
+    for precision, chunks in ((1, 48), (0, 24)):
+        with Context(sysd, systems.consts(), precision=precision) as ctx:
+            info = ctx.info()
+            assert info["splits"] == chunks, info
+            assert info["source_chunk"] == -(-n // chunks)
+            assert info["grid"] % 8 == 0
+            grids[precision] = info["grid"]
+    # mixed: 8 resident blocks per CU less one; FP64: 6 less one (same CU count)
+    assert grids[1] * 5 == grids[0] * 7

@@ -54,6 +54,8 @@ struct Configuration {
   std::string qd_path = "dots.cfg", pulse_path = "pulse.cfg";
   int precision = QUEST_PREC_FP64;  // arithmetic of the pair sum on the device
   int pair_method = QUEST_PAIR_ON_THE_FLY;  // or QUEST_PAIR_TABLE (the reference's coefficient table)
+  int accumulation = QUEST_ACC_BLOCK;       // mixed precision's 64-source block sums (or compensated)
+  bool autotune = false;                    // DeviceIntegrator: quest_autotune before the first step
   int device = 0;
 };
 inline Configuration config;
@@ -506,6 +508,7 @@ inline quest_params params_from_config(int dyadic_kind, double omega, int interp
   p.precision = config.precision;
   p.dyadic = dyadic_kind;
   p.device = config.device;
+  p.accumulation = config.accumulation;
   return p;
 }
 }  // namespace quest_detail
@@ -549,6 +552,8 @@ class HistoryInteraction : public Interaction {
   {
     if(config.pair_method != QUEST_PAIR_ON_THE_FLY)
       quest_detail::check(quest_set_pair_method(ctx.get(), config.pair_method), ctx.get());
+    else if(config.autotune)
+      quest_detail::check(quest_autotune(ctx.get(), 3, nullptr), ctx.get());
   }
 
   const ResultArray &evaluate(const int time_idx) override
@@ -707,6 +712,8 @@ class DeviceIntegrator {
     }
     if(config.pair_method != QUEST_PAIR_ON_THE_FLY)
       quest_detail::check(quest_set_pair_method(ctx.get(), config.pair_method), ctx.get());
+    else if(config.autotune)
+      quest_detail::check(quest_autotune(ctx.get(), 3, nullptr), ctx.get());
   }
   // Several pulse interactions (their fields add, in list order, ahead of the
   // history interaction -- the reference integrator's sum over interactions,

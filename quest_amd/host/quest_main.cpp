@@ -2,7 +2,8 @@
 // library.  Reads the reference's three input files and writes output.dat
 // in the reference's format.
 //
-//   quest_hip [--config input.cfg] [--precision fp64|mixed] [--host-loop]
+//   quest_hip [--config input.cfg] [--precision fp64|mixed]
+//             [--accumulation block|compensated] [--tune] [--host-loop]
 //             [--all-pulses] [--pair-method on-the-fly|table]
 //             [--output output.dat] [--format text|binary] [--every K]
 //             [--subsample COUNT:SEED] [--device N] [--help] [--version]
@@ -21,6 +22,9 @@
 // only, which is what the reference reads (pulse.cpp:37-46).
 // --pair-method table streams the reference's per-pair coefficient table
 // (FP64 only) instead of evaluating the Green's function per pair.
+// --accumulation compensated makes mixed precision's 64-source FP32 block
+// sums Kahan-compensated; --tune times the device loop's work splits
+// (quest_autotune) before the first step.
 // --format binary, --every K and --subsample COUNT:SEED write only every K-th
 // step and/or a seeded subsample of the dots, as text rows in output.dat's
 // format or as binary records (History::SampledWriter); with the device loop
@@ -116,6 +120,8 @@ Options parse_configs(int argc, char *argv[])
                    "  -v [ --version ]       print version string\n"
                    "  -c [ --config ] arg    path to configuration file (default input.cfg)\n"
                    "  --precision arg        fp64 (default) or mixed\n"
+                   "  --accumulation arg     block (default) or compensated (mixed precision's block sums)\n"
+                   "  --tune                 time the pair kernel's work splits before the run (device loop)\n"
                    "  --host-loop            host predictor-corrector, device interactions\n"
                    "  --all-pulses           every row of pulse.cfg (default: the first)\n"
                    "  --pair-method arg      on-the-fly (default) or table (FP64 only)\n"
@@ -135,6 +141,13 @@ Options parse_configs(int argc, char *argv[])
       if(v == "fp64") config.precision = QUEST_PREC_FP64;
       else if(v == "mixed") config.precision = QUEST_PREC_MIXED;
       else throw std::runtime_error("--precision must be fp64 or mixed");
+    } else if(a == "--accumulation") {
+      const std::string v = next();
+      if(v == "block") config.accumulation = QUEST_ACC_BLOCK;
+      else if(v == "compensated") config.accumulation = QUEST_ACC_COMPENSATED;
+      else throw std::runtime_error("--accumulation must be block or compensated");
+    } else if(a == "--tune") {
+      config.autotune = true;
     } else if(a == "--host-loop") {
       o.host_loop = true;
     } else if(a == "--all-pulses") {

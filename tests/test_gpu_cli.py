@@ -74,8 +74,13 @@ def test_reference_example_inputs(oracle_lib, tmp_path, loop):
 
 
 @pytest.mark.parametrize("loop,precision", [("device", "fp64"), ("host", "fp64"), ("device", "mixed"),
-                                            ("device", "table"), ("host", "table")])
+                                            ("device", "table"), ("host", "table"),
+                                            ("device", "mixed+compensated"), ("device", "mixed+tune"),
+                                            ("device", "fp64+tune")])
 def test_dense_system(oracle_lib, tmp_path, loop, precision):
+    """A 40-dot system with close pairs under a short pi/2 pulse, 300 steps;
+    "+compensated" runs --accumulation compensated, "+tune" --tune (the work
+    split timed before the run: the results move at rounding level only)."""
     n, steps = 40, 300
     sysd = dense_system(n, 0.2, seed=77)
     with open(tmp_path / "dots.cfg", "w") as f:
@@ -94,8 +99,10 @@ def test_dense_system(oracle_lib, tmp_path, loop, precision):
         f.write(f"[parameters]\n num_particles = {n}\n total_time = {steps * 1e-4}\n timestep = 1e-4\n"
                 f" interpolation_order = 4\n[constants]\n c0 = 299.792458\n hbar = 0.65821193\n"
                 f" mu0 = 2.0133545e-04\n laser_frequency = {omega}\n")
-    args = (["--pair-method", "table"] if precision == "table" else ["--precision", precision]) + \
-        (["--host-loop"] if loop == "host" else [])
+    base, *flags = precision.split("+")
+    args = (["--pair-method", "table"] if base == "table" else ["--precision", base]) + \
+        (["--accumulation", "compensated"] if "compensated" in flags else []) + \
+        (["--tune"] if "tune" in flags else []) + (["--host-loop"] if loop == "host" else [])
     run_cli(tmp_path, args)
     got = read_output(tmp_path / "output.dat", n)
     consts = {"c0": 299.792458, "hbar": 0.65821193, "mu0": 2.0133545e-04, "laser_omega": omega, "dt": 1e-4}
@@ -103,7 +110,7 @@ def test_dense_system(oracle_lib, tmp_path, loop, precision):
     assert got.shape[0] == steps
     err = rel_err(got, want)
     print(f"{loop} loop {precision}: rel err {err:.3e}, max|rho01| {np.abs(want[:, :, 1]).max():.3f}")
-    assert err < (1e-6 if precision == "mixed" else 1e-10)
+    assert err < (1e-6 if base == "mixed" else 1e-10)
 
 
 @pytest.mark.parametrize("loop", ["device", "host"])

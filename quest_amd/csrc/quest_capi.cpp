@@ -1262,9 +1262,10 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   for(int f : {-2, -1, 0, 1, 2}) add_len(f >= 0 ? base << f : base >> -f);
   for(int cnt : {8, 16, 24, 32, 48})
     if(nmax / cnt >= 64) add_len((nmax + cnt - 1) / cnt);
-  // The pair kernel reads the ring and writes only the partial sums, which
-  // every step recomputes from scratch, so timing it between steps changes no
-  // state; the device step counter is restored below.
+  // The pair kernel reads the ring and the step counter and writes only the
+  // partial sums, which every step recomputes from scratch, so timing it
+  // between steps changes no state.  The counter is set to the next step (the
+  // value it holds between steps; before the first step it may not be set yet).
   HIPC(c, launch_set_step(c->d, std::max(c->cur_step, 0), c->stream));
   const bool mixed = c->P.precision == QUEST_PREC_MIXED;
   struct Events {  // destroyed on every return path

@@ -170,6 +170,20 @@ int quest_hist_evaluate(quest_ctx *ctx, int step, const double *rho01,
  *             device memory (QUEST_ERR_NOMEM).  Bound by HBM bandwidth. */
 #define QUEST_PAIR_ON_THE_FLY 0
 #define QUEST_PAIR_TABLE 1
+/* TIME_BLOCKED: the same sum as ON_THE_FLY (either precision), reordered in
+ *             time.  A pair of integer delay n reads, at step t, steps up to
+ *             t - n only, so at every step t0 that is a multiple of K = 8 one
+ *             pass over the pairs of delay >= K forms their sums for the K
+ *             steps t0 .. t0 + K - 1: the pair's weights (geometry, Green's
+ *             function, Lagrange weights) once, then K windows of the
+ *             source's rho01 samples.  The pairs of delay 1 .. K - 1 are
+ *             summed step by step from a short list, the zero-delay pairs as
+ *             in the other methods.  Memory: a sample ring of n x (the
+ *             largest delay + order + K, to a power of two, + K + order)
+ *             complex values and K slices of the partial sums.  Both
+ *             quest_run and quest_hist_evaluate drive it; quest_run replays
+ *             graphs of K steps. */
+#define QUEST_PAIR_TIME_BLOCKED 2
 
 /* Selects the pair method; call before the first quest_run or
  * quest_hist_evaluate (QUEST_ERR_STATE afterwards).  Replaces the choice the
@@ -278,9 +292,15 @@ int quest_pair_delays_range(quest_ctx *ctx, int64_t first, int64_t count, int32_
 int quest_set_timing(quest_ctx *ctx, int enable);
 
 /* Accumulated pair-kernel time (ms) and launch count since timing was
- * enabled or last reset; reset != 0 clears them after reading. */
+ * enabled or last reset; reset != 0 clears them after reading.  With
+ * QUEST_PAIR_TIME_BLOCKED a launch is one pass of K steps. */
 int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
                            int reset);
+
+/* The same for the per-step short-list kernel of QUEST_PAIR_TIME_BLOCKED
+ * (pairs of delay 1 .. K - 1); zero for the other methods. */
+int quest_short_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
+                            int reset);
 
 /* Times the pair kernel (`launches` launches each, after a warm-up) and keeps
  * the fastest configuration, in two stages: (1) source-chunk lengths (the
@@ -293,14 +313,17 @@ int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
  * state.  No counterpart in the reference, which has no tunable kernel. */
 int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);
 
-/* Integer facts about the context.  info: [16] int64 = n_dots, ring slots,
+/* Integer facts about the context.  info: [20] int64 = n_dots, ring slots,
  * delay bound, zero-delay pairs, work items, persistent grid, state slots,
  * precision, systems, source-chunk length, chunks of the largest system,
  * step graph captured (0/1), block size of the one-launch predictor-corrector
  * step (0: one launch per iteration, used when a connected group of
  * zero-delay partners exceeds 256 dots), size of the largest such group,
  * pair-kernel items claimed dynamically (1) or by a static stride (0),
- * pipelined (1) or plain (0) source loop of the mixed-precision pair kernel. */
+ * pipelined (1) or plain (0) source loop of the mixed-precision pair kernel,
+ * pair method (QUEST_PAIR_*), steps per pass of the time-blocked method (0
+ * for the others), its short-list pairs (delays 1 .. K - 1), its sample-ring
+ * slots. */
 int quest_ctx_info(const quest_ctx *ctx, int64_t *info);
 
 /* Blocks until all work queued on the context's stream is done. */

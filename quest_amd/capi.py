@@ -40,9 +40,9 @@ EXPORTS = [
     "quest_pair_delays", "quest_set_timing", "quest_pair_kernel_time", "quest_ctx_info",
     "quest_sync", "quest_batch_create", "quest_set_pulses", "quest_autotune",
     "quest_set_pulse_list", "quest_set_pair_method", "quest_pair_delays_range",
-    "quest_run_noalloc",
+    "quest_run_noalloc", "quest_short_kernel_time",
 ]
-PAIR_ON_THE_FLY, PAIR_TABLE = 0, 1  # QUEST_PAIR_*
+PAIR_ON_THE_FLY, PAIR_TABLE, PAIR_TIME_BLOCKED = 0, 1, 2  # QUEST_PAIR_*
 CLAIM_AUTO, CLAIM_STATIC, CLAIM_DYNAMIC = 0, 1, 2  # QUEST_CLAIM_*
 LOOP_AUTO, LOOP_PLAIN, LOOP_PIPELINED = 0, 1, 2     # QUEST_LOOP_*
 ACC_BLOCK, ACC_COMPENSATED = 0, 1                   # QUEST_ACC_*
@@ -107,6 +107,7 @@ def load():
     L.quest_pair_delays_range.argtypes = [vp, C.c_int64, C.c_int64, C.POINTER(C.c_int32)]
     L.quest_set_timing.argtypes = [vp, C.c_int]
     L.quest_pair_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int]
+    L.quest_short_kernel_time.argtypes = [vp, dp, C.POINTER(C.c_int64), C.c_int]
     L.quest_ctx_info.argtypes = [vp, C.POINTER(C.c_int64)]
     L.quest_sync.argtypes = [vp]
     L.quest_batch_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(C.c_int), dp, dp, dp, dp,
@@ -238,8 +239,9 @@ class Context:
         self._check(self._L.quest_set_pulse_list(self.ptr, int(system), len(pulses), arr))
 
     def set_pair_method(self, method: int):
-        """PAIR_ON_THE_FLY (default) or PAIR_TABLE (the reference's per-pair
-        coefficient table, FP64 only); before the first step."""
+        """PAIR_ON_THE_FLY (default), PAIR_TABLE (the reference's per-pair
+        coefficient table, FP64 only) or PAIR_TIME_BLOCKED (one pass over the
+        pairs of delay >= 8 per 8 steps); before the first step."""
         self._check(self._L.quest_set_pair_method(self.ptr, int(method)))
 
     def autotune(self, launches: int = 3) -> int:
@@ -314,12 +316,20 @@ class Context:
         self._check(self._L.quest_pair_kernel_time(self.ptr, C.byref(ms), C.byref(cnt), int(reset)))
         return ms.value, cnt.value
 
+    def short_kernel_time(self, reset: bool = False):
+        """Time-blocked method: accumulated short-list kernel time (ms) and launches."""
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self._check(self._L.quest_short_kernel_time(self.ptr, C.byref(ms), C.byref(cnt), int(reset)))
+        return ms.value, cnt.value
+
     def info(self) -> dict:
-        a = (C.c_int64 * 16)()
+        a = (C.c_int64 * 20)()
         self._check(self._L.quest_ctx_info(self.ptr, a))
         keys = ["n", "ring_slots", "max_delay_bound", "near_pairs", "work_items", "grid",
                 "state_slots", "precision", "systems", "source_chunk", "splits", "graph",
-                "step_block", "max_group", "dynamic_items", "pipelined"]
+                "step_block", "max_group", "dynamic_items", "pipelined", "pair_method",
+                "block_steps", "short_pairs", "sample_slots"]
         return dict(zip(keys, list(a)))
 
     def sync(self) -> None:

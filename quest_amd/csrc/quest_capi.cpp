@@ -61,6 +61,13 @@ struct quest_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double2 *scratch = nullptr;  // this device's static scratch array (quest_run_noalloc)
+  int short_total = 0;         // time-blocked mode: pairs of delay 1 .. K - 1
+  int slice_buf = 0;           // time-blocked mode: the current block's partial sums are in the
+                               // context's buffer (0) or the static scratch (1)
+  size_t hist_bytes = 0;       // time-blocked mode: the sample ring
+  std::vector<int> ev_kind;    // per timed event pair: 0 pair kernel, 1 short-list kernel
+  double short_ms = 0;
+  int64_t short_launches = 0;
 };
 
 namespace {
@@ -313,11 +320,14 @@ int build_work(quest_ctx *c, int chunk_len)
     HIPC(c, hipMemcpyAsync(d.work_row, row.data(), sizeof(int) * row.size(), hipMemcpyHostToDevice, c->stream));
   }
   HIPC(c, hipMemcpyAsync(d.sys_chunks, chunks.data(), sizeof(int) * c->n_sys, hipMemcpyHostToDevice, c->stream));
-  if(max_chunks + 1 > c->partial_rows) {  // + the spare row of empty items
+  // + the spare row of empty items; the time-blocked method keeps one slice
+  // of rows per step of its pass
+  const int rows = (max_chunks + 1) * (c->k.pair_blocked ? c->k.block_steps : 1);
+  if(rows > c->partial_rows) {
     if(d.partial) HIPC(c, hipFree(d.partial));
     d.partial = nullptr;
-    HIPC(c, hipMalloc(&d.partial, sizeof(double2) * static_cast<size_t>(c->n) * (max_chunks + 1)));
-    c->partial_rows = max_chunks + 1;
+    HIPC(c, hipMalloc(&d.partial, sizeof(double2) * static_cast<size_t>(c->n) * rows));
+    c->partial_rows = rows;
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   c->chunk_len = chunk_len;
@@ -356,7 +366,7 @@ int default_chunk(const quest_ctx *c)
 int default_blocks_per_cu(const quest_ctx *c)
 {
   const int full = pair_blocks_per_cu(c->k, c->P.precision);
-  return c->pair_method == QUEST_PAIR_ON_THE_FLY && full >= 4 ? full - 1 : full;
+  return c->pair_method == QUEST_PAIR_ON_THE_FLY && full >= 4 ? full - 1 : full;  // blocked, table: full
 }
 
 // The three factor tables of e^{i theta s} (phase_at on the device): each
@@ -387,6 +397,7 @@ int start_state(quest_ctx *c)
 {
   HIPC(c, launch_fill_ground_state(c->k, c->d, c->stream));
   HIPC(c, launch_clear_ring(c->k, c->d, c->ring_bytes, c->stream));
+  if(c->d.hist) HIPC(c, launch_clear(c->d.hist, c->hist_bytes, c->stream));  // rho01 = 0 before step 0
   c->bad = -1;
   return QUEST_OK;
 }
@@ -426,10 +437,16 @@ int collect_timing(quest_ctx *c)
   for(size_t i = 0; i + 1 < c->ev_used; i += 2) {
     float ms = 0;
     HIPC(c, hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
-    c->pair_ms += ms;
-    c->pair_launches += 1;
+    if(i / 2 < c->ev_kind.size() && c->ev_kind[i / 2] == 1) {
+      c->short_ms += ms;
+      c->short_launches += 1;
+    } else {
+      c->pair_ms += ms;
+      c->pair_launches += 1;
+    }
   }
   c->ev_used = 0;
+  c->ev_kind.clear();
   return QUEST_OK;
 }
 
@@ -445,6 +462,7 @@ int pair_sum(quest_ctx *c, DeviceState &d, bool timed)
     e0 = pool_event(c);
     e1 = pool_event(c);
     if(!e0 || !e1) return fail(c, QUEST_ERR_HIP, "hipEventCreate failed");
+    c->ev_kind.push_back(0);
     HIPC(c, hipEventRecord(e0, c->stream));
   }
   HIPC(c, launch_pair_past(c->k, d, c->P.precision, c->stream));
@@ -453,11 +471,33 @@ int pair_sum(quest_ctx *c, DeviceState &d, bool timed)
 }
 int pair_sum(quest_ctx *c) { return pair_sum(c, c->d, true); }
 
-// One whole step: pair sum, predictor, 10 x (evaluate, correct), final
-// evaluation and commit, step counter advance.
-int enqueue_step(quest_ctx *c, DeviceState &d, bool timed)
+// Time-blocked method: the short list's sum of this step, bracketed by HIP
+// events when timing.
+int short_sum(quest_ctx *c, DeviceState &d, bool timed)
 {
-  int rc = pair_sum(c, d, timed);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  timed = timed && c->timing;
+  if(timed) {
+    e0 = pool_event(c);
+    e1 = pool_event(c);
+    if(!e0 || !e1) return fail(c, QUEST_ERR_HIP, "hipEventCreate failed");
+    c->ev_kind.push_back(1);
+    HIPC(c, hipEventRecord(e0, c->stream));
+  }
+  HIPC(c, launch_pair_short(c->k, d, c->P.precision, c->stream));
+  if(timed) HIPC(c, hipEventRecord(e1, c->stream));
+  return QUEST_OK;
+}
+
+// One whole step: pair sum, predictor, 10 x (evaluate, correct), final
+// evaluation and commit, step counter advance.  `step` is the step being
+// queued: the time-blocked method forms its pair sums at the steps that are
+// multiples of K and the short list's sum at every step.
+int enqueue_step(quest_ctx *c, DeviceState &d, bool timed, int step)
+{
+  int rc = QUEST_OK;
+  if(!c->k.pair_blocked || step % c->k.block_steps == 0) rc = pair_sum(c, d, timed);
+  if(rc == QUEST_OK && c->k.pair_blocked) rc = short_sum(c, d, timed);
   if(rc != QUEST_OK) return rc;
   if(c->step_block > 0) {
     HIPC(c, launch_pc_step(c->k, d, c->P.precision, c->fc, c->P.history_window, c->step_block,
@@ -470,17 +510,21 @@ int enqueue_step(quest_ctx *c, DeviceState &d, bool timed)
   HIPC(c, launch_advance_step(d, c->stream));
   return QUEST_OK;
 }
-int enqueue_step(quest_ctx *c) { return enqueue_step(c, c->d, true); }
 
-// One step captured as a graph (kernel launches only; no memsets), replayed
-// for every step: the kernels read the step number from the device counter.
+// Steps captured per graph: one (replayed for every step; the kernels read
+// the step number from the device counter), or K for the time-blocked method
+// (replayed from steps that are multiples of K).
+int graph_steps(const quest_ctx *c) { return c->k.pair_blocked ? c->k.block_steps : 1; }
+
+// The graph of graph_steps(c) steps (kernel launches only; no memsets).
 int ensure_graph(quest_ctx *c)
 {
   if(c->graph_exec) return QUEST_OK;
   HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   const bool was_timing = c->timing;
   c->timing = false;
-  const int rc = enqueue_step(c);
+  int rc = QUEST_OK;
+  for(int i = 0; i < graph_steps(c) && rc == QUEST_OK; ++i) rc = enqueue_step(c, c->d, false, i);
   c->timing = was_timing;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -503,7 +547,7 @@ void free_device(quest_ctx *c)
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.io, d.sys_of, d.sys_range, d.sys_chunks,
                   d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff,
-                  d.step_map, d.near_lane};
+                  d.step_map, d.near_lane, d.hist, d.rot, d.short_start, d.short_src, d.short_sum};
   for(void *p : ptrs)
     if(p) (void)hipFree(p);
   d = DeviceState{};
@@ -702,6 +746,7 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   k.half_phase_im = std::sin(-0.5 * k.theta);
   k.dyadic = p->dyadic;
   k.n_systems = n_sys;
+  k.block_steps = 1;
   k.compensated = p->precision == QUEST_PREC_MIXED && p->accumulation == QUEST_ACC_COMPENSATED;
   // FP32 positions are stored relative to the centre of all systems' boxes,
   // in units of c0 dt (so a pair's FP32 distance is its delay tau); each
@@ -960,6 +1005,99 @@ int enable_table(quest_ctx *c)
   return QUEST_OK;
 }
 
+void free_blocked(quest_ctx *c)
+{
+  DeviceState &d = c->d;
+  for(void *p : {d.hist, d.rot, static_cast<void *>(d.short_start), static_cast<void *>(d.short_src),
+                 static_cast<void *>(d.short_sum)})
+    if(p) (void)hipFree(p);
+  d.hist = d.rot = nullptr;
+  d.short_start = d.short_src = nullptr;
+  d.short_sum = nullptr;
+  c->k.pair_blocked = 0;
+  c->k.block_steps = 1;
+  c->short_total = 0;
+  c->hist_bytes = 0;
+}
+
+// Time-blocked mode: the rho01 sample ring, the rotation table
+// e^{-i theta (n + 1/2)}, and the short list (pairs of delay 1 .. K - 1, the
+// precision's own split, as the pair kernels classify them).  The ring holds
+// more than the largest delay + order + K steps, so that the pass of the
+// current block can be formed again from it at any step of the block
+// (refill_block).
+int enable_blocked(quest_ctx *c)
+{
+  Consts &k = c->k;
+  DeviceState &d = c->d;
+  const int M = c->P.interp_order, K = kBlockSteps;
+  const bool f64 = c->P.precision == QUEST_PREC_FP64;
+  const size_t r2 = f64 ? sizeof(double2) : sizeof(float2);
+  k.block_steps = K;
+  k.hist_slots = next_pow2(c->dmax_bound + M + K + 1);
+  k.hist_mask = k.hist_slots - 1;
+  k.hist_row = (k.hist_slots + K + M - 1 + 1) / 2 * 2;
+  const size_t nn = static_cast<size_t>(c->n);
+  // the pair kernel addresses a window with a 32-bit byte offset
+  if(nn * k.hist_row * r2 > 0xFFFFFFFFull)
+    return fail(c, QUEST_ERR_ARG, "time-blocked method: dots x sample-ring row (" + std::to_string(nn) + " x " +
+                                      std::to_string(k.hist_row) + ") exceeds 4 GiB of samples");
+  c->hist_bytes = (nn * k.hist_row * r2 + 15) / 16 * 16;
+  if(hipMalloc(&d.hist, c->hist_bytes) != hipSuccess || hipMalloc(&d.rot, k.hist_slots * r2) != hipSuccess ||
+     hipMalloc(reinterpret_cast<void **>(&d.short_start), sizeof(int) * (nn + 1)) != hipSuccess ||
+     hipMalloc(reinterpret_cast<void **>(&d.short_sum), sizeof(double2) * std::max<size_t>(nn, 1)) != hipSuccess) {
+    free_blocked(c);
+    return fail(c, QUEST_ERR_NOMEM, "time-blocked method: allocation failed");
+  }
+  {
+    // e^{-i theta (j + 1/2)}, each from a long-double argument reduction
+    const long double theta = static_cast<long double>(k.omega) * static_cast<long double>(k.dt);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    std::vector<double2> h(k.hist_slots);
+    std::vector<float2> hf(k.hist_slots);
+    for(int j = 0; j < k.hist_slots; ++j) {
+      const long double a = fmodl(-theta * (static_cast<long double>(j) + 0.5L), two_pi);
+      h[j] = make_double2(static_cast<double>(cosl(a)), static_cast<double>(sinl(a)));
+      hf[j] = make_float2(static_cast<float>(cosl(a)), static_cast<float>(sinl(a)));
+    }
+    if(f64) HIPC(c, hipMemcpyAsync(d.rot, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, c->stream));
+    else HIPC(c, hipMemcpyAsync(d.rot, hf.data(), sizeof(float2) * hf.size(), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));  // h, hf go out of scope
+  }
+  HIPC(c, launch_delay_list(k, d, c->P.precision, 1, K, false, nullptr, nullptr, nullptr, d.counts, c->stream));
+  std::vector<int> counts(nn);
+  HIPC(c, hipMemcpyAsync(counts.data(), d.counts, sizeof(int) * nn, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  std::vector<int> start(nn + 1, 0);
+  for(size_t i = 0; i < nn; ++i) start[i + 1] = start[i] + counts[i];
+  c->short_total = start[nn];
+  if(hipMalloc(reinterpret_cast<void **>(&d.short_src), sizeof(int) * std::max(c->short_total, 1)) != hipSuccess) {
+    free_blocked(c);
+    return fail(c, QUEST_ERR_NOMEM, "time-blocked method: short list allocation failed");
+  }
+  HIPC(c, hipMemcpyAsync(d.short_start, start.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, launch_delay_list(k, d, c->P.precision, 1, K, true, d.short_start, d.short_src, nullptr, d.counts,
+                            c->stream));
+  HIPC(c, launch_clear(d.hist, c->hist_bytes, c->stream));  // rho01 = 0 before step 0 (main.cpp:32-36)
+  HIPC(c, hipStreamSynchronize(c->stream));
+  k.pair_blocked = 1;
+  return QUEST_OK;
+}
+
+// Time-blocked method, at a step inside a block: forms the current block's
+// pass again (into d's partial sums), e.g. after the partial sums moved to
+// another buffer or the work split changed.  The block started at
+// K floor(step / K); the sample ring still holds everything its pass reads.
+int refill_block(quest_ctx *c, DeviceState &d, int step)
+{
+  if(!c->k.pair_blocked || step % c->k.block_steps == 0) return QUEST_OK;
+  HIPC(c, launch_set_step(d, step - step % c->k.block_steps, c->stream));
+  const int rc = pair_sum(c, d, false);
+  if(rc != QUEST_OK) return rc;
+  HIPC(c, launch_set_step(d, step, c->stream));
+  return QUEST_OK;
+}
+
 // Pulse list of system s on the host copy: the given pulses first, the rest off.
 void put_pulses(quest_ctx *c, int s, int n, const quest_pulse *pl)
 {
@@ -1084,7 +1222,8 @@ int quest_hist_evaluate(quest_ctx *c, int step, const double *rho01, double *out
   if(step != c->cur_step) {
     if(c->cur_step >= 0) HIPC(c, launch_hist_commit(c->k, c->d, c->P.precision, c->cur_step, c->stream));
     HIPC(c, launch_set_step(c->d, step, c->stream));
-    const int rc = pair_sum(c);
+    int rc = (!c->k.pair_blocked || step % c->k.block_steps == 0) ? pair_sum(c) : QUEST_OK;
+    if(rc == QUEST_OK && c->k.pair_blocked) rc = short_sum(c, c->d, true);
     if(rc != QUEST_OK) return rc;
     HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
     c->cur_step = step;
@@ -1137,17 +1276,22 @@ int quest_set_pulse_list(quest_ctx *c, int system, int n_pulses, const quest_pul
 int quest_set_pair_method(quest_ctx *c, int method)
 {
   if(!c) return fail(nullptr, QUEST_ERR_ARG, "ctx is NULL");
-  if(method != QUEST_PAIR_ON_THE_FLY && method != QUEST_PAIR_TABLE)
-    return fail(c, QUEST_ERR_ARG, "method must be QUEST_PAIR_ON_THE_FLY or QUEST_PAIR_TABLE");
+  if(method != QUEST_PAIR_ON_THE_FLY && method != QUEST_PAIR_TABLE && method != QUEST_PAIR_TIME_BLOCKED)
+    return fail(c, QUEST_ERR_ARG,
+                "method must be QUEST_PAIR_ON_THE_FLY, QUEST_PAIR_TABLE or QUEST_PAIR_TIME_BLOCKED");
   if(method == c->pair_method) return QUEST_OK;
   if(c->mode != 0) return fail(c, QUEST_ERR_STATE, "choose the pair method before the first step");
   HIPC(c, hipSetDevice(c->P.device));
   drop_graph(c);
+  free_table(c);
+  free_blocked(c);
+  c->pair_method = QUEST_PAIR_ON_THE_FLY;
   if(method == QUEST_PAIR_TABLE) {
     const int rc = enable_table(c);
     if(rc != QUEST_OK) return rc;
-  } else {
-    free_table(c);
+  } else if(method == QUEST_PAIR_TIME_BLOCKED) {
+    const int rc = enable_blocked(c);
+    if(rc != QUEST_OK) return rc;
   }
   c->pair_method = method;
   c->blocks_per_cu = default_blocks_per_cu(c);
@@ -1201,16 +1345,35 @@ int quest_run(quest_ctx *c, int n_steps)
 {
   int rc = run_begin(c, n_steps);
   if(rc != QUEST_OK) return rc;
+  int step = c->cur_step;
+  const int end = step + n_steps;
+  if(c->slice_buf != 0 && n_steps > 0) {  // the block's partial sums are in the static scratch
+    rc = refill_block(c, c->d, step);
+    if(rc != QUEST_OK) return rc;
+    c->slice_buf = 0;
+  }
   if(c->timing) {
     // kernel by kernel, so that HIP events can bracket every pair-kernel launch
-    for(int i = 0; i < n_steps; ++i) {
-      rc = enqueue_step(c);
+    for(; step < end; ++step) {
+      rc = enqueue_step(c, c->d, true, step);
       if(rc != QUEST_OK) return rc;
     }
-  } else if(n_steps > 0) {
-    rc = ensure_graph(c);
-    if(rc != QUEST_OK) return rc;
-    for(int i = 0; i < n_steps; ++i) HIPC(c, hipGraphLaunch(c->graph_exec, c->stream));
+  } else {
+    // steps one by one up to a graph boundary, whole graphs, then the rest
+    const int G = graph_steps(c);
+    for(; step < end && step % G != 0; ++step) {
+      rc = enqueue_step(c, c->d, false, step);
+      if(rc != QUEST_OK) return rc;
+    }
+    if(end - step >= G) {
+      rc = ensure_graph(c);
+      if(rc != QUEST_OK) return rc;
+      for(; end - step >= G; step += G) HIPC(c, hipGraphLaunch(c->graph_exec, c->stream));
+    }
+    for(; step < end; ++step) {
+      rc = enqueue_step(c, c->d, false, step);
+      if(rc != QUEST_OK) return rc;
+    }
   }
   if(c->timing) {
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -1224,17 +1387,25 @@ int quest_run_noalloc(quest_ctx *c, int n_steps)
 {
   if(c && !c->scratch) return fail(c, QUEST_ERR_HIP, "the static scratch array has no device address");
   // the pair kernel's partial rows (source chunks, plus the spare row of empty items)
-  if(c && sizeof(double2) * static_cast<size_t>(c->n) * (c->k.max_chunks + 1) > kScratchBytes)
+  // (time-blocked method: one slice of them per step of the pass)
+  const size_t rows = c ? static_cast<size_t>(c->k.max_chunks + 1) * (c->k.pair_blocked ? c->k.block_steps : 1) : 0;
+  if(c && sizeof(double2) * static_cast<size_t>(c->n) * rows > kScratchBytes)
     return fail(c, QUEST_ERR_ARG,
-                "quest_run_noalloc: the pair sums need " + std::to_string(c->k.max_chunks + 1) + " x " +
-                    std::to_string(c->n) + " x 16 B, more than the static scratch of " +
-                    std::to_string(kScratchBytes >> 20) + " MiB; use quest_run or fewer source chunks");
+                "quest_run_noalloc: the pair sums need " + std::to_string(rows) + " x " + std::to_string(c->n) +
+                    " x 16 B, more than the static scratch of " + std::to_string(kScratchBytes >> 20) +
+                    " MiB; use quest_run or fewer source chunks");
   int rc = run_begin(c, n_steps);
   if(rc != QUEST_OK) return rc;
   DeviceState d = c->d;
   d.partial = c->scratch;
-  for(int i = 0; i < n_steps; ++i) {
-    rc = enqueue_step(c, d, false);
+  int step = c->cur_step;
+  if(c->slice_buf != 1 && n_steps > 0) {  // the block's partial sums are in the context's buffer
+    rc = refill_block(c, d, step);
+    if(rc != QUEST_OK) return rc;
+    c->slice_buf = 1;
+  }
+  for(int i = 0; i < n_steps; ++i, ++step) {
+    rc = enqueue_step(c, d, false, step);
     if(rc != QUEST_OK) return rc;
   }
   return run_end(c, n_steps);
@@ -1303,7 +1474,7 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   // claiming and source loop: the caller's choice, else static and (mixed) the
   // default loop in the first stage
   const bool dyn_fixed = c->P.item_claim != QUEST_CLAIM_AUTO;
-  const bool pf_fixed = !mixed || c->P.source_loop != QUEST_LOOP_AUTO;
+  const bool pf_fixed = !mixed || c->P.source_loop != QUEST_LOOP_AUTO || c->k.pair_blocked;
   const int dyn0 = c->P.item_claim == QUEST_CLAIM_DYNAMIC ? 1 : 0;
   const int pf0 = mixed ? (c->P.source_loop == QUEST_LOOP_AUTO ? c->k.pipelined
                                                                : c->P.source_loop == QUEST_LOOP_PIPELINED)
@@ -1341,10 +1512,25 @@ int quest_autotune(quest_ctx *c, int launches, int *chunk_len)
   c->blocks_per_cu = std::max(1, pair_blocks_per_cu(c->k, c->P.precision) - best.occ);
   int rc = build_work(c, best.len);
   if(rc != QUEST_OK) return rc;
+  if(c->k.pair_blocked && c->mode != 0 && c->cur_step >= 0) {
+    // the timed passes overwrote the current block's partial sums: per-call
+    // mode has formed the block of its current step, whole-run mode that of
+    // the step before its next one unless that starts a block
+    if(c->mode == 1 && c->cur_step % c->k.block_steps == 0) {
+      HIPC(c, launch_set_step(c->d, c->cur_step, c->stream));
+      rc = pair_sum(c, c->d, false);
+    } else {
+      rc = refill_block(c, c->d, c->cur_step);
+    }
+    if(rc != QUEST_OK) return rc;
+    c->slice_buf = 0;
+  }
   if(c->mode == 1 && c->cur_step >= 0) {
     // per-call mode keeps the current step's pair sum in `field`: recompute it
-    rc = pair_sum(c);
-    if(rc != QUEST_OK) return rc;
+    if(!c->k.pair_blocked) {
+      rc = pair_sum(c);
+      if(rc != QUEST_OK) return rc;
+    }
     HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -1451,6 +1637,18 @@ int quest_pair_kernel_time(quest_ctx *c, double *ms, int64_t *launches, int rese
   return QUEST_OK;
 }
 
+int quest_short_kernel_time(quest_ctx *c, double *ms, int64_t *launches, int reset)
+{
+  if(!c) return QUEST_ERR_ARG;
+  if(ms) *ms = c->short_ms;
+  if(launches) *launches = c->short_launches;
+  if(reset) {
+    c->short_ms = 0;
+    c->short_launches = 0;
+  }
+  return QUEST_OK;
+}
+
 int quest_ctx_info(const quest_ctx *c, int64_t *info)
 {
   if(!c || !info) return QUEST_ERR_ARG;
@@ -1470,6 +1668,10 @@ int quest_ctx_info(const quest_ctx *c, int64_t *info)
   info[13] = c->max_group;
   info[14] = c->k.dyn_items;
   info[15] = c->k.pipelined;
+  info[16] = c->pair_method;
+  info[17] = c->k.pair_blocked ? c->k.block_steps : 0;
+  info[18] = c->short_total;
+  info[19] = c->k.pair_blocked ? c->k.hist_slots : 0;
   return QUEST_OK;
 }
 

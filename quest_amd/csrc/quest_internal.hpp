@@ -17,6 +17,7 @@ constexpr int kMaxWindow = 64;     // PC history window limit
 constexpr int kBlock = 256;        // threads per block of the per-dot and pair kernels
 constexpr int kWave = 64;          // wavefront size; a pair-kernel work item is one wave of targets
 constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
+constexpr int kBlockSteps = 8;     // steps of one pass of the time-blocked pair sum
 // e^{i theta s} tables: 2^10 entries each for the low and middle ten bits of
 // the step, 2^11 for the high bits (steps below 2^31)
 constexpr int kPhaseLevel = 1024;
@@ -80,6 +81,14 @@ struct Consts {
   int samp_slots;     // rho01 sample ring: power of two > max delay + order + 1
   int samp_mask;
   long long table_plane;  // table entries per coefficient plane (sum over systems of n_s^2)
+  // time-blocked mode (QUEST_PAIR_TIME_BLOCKED): the pair sums of block_steps
+  // steps from one pass over the pairs of delay >= block_steps, a short list
+  // for delays 1 .. block_steps - 1, and a rho01 sample ring
+  int pair_blocked;   // 1: time-blocked pair sums
+  int block_steps;    // K, the steps of one pass (a launch at every step that is a multiple of K)
+  int hist_slots;     // sample ring: power of two > max delay + order
+  int hist_mask;
+  int hist_row;       // entries per source row: hist_slots + K + order - 1, rounded up to even
 };
 
 struct PulseParams {
@@ -137,6 +146,12 @@ struct DeviceState {
                                        // sys_toff[s] + (b - begin_s) n_s + (a - begin_s)
   int16_t *table_n = nullptr;          // [table_plane] integer delay (0 for excluded pairs)
   long long *sys_toff = nullptr;       // [systems] first table entry of each system
+  // time-blocked mode only
+  void *hist = nullptr;                // rho01 sample ring [n][hist_row] (double2 FP64, float2 mixed)
+  void *rot = nullptr;                 // [hist_slots] e^{-i theta (n + 1/2)} (double2 FP64, float2 mixed)
+  int *short_start = nullptr;          // [n + 1] short list (delays 1 .. K - 1), CSR by target
+  int *short_src = nullptr;            // [short pairs]
+  double2 *short_sum = nullptr;        // [n] the short list's sum at the current step
   // one-launch predictor-corrector step: thread -> dot (-1 idle), each group of
   // zero-delay partners within one block, all groups in the leading blocks
   int *step_map = nullptr;
@@ -180,8 +195,16 @@ hipError_t launch_permute_out(const Consts &k, DeviceState &d, hipStream_t s);
 hipError_t launch_pulse(const Consts &k, DeviceState &d, hipStream_t s);
 // Occupancy of the pair kernel (blocks per CU) for sizing the persistent grid.
 int pair_blocks_per_cu(const Consts &k, int precision);
+// Lists of the sources of each target whose pair delay (the precision's own
+// split) lies in [lo, hi), b != a: counts[a] (FILL false) or, with start
+// given, src[start[a] ..] in source order and tgt (may be null) (FILL true).
+// The zero-delay list is [0, 1); the time-blocked short list [1, K).
 hipError_t launch_near_count(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipStream_t s);
+hipError_t launch_delay_list(const Consts &k, DeviceState &d, int precision, int lo, int hi, bool fill,
+                             const int *start, int *src, int *tgt, int *counts, hipStream_t s);
+// Time-blocked mode: the short list's sum at the current step into d.short_sum.
+hipError_t launch_pair_short(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s);
 hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int precision, int32_t *out, int64_t first,
                               int64_t count, hipStream_t s);

@@ -642,6 +642,312 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
   items.finish();
 }
 
+// ------------------------------------------------- time-blocked pair sums
+//
+// The reference's sum for target a at step t (history_interaction.cpp:51-67),
+//   Omega_a(t) = sum_b sum_m coef_ab[m] rho01_b(t - n_ab - m),
+// reads, for a pair of integer delay n, steps up to t - n only.  At the start
+// of step t0 (steps < t0 finished) every pair with n >= K therefore holds all
+// it needs for the K steps t0 .. t0 + K - 1: its weights (geometry, Green's
+// function and Lagrange weights, the expensive part) are formed once and
+// applied to K consecutive windows of the source's rho01 samples.  The pairs
+// with 1 <= n < K (a short list, ~30 per dot at K = 8 and 512 um^-3) are
+// summed step by step (pair_short), and the zero-delay pairs stay on the near
+// list.
+//
+// Sample ring: rho01 of source b at finished step u in row b, entry
+// u & hist_mask, and again at hist_slots + (u & hist_mask) when that is below
+// K + M - 1, so that a window of K + M consecutive steps is one contiguous
+// run of the row whatever its start (2 Re rho01 in the lab frame, the
+// reference's polarization_prefactor, green_function.h:23-27).
+//
+// Per pair, with x the delay's fraction and p(x) = sum_m L_m(x) rho01(s - m)
+// the reference's interpolant (lagrange_set.cpp:17-41), the pair term of
+// pair_term is  A p + B p' + C p''/2  with complex A, B, C fixed by the
+// geometry (rotation e^{-i theta (n + x)} included).  So its weight on
+// sample s - m is W_m = A L_m(x) + B L_m'(x) + C L_m''(x)/2, formed from the
+// monomial coefficients of the L_m as W_m = sum_j l[j][m] Gamma_j with
+// Gamma_j = A x^j + j B x^(j-1) + j(j-1)/2 C x^(j-2).
+
+// Monomial coefficients of the Lagrange basis on nodes 0..M:
+// L_m(x) = sum_j c[j][m] x^j, exact rationals (integer products over one
+// integer denominator) rounded once.
+template <int M> struct LagrangeMonomials {
+  double c[M + 1][M + 1];
+  constexpr LagrangeMonomials() : c{}
+  {
+    for(int m = 0; m <= M; ++m) {
+      double p[M + 1] = {};
+      p[0] = 1.0;
+      double den = 1.0;
+      int deg = 0;
+      for(int j = 0; j <= M; ++j) {
+        if(j == m) continue;
+        for(int e = deg + 1; e >= 1; --e) p[e] = p[e - 1] - j * p[e];  // times (x - j)
+        p[0] = -j * p[0];
+        ++deg;
+        den *= m - j;
+      }
+      for(int j = 0; j <= M; ++j) c[j][m] = p[j] / den;
+    }
+  }
+};
+
+// Weights W_m = wr[m] + i wi[m] of one pair on samples s - m, m = 0..M,
+// before the global factor -K (rotating) or K (lab) and the unit scale.
+// live false: all weights 0 (the pair belongs to another list).  rot_n =
+// e^{-i theta (n + 1/2)} (rotating frame), x = xc + 1/2.
+template <typename R, int M, bool LAB, bool GEN>
+__device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R dy, R dz,
+                                              const typename V4<R>::type &da, const typename V4<R>::type &dah,
+                                              const typename V4<R>::type &db, R xc,
+                                              const typename V2<R>::type &rot_n, bool live, R (&wr)[M + 1],
+                                              R (&wi)[M + 1])
+{
+  const R ir2 = ir * ir;
+  const R ba = da.x * dx + da.y * dy + da.z * dz;
+  const R bb = db.x * dx + db.y * dy + db.z * dz;
+  const R aabh = dah.x * db.x + dah.y * db.y + dah.z * db.z;
+  const R bq = ba * bb * ir2;
+  const R g3h = live ? fmaT(R(-1.5), bq, aabh) * ir2 : R(0);
+  const R g1 = live ? fmaT(R(2), aabh, -bq) * ir : R(0);
+  // the pair term's q0, q1, q2 coefficients (pair_term):
+  //   A = g3h (ir + i theta) - g1 theta^2/2,  B = -g3h - i theta g1,  C = g1
+  R ar, ai, br, bi, cr, ci;
+  if constexpr(LAB) {
+    ar = g3h * ir;
+    ai = 0;
+    br = -g3h;
+    bi = 0;
+    cr = g1;
+    ci = 0;
+  } else {
+    const R a_r = fmaT(g3h, ir, -(g1 * K.oo)), a_i = g3h * K.theta;
+    const R b_i = -(K.theta * g1);
+    R sn, cs;
+    if constexpr(sizeof(R) == sizeof(double)) {
+      if constexpr(GEN)
+        sincos(K.theta * xc, &sn, &cs);
+      else
+        sincos_centred64(K.tg, xc, sn, cs);
+    } else {
+      if constexpr(GEN)
+        __sincosf(K.theta * xc, &sn, &cs);
+      else
+        sincos_theta(K.sn, K.cs, xc, sn, cs);
+    }
+    // rotation e^{-i theta (n + 1/2)} e^{-i theta xc}
+    const R rr = fmaT(rot_n.x, cs, rot_n.y * sn);
+    const R ri = fmaT(rot_n.y, cs, -(rot_n.x * sn));
+    ar = fmaT(rr, a_r, -(ri * a_i));
+    ai = fmaT(rr, a_i, ri * a_r);
+    br = fmaT(-rr, g3h, -(ri * b_i));
+    bi = fmaT(rr, b_i, -(ri * g3h));
+    cr = rr * g1;
+    ci = ri * g1;
+  }
+  const R x = xc + R(0.5);
+  R pw[M + 1];
+  pw[0] = R(1);
+#pragma unroll
+  for(int j = 1; j <= M; ++j) pw[j] = j == 1 ? x : pw[j - 1] * x;
+  R gr[M + 1], gi[M + 1];
+  gr[0] = ar;
+  gi[0] = ai;
+#pragma unroll
+  for(int j = 1; j <= M; ++j) {
+    const R q = j == 1 ? R(1) : R(j) * pw[j - 1];
+    gr[j] = fmaT(ar, pw[j], br * q);
+    gi[j] = fmaT(ai, pw[j], bi * q);
+    if(j >= 2) {
+      const R s = j == 2 ? R(1) : R(j * (j - 1) / 2) * pw[j - 2];
+      gr[j] = fmaT(cr, s, gr[j]);
+      gi[j] = fmaT(ci, s, gi[j]);
+    }
+  }
+  constexpr LagrangeMonomials<M> L{};
+#pragma unroll
+  for(int m = 0; m <= M; ++m) {
+    R sr = R(0), si = R(0);
+    bool first = true;
+#pragma unroll
+    for(int j = 0; j <= M; ++j) {
+      const double c = L.c[j][m];
+      if(c == 0.0) continue;
+      if(first) {
+        sr = R(c) * gr[j];
+        si = R(c) * gi[j];
+        first = false;
+      } else {
+        sr = fmaT(R(c), gr[j], sr);
+        si = fmaT(R(c), gi[j], si);
+      }
+    }
+    wr[m] = sr;
+    wi[m] = si;
+  }
+}
+
+// The time-blocked pair sum: for the K steps t0 .. t0 + K - 1 (t0 = the step
+// counter) and every ordered pair of delay n >= K, the weighted windows of
+// the source's samples, into partial slice j = 0..K-1 (slice j, row r, target
+// a at (j (max_chunks + 1) + r) n + a).  Same work list, items and claiming
+// as pair_past_*; FP32 (mixed) sums are folded into FP64 every 64 sources.
+template <typename R, int M, int K, bool LAB, bool DYN, bool GEN>
+__global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type *__restrict__ pd,
+                                                     const double4 *__restrict__ pos64,
+                                                     const typename V2<R>::type *__restrict__ hist,
+                                                     const typename V2<R>::type *__restrict__ rot,
+                                                     const int4 *__restrict__ work, const int *__restrict__ work_row,
+                                                     int *step_ptr, double2 *__restrict__ partial, Consts k)
+{
+  using R2 = typename V2<R>::type;
+  using R4 = typename V4<R>::type;
+  constexpr bool F64 = sizeof(R) == sizeof(double);
+  const PairK<R> Kc(k);
+  const int t0 = *step_ptr;
+  const int sm = (t0 - M) & k.hist_mask;  // window start of a delay-0 pair
+  const unsigned hmask = static_cast<unsigned>(k.hist_mask);
+  const unsigned row_len = static_cast<unsigned>(k.hist_row);
+  const size_t slice = static_cast<size_t>(k.max_chunks + 1) * k.n;
+  const float ms = kRoundMagic + static_cast<float>(sm);
+  const int code0 = kRoundMagicBits + sm;  // n = code0 - code (FP32 split)
+  ItemCursor<DYN> items(step_ptr + 1);
+  for(int w = items.first(); w < k.n_work; w = items.next()) {
+    const int4 wi = work[w];
+    const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
+    const int a = tgt < wi.y ? tgt : wi.y - 1;
+    const R4 pa = pd[2 * a], da = pd[2 * a + 1];
+    R4 dah = da;
+    dah.x = R(0.5) * da.x;
+    dah.y = R(0.5) * da.y;
+    dah.z = R(0.5) * da.z;
+    const double4 pa64 = pos64[a];
+    // one source into the step sums (sr, si)
+    auto source = [&](int b, R(&sr)[K], R(&si)[K]) {
+      const R4 pb = pd[2 * b], db = pd[2 * b + 1];
+      const R dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+      R xc, ir;
+      int n;
+      if constexpr(F64) {
+        n = split_l64(dx, dy, dz, pa64, pos64, b, Kc.cdt, xc, ir);
+      } else {
+        const int code = split_mixed(dx, dy, dz, Kc.eps_rel, Kc.lim0, ms, pa64, pos64, b, Kc.cdt, Kc.icdt64, xc, ir);
+        n = code0 - code;
+      }
+      // window of steps t0 - n - M .. t0 - n + K - 1 in the source's row
+      const unsigned first = (static_cast<unsigned>(sm - n) & hmask) + static_cast<unsigned>(b) * row_len;
+      const R2 *win = reinterpret_cast<const R2 *>(reinterpret_cast<const char *>(hist) +
+                                                   first * static_cast<unsigned>(sizeof(R2)));
+      R2 smp[K + M];
+#pragma unroll
+      for(int i = 0; i < K + M; ++i) smp[i] = win[i];
+      const R2 rn = rot[n];
+      R wr[M + 1], wim[M + 1];
+      block_weights<R, M, LAB, GEN>(Kc, ir, dx, dy, dz, da, dah, db, xc, rn, n >= K, wr, wim);
+      // step t0 + j reads sample t0 + j - n - m = window entry j - m + M
+#pragma unroll
+      for(int i = 0; i < K + M; ++i)
+#pragma unroll
+        for(int m = 0; m <= M; ++m) {
+          const int j = i + m - M;
+          if(j < 0 || j >= K) continue;
+          sr[j] = fmaT(wr[m], smp[i].x, sr[j]);
+          sr[j] = fmaT(-wim[m], smp[i].y, sr[j]);
+          si[j] = fmaT(wr[m], smp[i].y, si[j]);
+          si[j] = fmaT(wim[m], smp[i].x, si[j]);
+        }
+    };
+    double accr[K], acci[K];
+#pragma unroll
+    for(int j = 0; j < K; ++j) accr[j] = acci[j] = 0.0;
+    if constexpr(F64) {
+      for(int b = wi.z; b < wi.w; ++b) source(b, accr, acci);
+    } else {
+      // FP32 sums of up to 64 sources, folded into FP64
+      for(int bt = wi.z; bt < wi.w; bt += kChunk) {
+        const int be = min(wi.w, bt + kChunk);
+        R sr[K], si[K];
+#pragma unroll
+        for(int j = 0; j < K; ++j) sr[j] = si[j] = R(0);
+        for(int b = bt; b < be; ++b) source(b, sr, si);
+#pragma unroll
+        for(int j = 0; j < K; ++j) {
+          accr[j] += static_cast<double>(sr[j]);
+          acci[j] += static_cast<double>(si[j]);
+        }
+      }
+    }
+    if(tgt < wi.y) {
+      double2 *out = partial + static_cast<size_t>(work_row[w]) * k.n + tgt;
+#pragma unroll
+      for(int j = 0; j < K; ++j)
+        out[j * slice] = make_double2(accr[j] * k.unit_scale, acci[j] * k.unit_scale);
+    }
+  }
+  items.finish();
+}
+
+// The pairs of delay 1 <= n < K of every target (the short list, CSR by
+// target), at the current step: per pair the weights of block_weights and
+// the M + 1 samples it reads (all finished), summed in FP64.  One thread per
+// target.
+template <typename R, int M, bool LAB, bool GEN>
+__global__ __launch_bounds__(kBlock) void pair_short(const typename V4<R>::type *__restrict__ pd,
+                                                     const double4 *__restrict__ pos64,
+                                                     const typename V2<R>::type *__restrict__ hist,
+                                                     const typename V2<R>::type *__restrict__ rot,
+                                                     const int *__restrict__ short_start,
+                                                     const int *__restrict__ short_src, const int *step_ptr,
+                                                     double2 *__restrict__ out, Consts k)
+{
+  using R2 = typename V2<R>::type;
+  using R4 = typename V4<R>::type;
+  constexpr bool F64 = sizeof(R) == sizeof(double);
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if(a >= k.n) return;
+  const PairK<R> Kc(k);
+  const int step = *step_ptr;
+  const R4 pa = pd[2 * a], da = pd[2 * a + 1];
+  R4 dah = da;
+  dah.x = R(0.5) * da.x;
+  dah.y = R(0.5) * da.y;
+  dah.z = R(0.5) * da.z;
+  const double4 pa64 = pos64[a];
+  double accr = 0, acci = 0;
+  const int e1 = short_start[a + 1];
+  for(int e = short_start[a]; e < e1; ++e) {
+    const int b = short_src[e];
+    const R4 pb = pd[2 * b], db = pd[2 * b + 1];
+    const R dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+    R xc, ir;
+    int n;
+    if constexpr(F64) {
+      n = split_l64(dx, dy, dz, pa64, pos64, b, Kc.cdt, xc, ir);
+    } else {
+      const int code = split_mixed(dx, dy, dz, Kc.eps_rel, Kc.lim0, kRoundMagic, pa64, pos64, b, Kc.cdt,
+                                   Kc.icdt64, xc, ir);
+      n = kRoundMagicBits - code;
+    }
+    const R2 rn = rot[n];
+    R wr[M + 1], wim[M + 1];
+    block_weights<R, M, LAB, GEN>(Kc, ir, dx, dy, dz, da, dah, db, xc, rn, true, wr, wim);
+    const R2 *row = hist + static_cast<size_t>(b) * k.hist_row;
+    R tr = R(0), ti = R(0);
+#pragma unroll
+    for(int m = 0; m <= M; ++m) {
+      const R2 s = row[(step - n - m) & k.hist_mask];
+      tr = fmaT(wr[m], s.x, tr);
+      tr = fmaT(-wim[m], s.y, tr);
+      ti = fmaT(wr[m], s.y, ti);
+      ti = fmaT(wim[m], s.x, ti);
+    }
+    accr += static_cast<double>(tr);
+    acci += static_cast<double>(ti);
+  }
+  out[a] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
+}
+
 // Past-only pair sum from the coefficient table (the reference's evaluate,
 // history_interaction.cpp:47-70, as a gather: per target, sum over sources of
 // coef[m] rho01_b(step - n - m)).  Same work list, grid and partial layout as
@@ -709,22 +1015,32 @@ __device__ __forceinline__ double2 hist_scale(const Consts &k, const double2 *ph
   return make_double2(-k.k_green * ph.x, k.k_green * ph.y);
 }
 
-// Sum of the pair-kernel partials of target a over its system's source chunks.
-__device__ __forceinline__ double2 reduce_partials(const Consts &k, const DeviceState &d, int a)
+// Sum of the pair-kernel partials of target a over its system's source chunks
+// (rows of `partial`, n entries each).
+__device__ __forceinline__ double2 reduce_partials(const Consts &k, const DeviceState &d,
+                                                   const double2 *partial, int a)
 {
   double2 s = make_double2(0.0, 0.0);
   const int rows = d.sys_chunks[d.sys_of[a]];
   // unrolled so that several rows' loads are in flight; the sum stays in row order
 #pragma unroll 8
-  for(int j = 0; j < rows; ++j) s = cadd(s, d.partial[static_cast<size_t>(j) * k.n + a]);
+  for(int j = 0; j < rows; ++j) s = cadd(s, partial[static_cast<size_t>(j) * k.n + a]);
   return s;
 }
 
 // Field of the past pair sum at target a: the partials times -K e^{-i theta
-// step} (on-the-fly) or as they are (the table holds complete coefficients).
+// step} (on-the-fly), as they are (the table holds complete coefficients), or
+// (time-blocked) the slice of this step plus the short list's sum, times -K
+// (rotating; the rotation is in the weights) or K (lab).
 __device__ __forceinline__ double2 pair_field(const Consts &k, const DeviceState &d, int a, int step)
 {
-  const double2 sum = reduce_partials(k, d, a);
+  if(k.pair_blocked) {
+    const double2 *slice = d.partial + static_cast<size_t>(step % k.block_steps) * (k.max_chunks + 1) * k.n;
+    const double2 sum = cadd(reduce_partials(k, d, slice, a), d.short_sum[a]);
+    const double g = k.dyadic == QUEST_DYADIC_LAB ? k.k_green : -k.k_green;
+    return make_double2(g * sum.x, g * sum.y);
+  }
+  const double2 sum = reduce_partials(k, d, d.partial, a);
   return k.pair_table ? sum : cmul(sum, hist_scale(k, d.phase, step));
 }
 
@@ -842,11 +1158,25 @@ __device__ __forceinline__ double2 near_current(const Consts &k, const DeviceSta
 // dot with itself, zero-delay partners, which the near list evaluates), so
 // those terms vanish without a per-pair test; the slot's previous content
 // (step s + 1 - ring_slots) is older than any delay.
+//   Time-blocked method: `ring` is the sample ring instead, and slot s of
+// source a gets rho01(s) (2 Re rho01 in the lab frame) at entry s & hist_mask
+// of row a, and again hist_slots entries further while that is below
+// K + M - 1 (the window's contiguous tail).
 template <typename R2>
 __device__ __forceinline__ void build_slot(const Consts &k, const double2 *phase, R2 *ring,
                                            double2 *samp, const double2 *y1, double2 rho_s, int a,
                                            int s)
 {
+  if(k.pair_blocked) {
+    R2 v;
+    v.x = k.dyadic == QUEST_DYADIC_LAB ? 2.0 * rho_s.x : rho_s.x;
+    v.y = k.dyadic == QUEST_DYADIC_LAB ? 0.0 : rho_s.y;
+    R2 *row = ring + static_cast<size_t>(a) * k.hist_row;
+    const int e = s & k.hist_mask;
+    row[e] = v;
+    if(e < k.block_steps + k.order - 1) row[k.hist_slots + e] = v;
+    return;
+  }
   if(samp) samp[static_cast<size_t>(a) * k.samp_slots + (s & k.samp_mask)] = rho_s;
   // loops unrolled to the largest order (guards on k.order) keep smp in registers
   double2 smp[kMaxOrder + 1];
@@ -1143,21 +1473,24 @@ __global__ void advance_step_kernel(int *step) { *step += 1; }
 // ------------------------------------------------------- zero-delay list
 
 // Number (FILL = false) or list (FILL = true, in source order) of the sources
-// of each target whose pair the pair kernel of this precision classifies as
-// zero-delay.
+// b != a of each target whose pair the pair kernels of this precision
+// classify with an integer delay in [lo, hi): the zero-delay list ([0, 1))
+// and the time-blocked method's short list ([1, K)).
 template <bool F64, bool FILL>
-__global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState d)
+__global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState d, int lo, int hi,
+                                                           const int *__restrict__ start, int *__restrict__ src,
+                                                           int *__restrict__ tgt, int *__restrict__ counts)
 {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if(a >= k.n) return;
   int cnt = 0;
-  int base = FILL ? d.near_start[a] : 0;
+  int base = FILL ? start[a] : 0;
   const int2 range = d.sys_range[d.sys_of[a]];
   // FP32 prefilter: a pair whose FP32 distance (units of c0 dt, error at most
-  // eps_abs + eps_rel r) is at least 1 + 2 (eps_abs + eps_rel) + 1e-3 has delay
-  // >= 1 for certain and skips the exact split; the lists are unchanged.
+  // eps_abs + eps_rel r) is at least hi + 2 (eps_abs + eps_rel hi) + 1e-3 has
+  // delay >= hi for certain and skips the exact split; the lists are unchanged.
   const float4 pa32 = d.pd32[2 * a];
-  const double cut = 1.0 + 2.0 * (k.eps_abs + k.eps_rel) + 1e-3;
+  const double cut = hi + 2.0 * (k.eps_abs + k.eps_rel * hi) + 1e-3;
   const float cut2 = static_cast<float>(cut * cut);
   const double4 pa64 = d.pos64[a];
   const PairK<float> K(k);
@@ -1174,8 +1507,8 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
       n = kRoundMagicBits - split_mixed(pa32.x - pb.x, pa32.y - pb.y, pa32.z - pb.z, K.eps_rel, K.lim0,
                                         kRoundMagic, pa64, d.pos64, b, K.cdt, K.icdt64, x, ir);
     }
-    if(b != a && n < 1) {
-      if(FILL) d.near_src[base + cnt] = b;
+    if(b != a && n >= lo && n < hi) {
+      if(FILL) src[base + cnt] = b;
       ++cnt;
     }
   };
@@ -1197,9 +1530,9 @@ __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState
   }
   for(; b < range.y; ++b)
     if(!far(d.pd32[2 * b])) consider(b);
-  if(!FILL) d.counts[a] = cnt;
-  else
-    for(int e = 0; e < cnt; ++e) d.near_tgt[base + e] = a;
+  if(!FILL) counts[a] = cnt;
+  else if(tgt)
+    for(int e = 0; e < cnt; ++e) tgt[base + e] = a;
 }
 
 // Lagrange weights at x on nodes 0..M and their first and second x-derivatives,
@@ -1374,6 +1707,57 @@ void launch_pair_mld(const Consts &k, DeviceState &d, int precision, hipStream_t
   }
 }
 
+template <typename R, int M, bool LAB, bool DYN, bool GEN>
+void launch_block_mld(const Consts &k, DeviceState &d, hipStream_t s)
+{
+  using R2 = typename V2<R>::type;
+  using R4 = typename V4<R>::type;
+  const R4 *pd;
+  if constexpr(sizeof(R) == sizeof(double))
+    pd = d.pd64;
+  else
+    pd = d.pd32;
+  hipLaunchKernelGGL((pair_block<R, M, kBlockSteps, LAB, DYN, GEN>), dim3(k.grid), dim3(kBlock), 0, s, pd, d.pos64,
+                     static_cast<const R2 *>(d.hist), static_cast<const R2 *>(d.rot), d.work, d.work_row, d.step,
+                     d.partial, k);
+}
+
+template <typename R, int M>
+hipError_t launch_block_m(const Consts &k, DeviceState &d, hipStream_t s)
+{
+  if(k.dyadic == QUEST_DYADIC_LAB)
+    k.dyn_items ? launch_block_mld<R, M, true, true, false>(k, d, s) : launch_block_mld<R, M, true, false, false>(k, d, s);
+  else if(k.trig_poly)
+    k.dyn_items ? launch_block_mld<R, M, false, true, false>(k, d, s)
+                : launch_block_mld<R, M, false, false, false>(k, d, s);
+  else
+    k.dyn_items ? launch_block_mld<R, M, false, true, true>(k, d, s) : launch_block_mld<R, M, false, false, true>(k, d, s);
+  return hipGetLastError();
+}
+
+template <typename R, int M>
+hipError_t launch_short_m(const Consts &k, DeviceState &d, hipStream_t s)
+{
+  using R2 = typename V2<R>::type;
+  using R4 = typename V4<R>::type;
+  const R4 *pd;
+  if constexpr(sizeof(R) == sizeof(double))
+    pd = d.pd64;
+  else
+    pd = d.pd32;
+  const R2 *h = static_cast<const R2 *>(d.hist), *rt = static_cast<const R2 *>(d.rot);
+  if(k.dyadic == QUEST_DYADIC_LAB)
+    hipLaunchKernelGGL((pair_short<R, M, true, false>), dot_grid(k.n), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+                       d.short_start, d.short_src, d.step, d.short_sum, k);
+  else if(k.trig_poly)
+    hipLaunchKernelGGL((pair_short<R, M, false, false>), dot_grid(k.n), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+                       d.short_start, d.short_src, d.step, d.short_sum, k);
+  else
+    hipLaunchKernelGGL((pair_short<R, M, false, true>), dot_grid(k.n), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+                       d.short_start, d.short_src, d.step, d.short_sum, k);
+  return hipGetLastError();
+}
+
 template <int M>
 hipError_t launch_pair_m(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
@@ -1404,10 +1788,23 @@ hipError_t launch_table_m(const Consts &k, DeviceState &d, hipStream_t s)
 }
 
 template <int M>
-int blocks_per_cu_m(int precision, bool table, bool gen, bool pf, bool comp)
+int blocks_per_cu_m(int precision, bool table, bool gen, bool pf, bool comp, bool blocked)
 {
   int nb = 0;
   hipError_t e;
+  if(blocked) {
+    if(precision == QUEST_PREC_FP64)
+      e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_block<double, M, kBlockSteps, false, false, true>,
+                                                             kBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &nb, pair_block<double, M, kBlockSteps, false, false, false>, kBlock, 0);
+    else
+      e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_block<float, M, kBlockSteps, false, false, true>,
+                                                             kBlock, 0)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_block<float, M, kBlockSteps, false, false, false>,
+                                                             kBlock, 0);
+    return e == hipSuccess && nb > 0 ? nb : 1;
+  }
   // the FP32 variant of the context's loop (plain / pipelined, block /
   // compensated sums, short series / library sincos)
   auto f32 = [&](auto pf_c, auto comp_c) {
@@ -1498,6 +1895,18 @@ hipError_t launch_pair_past(const Consts &k, DeviceState &d, int precision, hipS
     default: return hipErrorInvalidValue;
     }
   }
+  if(k.pair_blocked) {
+    const bool f64 = precision == QUEST_PREC_FP64;
+    switch(k.order) {
+    case 1: return f64 ? launch_block_m<double, 1>(k, d, s) : launch_block_m<float, 1>(k, d, s);
+    case 2: return f64 ? launch_block_m<double, 2>(k, d, s) : launch_block_m<float, 2>(k, d, s);
+    case 3: return f64 ? launch_block_m<double, 3>(k, d, s) : launch_block_m<float, 3>(k, d, s);
+    case 4: return f64 ? launch_block_m<double, 4>(k, d, s) : launch_block_m<float, 4>(k, d, s);
+    case 5: return f64 ? launch_block_m<double, 5>(k, d, s) : launch_block_m<float, 5>(k, d, s);
+    case 6: return f64 ? launch_block_m<double, 6>(k, d, s) : launch_block_m<float, 6>(k, d, s);
+    default: return hipErrorInvalidValue;
+    }
+  }
   switch(k.order) {
   case 1: return launch_pair_m<1>(k, d, precision, s);
   case 2: return launch_pair_m<2>(k, d, precision, s);
@@ -1513,14 +1922,28 @@ int pair_blocks_per_cu(const Consts &k, int precision)
 {
   const bool t = k.pair_table != 0;
   const bool g = k.dyadic != QUEST_DYADIC_LAB && !k.trig_poly;
-  const bool pf = k.pipelined != 0, cp = k.compensated != 0;
+  const bool pf = k.pipelined != 0, cp = k.compensated != 0, bl = k.pair_blocked != 0;
   switch(k.order) {
-  case 1: return blocks_per_cu_m<1>(precision, t, g, pf, cp);
-  case 2: return blocks_per_cu_m<2>(precision, t, g, pf, cp);
-  case 3: return blocks_per_cu_m<3>(precision, t, g, pf, cp);
-  case 4: return blocks_per_cu_m<4>(precision, t, g, pf, cp);
-  case 5: return blocks_per_cu_m<5>(precision, t, g, pf, cp);
-  default: return blocks_per_cu_m<6>(precision, t, g, pf, cp);
+  case 1: return blocks_per_cu_m<1>(precision, t, g, pf, cp, bl);
+  case 2: return blocks_per_cu_m<2>(precision, t, g, pf, cp, bl);
+  case 3: return blocks_per_cu_m<3>(precision, t, g, pf, cp, bl);
+  case 4: return blocks_per_cu_m<4>(precision, t, g, pf, cp, bl);
+  case 5: return blocks_per_cu_m<5>(precision, t, g, pf, cp, bl);
+  default: return blocks_per_cu_m<6>(precision, t, g, pf, cp, bl);
+  }
+}
+
+hipError_t launch_pair_short(const Consts &k, DeviceState &d, int precision, hipStream_t s)
+{
+  const bool f64 = precision == QUEST_PREC_FP64;
+  switch(k.order) {
+  case 1: return f64 ? launch_short_m<double, 1>(k, d, s) : launch_short_m<float, 1>(k, d, s);
+  case 2: return f64 ? launch_short_m<double, 2>(k, d, s) : launch_short_m<float, 2>(k, d, s);
+  case 3: return f64 ? launch_short_m<double, 3>(k, d, s) : launch_short_m<float, 3>(k, d, s);
+  case 4: return f64 ? launch_short_m<double, 4>(k, d, s) : launch_short_m<float, 4>(k, d, s);
+  case 5: return f64 ? launch_short_m<double, 5>(k, d, s) : launch_short_m<float, 5>(k, d, s);
+  case 6: return f64 ? launch_short_m<double, 6>(k, d, s) : launch_short_m<float, 6>(k, d, s);
+  default: return hipErrorInvalidValue;
   }
 }
 
@@ -1529,10 +1952,10 @@ hipError_t launch_pc_step(const Consts &k, DeviceState &d, int precision, double
 {
   if(precision == QUEST_PREC_FP64)
     hipLaunchKernelGGL(pc_step_kernel<double2>, dim3(blocks), dim3(block), 0, s, k, d,
-                       static_cast<double2 *>(d.ring), future_coef, window, sync_blocks);
+                       static_cast<double2 *>(k.pair_blocked ? d.hist : d.ring), future_coef, window, sync_blocks);
   else
     hipLaunchKernelGGL(pc_step_kernel<float2>, dim3(blocks), dim3(block), 0, s, k, d,
-                       static_cast<float2 *>(d.ring), future_coef, window, sync_blocks);
+                       static_cast<float2 *>(k.pair_blocked ? d.hist : d.ring), future_coef, window, sync_blocks);
   return hipGetLastError();
 }
 
@@ -1547,10 +1970,10 @@ hipError_t launch_pc_iter(const Consts &k, DeviceState &d, int precision, double
 {
   if(precision == QUEST_PREC_FP64)
     hipLaunchKernelGGL(pc_iter_kernel<double2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
-                       static_cast<double2 *>(d.ring), future_coef, iter);
+                       static_cast<double2 *>(k.pair_blocked ? d.hist : d.ring), future_coef, iter);
   else
     hipLaunchKernelGGL(pc_iter_kernel<float2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
-                       static_cast<float2 *>(d.ring), future_coef, iter);
+                       static_cast<float2 *>(k.pair_blocked ? d.hist : d.ring), future_coef, iter);
   return hipGetLastError();
 }
 
@@ -1558,10 +1981,10 @@ hipError_t launch_hist_commit(const Consts &k, DeviceState &d, int precision, in
 {
   if(precision == QUEST_PREC_FP64)
     hipLaunchKernelGGL(hist_commit_kernel<double2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
-                       static_cast<double2 *>(d.ring), step);
+                       static_cast<double2 *>(k.pair_blocked ? d.hist : d.ring), step);
   else
     hipLaunchKernelGGL(hist_commit_kernel<float2>, dot_grid(k.n), dim3(kBlock), 0, s, k, d,
-                       static_cast<float2 *>(d.ring), step);
+                       static_cast<float2 *>(k.pair_blocked ? d.hist : d.ring), step);
   return hipGetLastError();
 }
 
@@ -1595,22 +2018,33 @@ hipError_t launch_pulse(const Consts &k, DeviceState &d, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t launch_delay_list(const Consts &k, DeviceState &d, int precision, int lo, int hi, bool fill,
+                             const int *start, int *src, int *tgt, int *counts, hipStream_t s)
+{
+  const bool f64 = precision == QUEST_PREC_FP64;
+  if(f64 && fill)
+    hipLaunchKernelGGL((near_scan_kernel<true, true>), dot_grid(k.n), dim3(kBlock), 0, s, k, d, lo, hi, start,
+                       src, tgt, counts);
+  else if(f64)
+    hipLaunchKernelGGL((near_scan_kernel<true, false>), dot_grid(k.n), dim3(kBlock), 0, s, k, d, lo, hi, start,
+                       src, tgt, counts);
+  else if(fill)
+    hipLaunchKernelGGL((near_scan_kernel<false, true>), dot_grid(k.n), dim3(kBlock), 0, s, k, d, lo, hi, start,
+                       src, tgt, counts);
+  else
+    hipLaunchKernelGGL((near_scan_kernel<false, false>), dot_grid(k.n), dim3(kBlock), 0, s, k, d, lo, hi, start,
+                       src, tgt, counts);
+  return hipGetLastError();
+}
+
 hipError_t launch_near_count(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
-  if(precision == QUEST_PREC_FP64)
-    hipLaunchKernelGGL((near_scan_kernel<true, false>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
-  else
-    hipLaunchKernelGGL((near_scan_kernel<false, false>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
-  return hipGetLastError();
+  return launch_delay_list(k, d, precision, 0, 1, false, nullptr, nullptr, nullptr, d.counts, s);
 }
 
 hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipStream_t s)
 {
-  if(precision == QUEST_PREC_FP64)
-    hipLaunchKernelGGL((near_scan_kernel<true, true>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
-  else
-    hipLaunchKernelGGL((near_scan_kernel<false, true>), dot_grid(k.n), dim3(kBlock), 0, s, k, d);
-  return hipGetLastError();
+  return launch_delay_list(k, d, precision, 0, 1, true, d.near_start, d.near_src, d.near_tgt, d.counts, s);
 }
 
 hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s)

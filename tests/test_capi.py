@@ -76,6 +76,6 @@ def test_struct_layouts_match_header():
                 names += [re.sub(r"\[\d+\]", "", v).strip() for v in m.group(1).split(",")]
         assert names == [f[0] for f in ctype._fields_], cname
     n_info = int(re.search(r"info:\s*\[(\d+)\]\s*int64", text).group(1))
-    assert n_info == 16
+    assert n_info == 20
     import inspect
     assert f"c_int64 * {n_info}" in inspect.getsource(capi.Context.info)

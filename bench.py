@@ -57,7 +57,8 @@ def fill_warmup(ctx, order=4):
     their cleared zeros the GPU holds a higher clock under its power limit
     (measured at N = 65536: 2.38 GHz with an all-zero ring, ~2.07 GHz once
     it is populated; 6.5 against 7.5 ms per mixed launch)."""
-    return max(10, int(ctx.info()["max_delay_bound"]) + order + 2)
+    warm = max(10, int(ctx.info()["max_delay_bound"]) + order + 2)
+    return (warm + 7) // 8 * 8  # the time-blocked method's passes start at multiples of 8
 
 
 def timed_steps(ctx, warmup, steps):
@@ -73,33 +74,37 @@ def timed_steps(ctx, warmup, steps):
     ctx.sync()
     ctx.set_timing(True)
     ctx.pair_kernel_time(reset=True)
+    ctx.short_kernel_time(reset=True)
     t0 = time.perf_counter()
     ctx.run(steps)              # returns after the stream has drained
     ctx.sync()
     secs = time.perf_counter() - t0
     kms, launches = ctx.pair_kernel_time()
+    sms, slaunches = ctx.short_kernel_time()
     ctx.set_timing(False)
-    ctx.run(1)                  # captures the step graph
+    ctx.run(8)                  # captures the step graph (one step, or the 8 steps of a time-blocked pass)
     ctx.sync()
     t0 = time.perf_counter()
     ctx.run(steps)
     ctx.sync()
     graph_secs = time.perf_counter() - t0
     return {"seconds": secs, "pair_ms": kms / max(launches, 1), "launches": launches,
+            "short_ms": sms / max(slaunches, 1), "short_launches": slaunches,
+            "steps_per_launch": max(1, int(ctx.info()["block_steps"])),
             "graph_ms_per_step": 1e3 * graph_secs / max(steps, 1), "warmup": warmup}
 
 
 def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0, tune=False):
-    """One context on the workload; with tune (the on-the-fly method only) the
-    work split is the autotuner's (quest_autotune, 3 launches per candidate,
-    before the warm-up), else the library's default; see timed_steps."""
-    from quest_amd import Context
+    """One context on the workload; with tune (not the table method) the work
+    split is the autotuner's (quest_autotune, 3 launches per candidate, before
+    the warm-up), else the library's default; see timed_steps."""
+    from quest_amd import PAIR_TABLE, Context
     ps, cs, fc = weights
     with Context(sysd, consts, precision=precision, pulse=pulse) as ctx:
         if method:
             ctx.set_pair_method(method)
         ctx.set_pc_weights(ps, cs, fc)
-        if tune and not method:
+        if tune and method != PAIR_TABLE:
             ctx.autotune(3)
         info = ctx.info()
         out = timed_steps(ctx, warmup, steps)
@@ -107,13 +112,15 @@ def run_gpu(sysd, consts, pulse, precision, warmup, steps, weights, method=0, tu
     return dict(out, rho=rho, info=info)
 
 
-def run_ensemble(ens, consts, precision, warmup, steps, weights):
+def run_ensemble(ens, consts, precision, warmup, steps, weights, method=0):
     """The batched workload: all systems in one context, work split autotuned
     before the warm-up; see timed_steps."""
     from quest_amd import Context
     ps, cs, fc = weights
     with Context([s for s, _ in ens], consts, precision=precision, pulse=[p for _, p in ens]) as ctx:
         ctx.set_pc_weights(ps, cs, fc)
+        if method:
+            ctx.set_pair_method(method)
         chunk = ctx.autotune(3)
         out = timed_steps(ctx, warmup, steps)
         info = ctx.info()
@@ -129,10 +136,11 @@ def ensemble_main(args):
     sizes = np.array([len(s["pos"]) for s, _ in ens], dtype=np.float64)
     pairs = float((sizes * sizes).sum())
     ordered = float((sizes * (sizes - 1)).sum())
-    mixed = run_ensemble(ens, consts, 1, args.warmup, args.steps, weights)
-    fp64 = None if args.no_fp64 else run_ensemble(ens, consts, 0, args.warmup, args.steps, weights)
+    method = 2 if args.pair_method == "time-blocked" else 0
+    mixed = run_ensemble(ens, consts, 1, args.warmup, args.steps, weights, method)
+    fp64 = None if args.no_fp64 else run_ensemble(ens, consts, 0, args.warmup, args.steps, weights, method)
     value = pairs * args.steps / mixed["seconds"]
-    achieved = F_ALG * ordered / (mixed["pair_ms"] * 1e-3) / 1e12
+    achieved = F_ALG * ordered * mixed["steps_per_launch"] / (mixed["pair_ms"] * 1e-3) / 1e12
     out = {
         "metric": "pair-interactions/s (sum of N^2 * steps / s), 256 batched systems",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
@@ -143,6 +151,7 @@ def ensemble_main(args):
                                "random dipoles, 2 rad/ps detuning, pulse area uniform in [pi/4, pi], "
                                "dt=1e-4 ps, order 4, 5000-step run",
                    "systems": 256, "dots": int(sizes.sum()), "pairs_per_step": pairs,
+                   "pair_method": "time-blocked (8 steps per pass)" if method else "on-the-fly",
                    "source_chunk_autotuned": mixed["chunk"], "work_items": mixed["info"]["work_items"],
                    "persistent_grid": mixed["info"]["grid"],
                    "dynamic_items": mixed["info"]["dynamic_items"]},
@@ -150,16 +159,19 @@ def ensemble_main(args):
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK, "traffic": None, "fma_issue_ceiling": FMA_CEILING,
                      "frac_of_fma_issue_ceiling": achieved / FMA_CEILING,
-                     "kernel": "pair_past_f32<4, false, {}, false, {}, false>".format(
-                         "true" if mixed["info"]["dynamic_items"] else "false",
-                         "true" if mixed["info"]["pipelined"] else "false"),
-                     "kernel_ms": mixed["pair_ms"], "flop_per_pair": F_ALG},
+                     "kernel": kernel_name(mixed["info"], 1, method),
+                     "kernel_ms": mixed["pair_ms"], "steps_per_launch": mixed["steps_per_launch"],
+                     "flop_per_pair": F_ALG},
     }
+    if method:
+        out["short_list"] = {"kernel": kernel_name(mixed["info"], 1, method, short=True),
+                             "us_per_step": 1e3 * mixed["short_ms"], "pairs": mixed["info"]["short_pairs"]}
     if fp64 is not None:
         v64 = pairs * args.steps / fp64["seconds"]
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
                        "graph_ms_per_step": fp64["graph_ms_per_step"],
-                       "pair_kernel_ms": fp64["pair_ms"], "source_chunk_autotuned": fp64["chunk"]}
+                       "pair_kernel_ms": fp64["pair_ms"], "steps_per_launch": fp64["steps_per_launch"],
+                       "source_chunk_autotuned": fp64["chunk"]}
         out["mixed_over_fp64"] = value / v64
         out["max_abs_drho_vs_fp64"] = float(np.abs(mixed["rho"] - fp64["rho"]).max())
         out["max_abs_rho01"] = float(np.abs(fp64["rho"][:, 1]).max())
@@ -360,7 +372,20 @@ def cpu_best(steps=10):
                        f"on {threads} threads (setup excluded)")}
 
 
-def driven_error(sysd, consts, weights, steps=100):
+def kernel_name(info, precision, method, short=False):
+    """The pair kernel (or the time-blocked method's short-list kernel) a
+    context of the bench's workload launches (order 4, rotating frame, short
+    rotation series)."""
+    real = "double" if precision == 0 else "float"
+    dyn = "true" if info["dynamic_items"] else "false"
+    if method == 2:
+        return f"pair_short<{real}, 4, false, false>" if short else f"pair_block<{real}, 4, 8, false, {dyn}, false>"
+    if precision == 0:
+        return f"pair_past_f64<4, false, {dyn}, false>"
+    return "pair_past_f32<4, false, {}, false, {}, false>".format(dyn, "true" if info["pipelined"] else "false")
+
+
+def driven_error(sysd, consts, weights, steps=100, method=0):
     """Mixed against FP64 in a driven state: the same system under a short,
     strong pulse (pi/2 area, sigma 15 steps, centred at step 50), so that
     |rho01| is large within the 100 steps; rho01 error relative to the FP64
@@ -374,6 +399,8 @@ def driven_error(sysd, consts, weights, steps=100):
     for precision in (0, 1):
         with Context(sysd, consts, precision=precision, pulse=pl) as ctx:
             ctx.set_pc_weights(*weights)
+            if method:
+                ctx.set_pair_method(method)
             ctx.run(steps)
             rho[precision], _ = ctx.state()
     err01 = rho01_rel_err(rho[1], rho[0])
@@ -402,8 +429,12 @@ def main(argv=None) -> int:
                          "default split; measured within 1 %% of the tuned one in sustained runs at N = 65536)")
     ap.add_argument("--no-driven", action="store_true",
                     help="skip the driven-state (short strong pulse) mixed-vs-FP64 error runs")
-    ap.add_argument("--pair-method", choices=("on-the-fly", "table"), default="on-the-fly",
-                    help="table: the FP64 coefficient-table method beside FP64 on the fly")
+    ap.add_argument("--pair-method", choices=("time-blocked", "on-the-fly", "table"), default="time-blocked",
+                    help="time-blocked (default): one pass per 8 steps (QUEST_PAIR_TIME_BLOCKED), with the "
+                         "per-step on-the-fly method reported beside it; on-the-fly: one pass per step; "
+                         "table: the FP64 coefficient-table method beside FP64 on the fly")
+    ap.add_argument("--no-per-step", action="store_true",
+                    help="time-blocked: skip the per-step method's comparison runs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -415,7 +446,7 @@ def main(argv=None) -> int:
         return ensemble_main(args)
     if args.pair_method == "table":
         return table_main(args)
-    from quest_amd import pc_weights, systems
+    from quest_amd import PAIR_ON_THE_FLY, PAIR_TIME_BLOCKED, pc_weights, systems
     sysd, full_steps = systems.workload(args.workload)
     consts = systems.consts()
     pulse = systems.pulse(math.pi / 2)
@@ -427,16 +458,18 @@ def main(argv=None) -> int:
     if args.copy_probe:
         copy_probe(args.copy_probe << 20)
     tune = args.tune
-    mixed = run_gpu(sysd, consts, pulse, 1, args.warmup, args.steps, weights, tune=tune)
-    fp64 = None if args.no_fp64 else run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights, tune=tune)
+    method = PAIR_TIME_BLOCKED if args.pair_method == "time-blocked" else PAIR_ON_THE_FLY
+    mixed = run_gpu(sysd, consts, pulse, 1, args.warmup, args.steps, weights, method=method, tune=tune)
+    fp64 = (None if args.no_fp64 else
+            run_gpu(sysd, consts, pulse, 0, args.warmup, args.steps, weights, method=method, tune=tune))
 
     pairs = float(n) * n
     value = pairs * args.steps / mixed["seconds"]
     ms_per_step = 1e3 * mixed["seconds"] / args.steps
     ordered = float(n) * (n - 1)
-    achieved = F_ALG * ordered / (mixed["pair_ms"] * 1e-3) / 1e12
-    kname = "pair_past_f32<4, false, {}, false, {}, false>".format(
-        "true" if mixed["info"]["dynamic_items"] else "false", "true" if mixed["info"]["pipelined"] else "false")
+    # algorithmic flops of one launch: F_ALG per ordered pair and step, times the steps one launch forms
+    achieved = F_ALG * ordered * mixed["steps_per_launch"] / (mixed["pair_ms"] * 1e-3) / 1e12
+    kname = kernel_name(mixed["info"], 1, method)
     out = {
         "metric": f"pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N={n}, ρ err≤1e-6",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": args.steps,
@@ -446,6 +479,9 @@ def main(argv=None) -> int:
         "config": {"workload": f"{args.workload}: N={n} dots at 512 um^-3, random dipoles, "
                                f"2 rad/ps detuning, pi/2 pulse, dt=1e-4 ps, order 4, "
                                f"{full_steps}-step run", "n_dots": n,
+                   "pair_method": ("time-blocked: one pass per 8 steps over the pairs of delay >= 8, "
+                                   "a per-step short list for delays 1-7" if method == PAIR_TIME_BLOCKED
+                                   else "on-the-fly, one pass per step"),
                    "source_chunk": mixed["info"]["source_chunk"],
                    "source_chunks_per_tile": mixed["info"]["splits"],
                    "persistent_grid": mixed["info"]["grid"],
@@ -455,22 +491,31 @@ def main(argv=None) -> int:
                    "drive": "none: pulse amplitude 0, every ring slot holds zeros" if args.zero_drive
                             else "pi/2 pulse",
                    "ring_slots": mixed["info"]["ring_slots"],
-                   "zero_delay_pairs": mixed["info"]["near_pairs"]},
+                   "zero_delay_pairs": mixed["info"]["near_pairs"],
+                   "short_list_pairs": mixed["info"]["short_pairs"]},
         "graph_ms_per_step": mixed["graph_ms_per_step"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK,
                      "traffic": profiled_traffic(kname) if args.workload == "n65536" else None,
                      "kernel": kname, "kernel_ms": mixed["pair_ms"],
+                     "steps_per_launch": mixed["steps_per_launch"],
+                     "kernel_ms_per_step": mixed["pair_ms"] / mixed["steps_per_launch"],
                      "flop_per_pair": F_ALG, "fma_issue_ceiling": FMA_CEILING,
                      "frac_of_fma_issue_ceiling": achieved / FMA_CEILING},
     }
+    if method == PAIR_TIME_BLOCKED:
+        out["short_list"] = {"kernel": kernel_name(mixed["info"], 1, method, short=True),
+                             "us_per_step": 1e3 * mixed["short_ms"], "pairs": mixed["info"]["short_pairs"]}
     if fp64 is not None:
         v64 = pairs * args.steps / fp64["seconds"]
-        a64 = F_ALG * ordered / (fp64["pair_ms"] * 1e-3) / 1e12
+        a64 = F_ALG * ordered * fp64["steps_per_launch"] / (fp64["pair_ms"] * 1e-3) / 1e12
         out["fp64"] = {"value": v64, "ms_per_step": 1e3 * fp64["seconds"] / args.steps,
                        "graph_ms_per_step": fp64["graph_ms_per_step"],
-                       "pair_kernel_ms": fp64["pair_ms"], "achieved_tflops": a64,
-                       "frac_of_fp64_peak": a64 / FP64_PEAK,
+                       "pair_kernel": kernel_name(fp64["info"], 0, method),
+                       "pair_kernel_ms": fp64["pair_ms"],
+                       "pair_kernel_ms_per_step": fp64["pair_ms"] / fp64["steps_per_launch"],
+                       "achieved_tflops": a64, "frac_of_fp64_peak": a64 / FP64_PEAK,
+                       "short_list_us_per_step": 1e3 * fp64["short_ms"],
                        "source_chunk": fp64["info"]["source_chunk"], "persistent_grid": fp64["info"]["grid"],
                        "dynamic_items": fp64["info"]["dynamic_items"]}
         out["mixed_over_fp64"] = value / v64
@@ -484,7 +529,20 @@ def main(argv=None) -> int:
                 "seed": 8, "max_abs_drho_vs_fp64": float(np.abs(mixed["rho"][sub] - fp64["rho"][sub]).max()),
                 "rel_rho01_err_vs_fp64": rho01_rel_err(mixed["rho"][sub], fp64["rho"][sub])}
         if not args.no_driven:
-            out["driven"] = driven_error(sysd, consts, weights)
+            out["driven"] = driven_error(sysd, consts, weights, method=method)
+    if method == PAIR_TIME_BLOCKED and not args.no_per_step:
+        # the per-step on-the-fly method on the same workload, for comparison
+        out["per_step_method"] = {}
+        for prec, key in ((1, "mixed"), (0, "fp64")):
+            if prec == 0 and args.no_fp64:
+                continue
+            r = run_gpu(sysd, consts, pulse, prec, args.warmup, args.steps, weights, tune=tune)
+            a = F_ALG * ordered / (r["pair_ms"] * 1e-3) / 1e12
+            out["per_step_method"][key] = {
+                "value": pairs * args.steps / r["seconds"], "ms_per_step": 1e3 * r["seconds"] / args.steps,
+                "pair_kernel": kernel_name(r["info"], prec, PAIR_ON_THE_FLY), "pair_kernel_ms": r["pair_ms"],
+                "achieved_tflops": a, "rel_rho01_vs_time_blocked_fp64":
+                    rho01_rel_err(r["rho"], fp64["rho"]) if fp64 is not None else None}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
         out["cpu_best"] = cpu_best()

@@ -890,8 +890,12 @@ __global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type 
 
 // The pairs of delay 1 <= n < K of every target (the short list, CSR by
 // target), at the current step: per pair the weights of block_weights and
-// the M + 1 samples it reads (all finished), summed in FP64.  One thread per
-// target.
+// the M + 1 samples it reads (all finished), summed in FP64.  kShortLanes
+// consecutive lanes share a target and take every kShortLanes-th pair of its
+// list (a chain of dependent loads per pair: list entry, source, samples),
+// then add their sums by a butterfly over the lanes.
+constexpr int kShortLanes = 8;
+
 template <typename R, int M, bool LAB, bool GEN>
 __global__ __launch_bounds__(kBlock) void pair_short(const typename V4<R>::type *__restrict__ pd,
                                                      const double4 *__restrict__ pos64,
@@ -904,8 +908,11 @@ __global__ __launch_bounds__(kBlock) void pair_short(const typename V4<R>::type 
   using R2 = typename V2<R>::type;
   using R4 = typename V4<R>::type;
   constexpr bool F64 = sizeof(R) == sizeof(double);
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if(a >= k.n) return;
+  static_assert(kWave % kShortLanes == 0, "a target's lanes lie in one wave");
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = gid % kShortLanes;
+  const bool valid = gid / kShortLanes < k.n;
+  const int a = valid ? gid / kShortLanes : k.n - 1;
   const PairK<R> Kc(k);
   const int step = *step_ptr;
   const R4 pa = pd[2 * a], da = pd[2 * a + 1];
@@ -915,8 +922,8 @@ __global__ __launch_bounds__(kBlock) void pair_short(const typename V4<R>::type 
   dah.z = R(0.5) * da.z;
   const double4 pa64 = pos64[a];
   double accr = 0, acci = 0;
-  const int e1 = short_start[a + 1];
-  for(int e = short_start[a]; e < e1; ++e) {
+  const int e1 = valid ? short_start[a + 1] : 0;
+  for(int e = short_start[a] + sub; e < e1; e += kShortLanes) {
     const int b = short_src[e];
     const R4 pb = pd[2 * b], db = pd[2 * b + 1];
     const R dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
@@ -945,7 +952,12 @@ __global__ __launch_bounds__(kBlock) void pair_short(const typename V4<R>::type 
     accr += static_cast<double>(tr);
     acci += static_cast<double>(ti);
   }
-  out[a] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
+#pragma unroll
+  for(int off = 1; off < kShortLanes; off <<= 1) {
+    accr += __shfl_xor(accr, off, kShortLanes);
+    acci += __shfl_xor(acci, off, kShortLanes);
+  }
+  if(valid && sub == 0) out[a] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
 }
 
 // Past-only pair sum from the coefficient table (the reference's evaluate,
@@ -1746,14 +1758,15 @@ hipError_t launch_short_m(const Consts &k, DeviceState &d, hipStream_t s)
   else
     pd = d.pd32;
   const R2 *h = static_cast<const R2 *>(d.hist), *rt = static_cast<const R2 *>(d.rot);
+  const dim3 grid(static_cast<unsigned>((static_cast<size_t>(k.n) * kShortLanes + kBlock - 1) / kBlock));
   if(k.dyadic == QUEST_DYADIC_LAB)
-    hipLaunchKernelGGL((pair_short<R, M, true, false>), dot_grid(k.n), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+    hipLaunchKernelGGL((pair_short<R, M, true, false>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt,
                        d.short_start, d.short_src, d.step, d.short_sum, k);
   else if(k.trig_poly)
-    hipLaunchKernelGGL((pair_short<R, M, false, false>), dot_grid(k.n), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+    hipLaunchKernelGGL((pair_short<R, M, false, false>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt,
                        d.short_start, d.short_src, d.step, d.short_sum, k);
   else
-    hipLaunchKernelGGL((pair_short<R, M, false, true>), dot_grid(k.n), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+    hipLaunchKernelGGL((pair_short<R, M, false, true>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt,
                        d.short_start, d.short_src, d.step, d.short_sum, k);
   return hipGetLastError();
 }

@@ -4,7 +4,7 @@
 //
 //   quest_hip [--config input.cfg] [--precision fp64|mixed]
 //             [--accumulation block|compensated] [--tune] [--host-loop]
-//             [--all-pulses] [--pair-method on-the-fly|table]
+//             [--all-pulses] [--pair-method time-blocked|on-the-fly|table]
 //             [--output output.dat] [--format text|binary] [--every K]
 //             [--subsample COUNT:SEED] [--device N] [--help] [--version]
 //
@@ -20,8 +20,10 @@
 // --all-pulses drives the dots with every row of pulse.cfg (one pulse
 // interaction each, fields added in file order) instead of the first row
 // only, which is what the reference reads (pulse.cpp:37-46).
-// --pair-method table streams the reference's per-pair coefficient table
-// (FP64 only) instead of evaluating the Green's function per pair.
+// --pair-method: time-blocked (default) forms the pair sums of the pairs of
+// delay >= 8 for 8 steps in one pass (QUEST_PAIR_TIME_BLOCKED); on-the-fly
+// evaluates every pair at every step; table streams the reference's per-pair
+// coefficient table (FP64 only).
 // --accumulation compensated makes mixed precision's 64-source FP32 block
 // sums Kahan-compensated; --tune times the device loop's work splits
 // (quest_autotune) before the first step.
@@ -108,6 +110,7 @@ struct Options {
 Options parse_configs(int argc, char *argv[])
 {
   Options o;
+  config.pair_method = QUEST_PAIR_TIME_BLOCKED;  // the driver's default (--pair-method)
   for(int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> std::string {
@@ -124,7 +127,7 @@ Options parse_configs(int argc, char *argv[])
                    "  --tune                 time the pair kernel's work splits before the run (device loop)\n"
                    "  --host-loop            host predictor-corrector, device interactions\n"
                    "  --all-pulses           every row of pulse.cfg (default: the first)\n"
-                   "  --pair-method arg      on-the-fly (default) or table (FP64 only)\n"
+                   "  --pair-method arg      time-blocked (default), on-the-fly or table (FP64 only)\n"
                    "  --output arg           output file (default output.dat)\n"
                    "  --format arg           text (default, output.dat's format) or binary\n"
                    "  --every arg            record every arg-th step (default 1)\n"
@@ -154,9 +157,10 @@ Options parse_configs(int argc, char *argv[])
       o.all_pulses = true;
     } else if(a == "--pair-method") {
       const std::string v = next();
-      if(v == "on-the-fly") config.pair_method = QUEST_PAIR_ON_THE_FLY;
+      if(v == "time-blocked") config.pair_method = QUEST_PAIR_TIME_BLOCKED;
+      else if(v == "on-the-fly") config.pair_method = QUEST_PAIR_ON_THE_FLY;
       else if(v == "table") config.pair_method = QUEST_PAIR_TABLE;
-      else throw std::runtime_error("--pair-method must be on-the-fly or table");
+      else throw std::runtime_error("--pair-method must be time-blocked, on-the-fly or table");
     } else if(a == "--output") {
       o.output = next();
     } else if(a == "--format") {

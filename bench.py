@@ -246,24 +246,29 @@ def copy_probe(nbytes: int) -> None:
         hip.hipFree(b)
 
 
-def profiled_traffic(kernel: str, profile: str = "pair_kernels_n65536_pmc.md"):
+PAIR_PMC_PROFILES = ("pair_block_n65536_pmc.md", "pair_kernels_n65536_pmc.md")
+
+
+def profiled_traffic(kernel: str, profile: str | None = None):
     """HBM bytes per launch of `kernel` from a committed rocprofv3 summary in
     profiles/ (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration):
-    the traffic_bytes column of the pair-kernel table, or HBM read + write
+    the traffic_bytes column of a pair-kernel table (the N = 65536 summaries
+    of both pair methods, unless `profile` names one), or HBM read + write
     bytes of the table-kernel one; None if absent."""
-    path = os.path.join(ROOT, "profiles", profile)
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("| `" + kernel + "`"):
-            cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            try:
-                if profile == "pair_kernels_n65536_pmc.md":
-                    return float(cells[-1])
-                if len(cells) == 7:  # the traffic table (the trace table above it has 5 columns)
-                    return float(cells[2]) + float(cells[3])
-            except (ValueError, IndexError):
-                return None
+    for prof in ((profile,) if profile else PAIR_PMC_PROFILES):
+        path = os.path.join(ROOT, "profiles", prof)
+        if not os.path.exists(path):
+            continue
+        for line in open(path):
+            if line.startswith("| `" + kernel + "`"):
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                try:
+                    if prof in PAIR_PMC_PROFILES:
+                        return float(cells[-1])
+                    if len(cells) == 7:  # the traffic table (the trace table above it has 5 columns)
+                        return float(cells[2]) + float(cells[3])
+                except (ValueError, IndexError):
+                    return None
     return None
 
 

@@ -12,8 +12,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_profiled_traffic_matches_bench_labels():
     sys.path.insert(0, ROOT)
     import bench
-    # the N = 65536 line's kernel (the default pipelined source loop)
-    t32 = bench.profiled_traffic("pair_past_f32<4, false, false, false, true, false>")
+    # the N = 65536 line's kernel (the time-blocked method's pass), and the
+    # per-step method's (the default pipelined source loop)
+    tb = bench.profiled_traffic(bench.kernel_name({"dynamic_items": 0, "pipelined": 1}, 1, 2))
+    assert tb is not None and 5e8 < tb < 1e10
+    t32 = bench.profiled_traffic(bench.kernel_name({"dynamic_items": 0, "pipelined": 1}, 1, 0))
     assert t32 is not None and 5e8 < t32 < 1e10
     ttab = bench.profiled_traffic("pair_table_f64<4, false>", "pair_table_n16384.md")
     assert ttab is not None and ttab > 1e10

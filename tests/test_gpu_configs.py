@@ -13,6 +13,9 @@ tests/test_golden.py):
   the oracle every 1000 steps in FP64 within 1e-10, and every system's mixed
   rho01 within 1e-6 of its FP64 rho01 (relative to the system's max |rho01|),
   populations within 1e-6 absolute.
+
+Both tests run the per-step pair method (QUEST_PAIR_ON_THE_FLY) and the
+time-blocked one (QUEST_PAIR_TIME_BLOCKED) to the same bounds.
 """
 import math
 import os
@@ -27,8 +30,12 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+METHODS = pytest.mark.parametrize("method", [0, 2], ids=["per_step", "time_blocked"])
+
+
+@METHODS
 @pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 1e-6)])
-def test_n64_full_run_matches_reference_method_fixture(precision, tol):
+def test_n64_full_run_matches_reference_method_fixture(precision, tol, method):
     from quest_amd import Context, systems
     g = np.load(os.path.join(GOLDEN, "n64_traj.npz"))
     sysd, steps = systems.workload("n64")
@@ -37,12 +44,13 @@ def test_n64_full_run_matches_reference_method_fixture(precision, tol):
     worst = 0.0
     with Context(sysd, systems.consts(), precision=precision, pulse=systems.pulse(math.pi / 2)) as ctx:
         ctx.set_pc_weights(g["ps"], g["cs"], float(g["fc"]))
+        ctx.set_pair_method(method)
         for r in range(steps // every):
             ctx.run(every)
             rho, drho = ctx.state()
             worst = max(worst, rel_err(rho, g["rec"][r]))
         final_d = rel_err(drho, g["yp"])
-    print(f"N=64 x {steps} steps, precision {precision}: worst rel err {worst:.3e} over "
+    print(f"N=64 x {steps} steps, method {method}, precision {precision}: worst rel err {worst:.3e} over "
           f"{steps // every} records, final derivative {final_d:.3e}, max|rho01| "
           f"{np.abs(g['rec'][:, :, 1]).max():.3f}")
     assert worst < tol
@@ -58,7 +66,8 @@ def _member_index():
 
 
 @pytest.mark.timeout(300)
-def test_ensemble_256_systems_5000_steps_both_precisions():
+@METHODS
+def test_ensemble_256_systems_5000_steps_both_precisions(method):
     from quest_amd import Context, systems
     g = np.load(os.path.join(GOLDEN, "ensemble_members.npz"))
     idx = _member_index()
@@ -74,6 +83,7 @@ def test_ensemble_256_systems_5000_steps_both_precisions():
     for precision in (0, 1):
         with Context(sys_list, consts, precision=precision, pulse=pulses) as ctx:
             ctx.set_pc_weights(g["ps"], g["cs"], float(g["fc"]))
+            ctx.set_pair_method(method)
             ctx.autotune(2)
             assert ctx.info()["systems"] == 256
             worst = 0.0
@@ -86,7 +96,7 @@ def test_ensemble_256_systems_5000_steps_both_precisions():
                         worst = max(worst, rel_err(rho[offs[i]:offs[i + 1]], g[f"rec{j}"][r]))
             final[precision] = rho
             if precision == 0:
-                print(f"FP64 members {list(idx)} (n = {[sizes[i] for i in idx]}) vs oracle: "
+                print(f"method {method}: FP64 members {list(idx)} (n = {[sizes[i] for i in idx]}) vs oracle: "
                       f"worst rel err {worst:.3e}")
                 assert worst < 1e-10
     e01, e00 = 0.0, 0.0

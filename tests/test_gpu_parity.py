@@ -369,8 +369,9 @@ def test_n4096_full_run_matches_oracle_fixture(precision, tol):
     assert worst < tol
 
 
+@pytest.mark.parametrize("method", [0, 2], ids=["per_step", "time_blocked"])
 @pytest.mark.parametrize("name", ["n65536", "n131072"])
-def test_full_size_mixed_matches_fp64(ref_weights, name):
+def test_full_size_mixed_matches_fp64(ref_weights, name, method):
     """The workloads of the measured configurations at their full sizes (the
     CPU oracle cannot run them: the reference method's table alone would be
     180 / 722 GB), mixed against FP64 on the GPU under a short, strong pulse
@@ -385,6 +386,7 @@ def test_full_size_mixed_matches_fp64(ref_weights, name):
     for precision in (0, 1):
         with Context(sysd, systems.consts(), precision=precision, pulse=pl) as ctx:
             ctx.set_pc_weights(*ref_weights)
+            ctx.set_pair_method(method)
             ctx.run(steps)
             rho, _ = ctx.state()
             info = ctx.info()
@@ -393,7 +395,8 @@ def test_full_size_mixed_matches_fp64(ref_weights, name):
     big = np.abs(out[0][:, 1]).max()
     err01 = np.abs(out[1][:, 1] - out[0][:, 1]).max() / big
     err00 = np.abs(out[1][:, 0] - out[0][:, 0]).max()
-    print(f"{name}: max|rho01| {big:.3f}, mixed vs FP64 rho01 {err01:.2e} relative, rho00 {err00:.2e}")
+    print(f"{name} method {method}: max|rho01| {big:.3f}, mixed vs FP64 rho01 {err01:.2e} relative, "
+          f"rho00 {err00:.2e}")
     assert big > 0.1
     assert np.all(np.isfinite(out[1]))
     assert err01 < 1e-6 and err00 < 1e-6
@@ -455,8 +458,9 @@ def test_run_every_order_matches_oracle(oracle_lib, ref_weights, order, precisio
 
 
 @pytest.mark.timeout(300)
+@pytest.mark.parametrize("method", [0, 2], ids=["per_step", "time_blocked"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_n131072_observers_match_oracle(precision):
+def test_n131072_observers_match_oracle(precision, method):
     """Full size, the N = 131072 workload: the per-call field at step 380 of a
     smooth synthetic rho01 history (every delay of the system, ~370 steps at
     most, reaches into it), at 4096 seeded observers, against the FP64
@@ -474,10 +478,11 @@ def test_n131072_observers_match_oracle(precision):
     n = sysd["pos"].shape[0]
     hist = smooth_history(n, OBS_STEP + 1)
     with Context(sysd, systems.consts(), precision=precision) as ctx:
+        ctx.set_pair_method(method)
         for s in range(OBS_STEP + 1):
             out = ctx.hist_evaluate(s, hist[s])
     obs = g["observers"]
     err = rel_err(out[obs], g["field"])
-    print(f"N={n} precision={precision}: {len(obs)} observers, rel err {err:.3e}, "
+    print(f"N={n} method={method} precision={precision}: {len(obs)} observers, rel err {err:.3e}, "
           f"max |field| {np.abs(g['field']).max():.3e}")
     assert err < (2e-11 if precision == 0 else 1e-6)

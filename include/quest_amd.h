@@ -251,7 +251,12 @@ int quest_run(quest_ctx *ctx, int n_steps);
  * on holds until the process ends, instead of the context's buffer.  Results
  * are those of quest_run, bit for bit.  QUEST_ERR_ARG if the partial sums do
  * not fit (at the default split: mixed above ~85k dots, FP64 above ~167k;
- * longer chunks via quest_params.source_chunk shrink them).  Every call on one GPU shares that GPU's array:
+ * longer chunks via quest_params.source_chunk shrink them; with
+ * QUEST_PAIR_TIME_BLOCKED there are 8 slices of them: mixed above ~10k dots,
+ * FP64 above ~20k).  With QUEST_PAIR_TIME_BLOCKED, a call that starts inside
+ * an 8-step pass whose partial sums the previous quest_run left in the
+ * context's buffer forms that pass again in the static array first (and
+ * quest_run does the reverse).  Every call on one GPU shares that GPU's array:
  * calls that overlap in time on one GPU (from several host threads) race and
  * must be serialised by the caller; a call has finished with the array when
  * it returns.  The pair kernel is not timed (quest_set_timing). */

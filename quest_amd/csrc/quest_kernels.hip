@@ -13,10 +13,15 @@
 //   state ring     double2 [state_slots][n] rho00, rho01 and derivatives, for
 //                  the predictor-corrector window
 //
+//   sample ring    (time-blocked method) rho01 itself per source and finished
+//                  step, rows with a mirrored tail, so that a window of K + M
+//                  steps is contiguous
+//
 // The pair sum is a gather: one thread per target dot, sources walked in
 // order.  Source data are wave-uniform (scalar loads); the ring read is a
-// per-lane gather of M + 1 complex values.  No MFMA: the work is a
-// gather-and-accumulate with ~90 VALU operations per ordered pair.
+// per-lane gather of M + 1 complex values (per step), or of K + M samples
+// (time-blocked, per 8 steps).  No MFMA: the work is a gather-and-accumulate,
+// ~78 VALU operations per ordered pair and step (per step), ~37 (time-blocked).
 #include <hip/hip_runtime.h>
 
 #include <climits>

@@ -58,7 +58,7 @@ def fill_warmup(ctx, order=4):
     (measured at N = 65536: 2.38 GHz with an all-zero ring, ~2.07 GHz once
     it is populated; 6.5 against 7.5 ms per mixed launch)."""
     warm = max(10, int(ctx.info()["max_delay_bound"]) + order + 2)
-    return (warm + 7) // 8 * 8  # the time-blocked method's passes start at multiples of 8
+    return (warm + 15) // 16 * 16  # the time-blocked method's passes start at multiples of 8 or 16
 
 
 def timed_steps(ctx, warmup, steps):
@@ -82,7 +82,7 @@ def timed_steps(ctx, warmup, steps):
     kms, launches = ctx.pair_kernel_time()
     sms, slaunches = ctx.short_kernel_time()
     ctx.set_timing(False)
-    ctx.run(8)                  # captures the step graph (one step, or the 8 steps of a time-blocked pass)
+    ctx.run(16)                 # captures the step graph (one step, or the 8 or 16 steps of a time-blocked pass)
     ctx.sync()
     t0 = time.perf_counter()
     ctx.run(steps)
@@ -151,7 +151,8 @@ def ensemble_main(args):
                                "random dipoles, 2 rad/ps detuning, pulse area uniform in [pi/4, pi], "
                                "dt=1e-4 ps, order 4, 5000-step run",
                    "systems": 256, "dots": int(sizes.sum()), "pairs_per_step": pairs,
-                   "pair_method": "time-blocked (8 steps per pass)" if method else "on-the-fly",
+                   "pair_method": (f"time-blocked ({mixed['steps_per_launch']} steps per pass)" if method
+                                   else "on-the-fly"),
                    "source_chunk_autotuned": mixed["chunk"], "work_items": mixed["info"]["work_items"],
                    "persistent_grid": mixed["info"]["grid"],
                    "dynamic_items": mixed["info"]["dynamic_items"]},
@@ -384,7 +385,9 @@ def kernel_name(info, precision, method, short=False):
     real = "double" if precision == 0 else "float"
     dyn = "true" if info["dynamic_items"] else "false"
     if method == 2:
-        return f"pair_short<{real}, 4, false, false>" if short else f"pair_block<{real}, 4, 8, false, {dyn}, false>"
+        if short:
+            return f"pair_short<{real}, 4, false, false>"
+        return f"pair_block<{real}, 4, {int(info.get('block_steps') or 8)}, false, {dyn}, false>"
     if precision == 0:
         return f"pair_past_f64<4, false, {dyn}, false>"
     return "pair_past_f32<4, false, {}, false, {}, false>".format(dyn, "true" if info["pipelined"] else "false")
@@ -418,7 +421,8 @@ def driven_error(sysd, consts, weights, steps=100, method=0):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=192,
+                    help="timed steps (default 192: whole passes of the time-blocked method)")
     ap.add_argument("--warmup", type=int, default=-1,
                     help="untimed steps before the timed ones; -1 (default): enough to fill the "
                          "interpolant ring's active window (max delay + order + 2 steps)")
@@ -435,7 +439,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-driven", action="store_true",
                     help="skip the driven-state (short strong pulse) mixed-vs-FP64 error runs")
     ap.add_argument("--pair-method", choices=("time-blocked", "on-the-fly", "table"), default="time-blocked",
-                    help="time-blocked (default): one pass per 8 steps (QUEST_PAIR_TIME_BLOCKED), with the "
+                    help="time-blocked (default): one pass per 8 or 16 steps (QUEST_PAIR_TIME_BLOCKED), with the "
                          "per-step on-the-fly method reported beside it; on-the-fly: one pass per step; "
                          "table: the FP64 coefficient-table method beside FP64 on the fly")
     ap.add_argument("--no-per-step", action="store_true",
@@ -484,9 +488,10 @@ def main(argv=None) -> int:
         "config": {"workload": f"{args.workload}: N={n} dots at 512 um^-3, random dipoles, "
                                f"2 rad/ps detuning, pi/2 pulse, dt=1e-4 ps, order 4, "
                                f"{full_steps}-step run", "n_dots": n,
-                   "pair_method": ("time-blocked: one pass per 8 steps over the pairs of delay >= 8, "
-                                   "a per-step short list for delays 1-7" if method == PAIR_TIME_BLOCKED
-                                   else "on-the-fly, one pass per step"),
+                   "pair_method": ("time-blocked: one pass per {0} steps over the pairs of delay >= {0}, "
+                                   "a per-step short list for delays 1-{1}".format(
+                                       mixed["steps_per_launch"], mixed["steps_per_launch"] - 1)
+                                   if method == PAIR_TIME_BLOCKED else "on-the-fly, one pass per step"),
                    "source_chunk": mixed["info"]["source_chunk"],
                    "source_chunks_per_tile": mixed["info"]["splits"],
                    "persistent_grid": mixed["info"]["grid"],

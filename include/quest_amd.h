@@ -94,6 +94,10 @@ typedef struct {
                           into FP64 per block, so the block sum's error no
                           longer grows with the block length).  FP64
                           contexts ignore it.                                */
+  int block_steps;     /* steps per pass of QUEST_PAIR_TIME_BLOCKED: 0 = auto
+                          (16 when the ordered pairs of delay below 16 are
+                          under 2 % of all -- large systems -- else 8), 8 or
+                          16.  The other pair methods ignore it.            */
 } quest_params;
 
 #define QUEST_CLAIM_AUTO 0
@@ -172,13 +176,13 @@ int quest_hist_evaluate(quest_ctx *ctx, int step, const double *rho01,
 #define QUEST_PAIR_TABLE 1
 /* TIME_BLOCKED: the same sum as ON_THE_FLY (either precision), reordered in
  *             time.  A pair of integer delay n reads, at step t, steps up to
- *             t - n only, so at every step t0 that is a multiple of K = 8 one
- *             pass over the pairs of delay >= K forms their sums for the K
- *             steps t0 .. t0 + K - 1: the pair's weights (geometry, Green's
- *             function, Lagrange weights) once, then K windows of the
- *             source's rho01 samples.  The pairs of delay 1 .. K - 1 are
- *             summed step by step from a short list, the zero-delay pairs as
- *             in the other methods.  Memory: a sample ring of n x (the
+ *             t - n only, so at every step t0 that is a multiple of K (8 or
+ *             16, quest_params.block_steps) one pass over the pairs of delay
+ *             >= K forms their sums for the K steps t0 .. t0 + K - 1: the
+ *             pair's weights (geometry, Green's function, Lagrange weights)
+ *             once, then K windows of the source's rho01 samples.  The pairs
+ *             of delay 1 .. K - 1 are summed step by step from a short list,
+ *             the zero-delay pairs as in the other methods.  Memory: a sample ring of n x (the
  *             largest delay + order + K, to a power of two, + K + order)
  *             complex values and K slices of the partial sums.  Both
  *             quest_run and quest_hist_evaluate drive it; quest_run replays

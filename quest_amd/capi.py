@@ -66,7 +66,8 @@ class Params(C.Structure):
                 ("laser_omega", C.c_double), ("dt", C.c_double), ("interp_order", C.c_int),
                 ("history_window", C.c_int), ("precision", C.c_int), ("dyadic", C.c_int),
                 ("device", C.c_int), ("source_chunk", C.c_int),
-                ("item_claim", C.c_int), ("source_loop", C.c_int), ("accumulation", C.c_int)]
+                ("item_claim", C.c_int), ("source_loop", C.c_int), ("accumulation", C.c_int),
+                ("block_steps", C.c_int)]
 
 
 class Pulse(C.Structure):
@@ -151,7 +152,7 @@ class Context:
 
     def __init__(self, system, consts, precision=PREC_FP64, dyadic=DYADIC_ROTATING,
                  order=4, window=22, device=0, source_chunk=0, pulse=None, item_claim=0, source_loop=0,
-                 accumulation=0):
+                 accumulation=0, block_steps=0):
         L = load()
         self._L = L
         self.ptr = None
@@ -173,7 +174,7 @@ class Context:
         self.sizes = sizes
         p = Params(consts["c0"], consts["hbar"], consts["mu0"], consts["laser_omega"],
                    consts["dt"], order, window, precision, dyadic, device, source_chunk, item_claim,
-                   source_loop, accumulation)
+                   source_loop, accumulation, block_steps)
         out = C.c_void_p()
         if batch:
             arr = (C.c_int * len(sizes))(*sizes)

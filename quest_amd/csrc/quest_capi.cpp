@@ -924,6 +924,8 @@ int check_params(const quest_params *p, int n)
     return fail(nullptr, QUEST_ERR_ARG, "source_loop must be QUEST_LOOP_AUTO, _PLAIN or _PIPELINED");
   if(p->accumulation != QUEST_ACC_BLOCK && p->accumulation != QUEST_ACC_COMPENSATED)
     return fail(nullptr, QUEST_ERR_ARG, "accumulation must be QUEST_ACC_BLOCK or QUEST_ACC_COMPENSATED");
+  if(p->block_steps != 0 && p->block_steps != kBlockStepsShort && p->block_steps != kBlockStepsLong)
+    return fail(nullptr, QUEST_ERR_ARG, "block_steps must be 0 (auto), 8 or 16");
   return QUEST_OK;
 }
 
@@ -1020,9 +1022,9 @@ void free_blocked(quest_ctx *c)
   c->hist_bytes = 0;
 }
 
-// Time-blocked mode: the rho01 sample ring, the rotation table
-// e^{-i theta (n + 1/2)}, and the short list (pairs of delay 1 .. K - 1, the
-// precision's own split, as the pair kernels classify them).  The ring holds
+// Time-blocked mode: the steps per pass K, the rho01 sample ring, the
+// rotation table e^{-i theta (n + 1/2)}, and the short list (pairs of delay
+// 1 .. K - 1, the precision's own split, as the pair kernels classify them).  The ring holds
 // more than the largest delay + order + K steps, so that the pass of the
 // current block can be formed again from it at any step of the block
 // (refill_block).
@@ -1030,14 +1032,40 @@ int enable_blocked(quest_ctx *c)
 {
   Consts &k = c->k;
   DeviceState &d = c->d;
-  const int M = c->P.interp_order, K = kBlockSteps;
+  const int M = c->P.interp_order;
   const bool f64 = c->P.precision == QUEST_PREC_FP64;
   const size_t r2 = f64 ? sizeof(double2) : sizeof(float2);
+  const size_t nn = static_cast<size_t>(c->n);
+  // steps per pass: the caller's, else 16 where the pairs of delay below 16
+  // (the short list of a 16-step pass) are under 2 % of all ordered pairs
+  std::vector<int> counts(nn);
+  auto count_short = [&](int hi) -> int64_t {
+    if(launch_delay_list(k, d, c->P.precision, 1, hi, false, nullptr, nullptr, nullptr, d.counts, c->stream) !=
+           hipSuccess ||
+       hipMemcpyAsync(counts.data(), d.counts, sizeof(int) * nn, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+       hipStreamSynchronize(c->stream) != hipSuccess)
+      return -1;
+    int64_t total = 0;
+    for(size_t i = 0; i < nn; ++i) total += counts[i];
+    return total;
+  };
+  int K = c->P.block_steps;
+  if(K == 0) {
+    const int64_t short16 = count_short(kBlockStepsLong);
+    if(short16 < 0) return fail(c, QUEST_ERR_HIP, "time-blocked method: short-list scan failed");
+    double ordered = 0;
+    for(int s = 0; s < c->n_sys; ++s) {
+      const double ns = c->sys_begin[s + 1] - c->sys_begin[s];
+      ordered += ns * (ns - 1);
+    }
+    K = static_cast<double>(short16) <= kLongPassShortFraction * ordered ? kBlockStepsLong : kBlockStepsShort;
+  }
+  if(K != kBlockStepsLong || c->P.block_steps != 0)  // the counts above are for [1, 16)
+    if(count_short(K) < 0) return fail(c, QUEST_ERR_HIP, "time-blocked method: short-list scan failed");
   k.block_steps = K;
   k.hist_slots = next_pow2(c->dmax_bound + M + K + 1);
   k.hist_mask = k.hist_slots - 1;
   k.hist_row = (k.hist_slots + K + M - 1 + 1) / 2 * 2;
-  const size_t nn = static_cast<size_t>(c->n);
   // the pair kernel addresses a window with a 32-bit byte offset
   if(nn * k.hist_row * r2 > 0xFFFFFFFFull)
     return fail(c, QUEST_ERR_ARG, "time-blocked method: dots x sample-ring row (" + std::to_string(nn) + " x " +
@@ -1064,10 +1092,6 @@ int enable_blocked(quest_ctx *c)
     else HIPC(c, hipMemcpyAsync(d.rot, hf.data(), sizeof(float2) * hf.size(), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));  // h, hf go out of scope
   }
-  HIPC(c, launch_delay_list(k, d, c->P.precision, 1, K, false, nullptr, nullptr, nullptr, d.counts, c->stream));
-  std::vector<int> counts(nn);
-  HIPC(c, hipMemcpyAsync(counts.data(), d.counts, sizeof(int) * nn, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
   std::vector<int> start(nn + 1, 0);
   for(size_t i = 0; i < nn; ++i) start[i + 1] = start[i] + counts[i];
   c->short_total = start[nn];

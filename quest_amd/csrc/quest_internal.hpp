@@ -17,7 +17,11 @@ constexpr int kMaxWindow = 64;     // PC history window limit
 constexpr int kBlock = 256;        // threads per block of the per-dot and pair kernels
 constexpr int kWave = 64;          // wavefront size; a pair-kernel work item is one wave of targets
 constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
-constexpr int kBlockSteps = 8;     // steps of one pass of the time-blocked pair sum
+// steps of one pass of the time-blocked pair sum: 8, or 16 where the pairs of
+// delay below 16 are few (quest_params.block_steps)
+constexpr int kBlockStepsShort = 8;
+constexpr int kBlockStepsLong = 16;
+constexpr double kLongPassShortFraction = 0.02;
 // e^{i theta s} tables: 2^10 entries each for the low and middle ten bits of
 // the step, 2^11 for the high bits (steps below 2^31)
 constexpr int kPhaseLevel = 1024;
@@ -85,7 +89,7 @@ struct Consts {
   // steps from one pass over the pairs of delay >= block_steps, a short list
   // for delays 1 .. block_steps - 1, and a rho01 sample ring
   int pair_blocked;   // 1: time-blocked pair sums
-  int block_steps;    // K, the steps of one pass (a launch at every step that is a multiple of K)
+  int block_steps;    // K, the steps of one pass (a launch at every step that is a multiple of K): 8 or 16
   int hist_slots;     // sample ring: power of two > max delay + order
   int hist_mask;
   int hist_row;       // entries per source row: hist_slots + K + order - 1, rounded up to even

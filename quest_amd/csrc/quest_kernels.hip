@@ -1734,9 +1734,13 @@ void launch_block_mld(const Consts &k, DeviceState &d, hipStream_t s)
     pd = d.pd64;
   else
     pd = d.pd32;
-  hipLaunchKernelGGL((pair_block<R, M, kBlockSteps, LAB, DYN, GEN>), dim3(k.grid), dim3(kBlock), 0, s, pd, d.pos64,
-                     static_cast<const R2 *>(d.hist), static_cast<const R2 *>(d.rot), d.work, d.work_row, d.step,
-                     d.partial, k);
+  const R2 *h = static_cast<const R2 *>(d.hist), *rt = static_cast<const R2 *>(d.rot);
+  if(k.block_steps == 16)
+    hipLaunchKernelGGL((pair_block<R, M, 16, LAB, DYN, GEN>), dim3(k.grid), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+                       d.work, d.work_row, d.step, d.partial, k);
+  else
+    hipLaunchKernelGGL((pair_block<R, M, 8, LAB, DYN, GEN>), dim3(k.grid), dim3(kBlock), 0, s, pd, d.pos64, h, rt,
+                       d.work, d.work_row, d.step, d.partial, k);
 }
 
 template <typename R, int M>
@@ -1805,22 +1809,23 @@ hipError_t launch_table_m(const Consts &k, DeviceState &d, hipStream_t s)
   return hipGetLastError();
 }
 
+template <int M, int K>
+hipError_t block_occupancy(int *nb, int precision, bool gen)
+{
+  if(precision == QUEST_PREC_FP64)
+    return gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, pair_block<double, M, K, false, false, true>, kBlock, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, pair_block<double, M, K, false, false, false>, kBlock, 0);
+  return gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, pair_block<float, M, K, false, false, true>, kBlock, 0)
+             : hipOccupancyMaxActiveBlocksPerMultiprocessor(nb, pair_block<float, M, K, false, false, false>, kBlock, 0);
+}
+
 template <int M>
-int blocks_per_cu_m(int precision, bool table, bool gen, bool pf, bool comp, bool blocked)
+int blocks_per_cu_m(int precision, bool table, bool gen, bool pf, bool comp, int blocked)
 {
   int nb = 0;
   hipError_t e;
   if(blocked) {
-    if(precision == QUEST_PREC_FP64)
-      e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_block<double, M, kBlockSteps, false, false, true>,
-                                                             kBlock, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &nb, pair_block<double, M, kBlockSteps, false, false, false>, kBlock, 0);
-    else
-      e = gen ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_block<float, M, kBlockSteps, false, false, true>,
-                                                             kBlock, 0)
-              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pair_block<float, M, kBlockSteps, false, false, false>,
-                                                             kBlock, 0);
+    e = blocked == 16 ? block_occupancy<M, 16>(&nb, precision, gen) : block_occupancy<M, 8>(&nb, precision, gen);
     return e == hipSuccess && nb > 0 ? nb : 1;
   }
   // the FP32 variant of the context's loop (plain / pipelined, block /
@@ -1940,7 +1945,8 @@ int pair_blocks_per_cu(const Consts &k, int precision)
 {
   const bool t = k.pair_table != 0;
   const bool g = k.dyadic != QUEST_DYADIC_LAB && !k.trig_poly;
-  const bool pf = k.pipelined != 0, cp = k.compensated != 0, bl = k.pair_blocked != 0;
+  const bool pf = k.pipelined != 0, cp = k.compensated != 0;
+  const int bl = k.pair_blocked ? k.block_steps : 0;
   switch(k.order) {
   case 1: return blocks_per_cu_m<1>(precision, t, g, pf, cp, bl);
   case 2: return blocks_per_cu_m<2>(precision, t, g, pf, cp, bl);

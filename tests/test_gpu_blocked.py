@@ -27,9 +27,13 @@ def _ctx(sysd, consts, weights, precision=0, dyadic=0, order=4, pulse=None, meth
     return ctx
 
 
+PASS_STEPS = pytest.mark.parametrize("block_steps", [8, 16], ids=["K8", "K16"])
+
+
+@PASS_STEPS
 @pytest.mark.parametrize("order", [2, 4, 6])
 @pytest.mark.parametrize("precision,dyadic", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_blocked_run_matches_oracle(oracle_lib, ref_weights, order, precision, dyadic):
+def test_blocked_run_matches_oracle(oracle_lib, ref_weights, order, precision, dyadic, block_steps):
     """A dense system whose delays (0 .. ~20 steps) fill the zero-delay list,
     the short list and the blocked pass, driven through a short strong
     pulse, 300 steps (a length that is not a multiple of 8)."""
@@ -43,14 +47,14 @@ def test_blocked_run_matches_oracle(oracle_lib, ref_weights, order, precision, d
     ref = oracle.solve(oracle_params(oracle, consts, order=order), sysd, oracle_pulse(oracle, pl), ps, cs, fc,
                        steps, mode=1, kind=dyadic)
     assert ref["status"] == 0
-    with _ctx(sysd, consts, ref_weights, precision, dyadic, order, pl) as ctx:
+    with _ctx(sysd, consts, ref_weights, precision, dyadic, order, pl, block_steps=block_steps) as ctx:
         info = ctx.info()
-        assert info["pair_method"] == 2 and info["block_steps"] == 8
+        assert info["pair_method"] == 2 and info["block_steps"] == block_steps
         assert info["short_pairs"] > 0 and info["near_pairs"] > 0
         ctx.run(steps)
         rho, drho = ctx.state()
     err = rel_err(rho, ref["y"]) if precision == 0 else float(np.abs(rho - ref["y"]).max())
-    print(f"blocked order {order} precision {precision} dyadic {dyadic}: err {err:.2e} "
+    print(f"blocked K={block_steps} order {order} precision {precision} dyadic {dyadic}: err {err:.2e} "
           f"short pairs {info['short_pairs']}")
     assert np.abs(ref["y"][:, 1]).max() > 0.1
     # measured: FP64 <= 9.2e-16, mixed <= 5e-10 (rotating) and 1.9e-11 (lab)
@@ -87,8 +91,9 @@ def test_blocked_equals_per_step_method(ref_weights, precision):
     assert err < (1e-13 if precision == 0 else 5e-9)
 
 
+@PASS_STEPS
 @pytest.mark.parametrize("precision", [0, 1])
-def test_blocked_runs_split_anywhere_are_bitwise_equal(ref_weights, precision):
+def test_blocked_runs_split_anywhere_are_bitwise_equal(ref_weights, precision, block_steps):
     """The same 45 steps in one call (graphs of 8 steps plus single steps), in
     calls that start and end inside blocks, with timing on (kernel by kernel)
     and per-call noalloc: identical states bit for bit."""
@@ -97,33 +102,34 @@ def test_blocked_runs_split_anywhere_are_bitwise_equal(ref_weights, precision):
     sysd = dense_system(150, 0.3, seed=81)
     pl = short_pulse()
     states = {}
-    with _ctx(sysd, consts, ref_weights, precision, pulse=pl) as ctx:
+    with _ctx(sysd, consts, ref_weights, precision, pulse=pl, block_steps=block_steps) as ctx:
         ctx.run(45)
         assert ctx.info()["graph"] == 1
         states["one"] = ctx.state()[0]
-    with _ctx(sysd, consts, ref_weights, precision, pulse=pl) as ctx:
+    with _ctx(sysd, consts, ref_weights, precision, pulse=pl, block_steps=block_steps) as ctx:
         for n in (3, 13, 1, 20, 8):
             ctx.run(n)
         states["split"] = ctx.state()[0]
-    with _ctx(sysd, consts, ref_weights, precision, pulse=pl) as ctx:
+    with _ctx(sysd, consts, ref_weights, precision, pulse=pl, block_steps=block_steps) as ctx:
         ctx.set_timing(True)
         ctx.run(45)
         ms, launches = ctx.pair_kernel_time()
         sms, slaunches = ctx.short_kernel_time()
-        assert launches == 6 and slaunches == 45 and ms > 0 and sms > 0
+        assert launches == (45 + block_steps - 1) // block_steps and slaunches == 45 and ms > 0 and sms > 0
         states["timed"] = ctx.state()[0]
-    with _ctx(sysd, consts, ref_weights, precision, pulse=pl) as ctx:
+    with _ctx(sysd, consts, ref_weights, precision, pulse=pl, block_steps=block_steps) as ctx:
         ctx.run(5)
-        ctx.run_noalloc(6)     # 5 .. 10: the block of step 8 in the static scratch
-        ctx.run(4)             # 11 .. 14: refilled into the context's buffer
+        ctx.run_noalloc(6)     # 5 .. 10: the pass under way, formed again in the static scratch
+        ctx.run(4)             # 11 .. 14: formed again in the context's buffer
         ctx.run_noalloc(30)
         states["noalloc"] = ctx.state()[0]
     for k in ("split", "timed", "noalloc"):
         assert np.array_equal(states[k], states["one"]), k
 
 
+@PASS_STEPS
 @pytest.mark.parametrize("precision", [0, 1])
-def test_blocked_per_call_matches_oracle(oracle_lib, ref_weights, precision):
+def test_blocked_per_call_matches_oracle(oracle_lib, ref_weights, precision, block_steps):
     """quest_hist_evaluate on a time-blocked context, driven as the
     reference's integrator drives evaluate (several calls per step, the last
     one with the step's final value): every evaluation against the oracle's
@@ -147,7 +153,7 @@ def test_blocked_per_call_matches_oracle(oracle_lib, ref_weights, precision):
     y[:, :, 0] = 1.0
     y[:, 22:, 1] = rho01
     worst = 0.0
-    with _ctx(sysd, consts, ref_weights, precision) as ctx:
+    with _ctx(sysd, consts, ref_weights, precision, block_steps=block_steps) as ctx:
         assert ctx.info()["short_pairs"] > 0
         for s in range(steps):
             for it in range(2):
@@ -159,19 +165,21 @@ def test_blocked_per_call_matches_oracle(oracle_lib, ref_weights, precision):
                 if s > 0:
                     worst = max(worst, rel_err(got, want))
             ctx.hist_evaluate(s, rho01[:, s])
-    print(f"per-call blocked precision {precision}: worst {worst:.2e}")
+    print(f"per-call blocked K={block_steps} precision {precision}: worst {worst:.2e}")
     # measured: FP64 1.9e-13, mixed 2.4e-7 (the per-step mixed kernel: up to 4.3e-7)
     assert worst < (1e-12 if precision == 0 else 1e-6)
 
 
-def test_blocked_autotune_keeps_the_run(ref_weights):
+@PASS_STEPS
+def test_blocked_autotune_keeps_the_run(ref_weights, block_steps):
     """quest_autotune inside a block (its timed passes overwrite the block's
     partial sums and change the split): the run continues as if untuned."""
     from quest_amd import systems
     consts = systems.consts()
     sysd = dense_system(300, 0.45, seed=93)
     pl = short_pulse()
-    with _ctx(sysd, consts, ref_weights, 0, pulse=pl) as a, _ctx(sysd, consts, ref_weights, 0, pulse=pl) as b:
+    with _ctx(sysd, consts, ref_weights, 0, pulse=pl, block_steps=block_steps) as a, \
+            _ctx(sysd, consts, ref_weights, 0, pulse=pl, block_steps=block_steps) as b:
         a.run(13)
         b.run(13)
         b.autotune(1)
@@ -228,9 +236,17 @@ def test_blocked_n4096_full_run_matches_oracle_fixture(precision, tol):
 
 def test_blocked_method_rules():
     """The method is chosen before the first step; noalloc refuses partial
-    sums beyond the static scratch."""
+    sums beyond the static scratch; the automatic pass length is 16 steps
+    for a large system (N = 65536: 0.3 % of its pairs have delays below 16)
+    and 8 for a dense small one; block_steps other than 0, 8, 16 is an
+    argument error."""
     from quest_amd import Context, QuestError, systems
     consts = systems.consts()
+    with pytest.raises(QuestError):
+        Context(dense_system(64, 0.3, seed=3), consts, block_steps=12)
+    with Context(dense_system(64, 0.3, seed=3), consts) as ctx:
+        ctx.set_pair_method(2)
+        assert ctx.info()["block_steps"] == 8
     with Context(dense_system(64, 0.3, seed=3), consts) as ctx:
         ctx.set_pair_method(2)
         ctx.set_pair_method(0)
@@ -243,5 +259,6 @@ def test_blocked_method_rules():
     sysd, _ = systems.workload("n65536")
     with Context(sysd, consts, precision=1) as ctx:
         ctx.set_pair_method(2)
+        assert ctx.info()["block_steps"] == 16
         with pytest.raises(QuestError, match="static scratch"):
             ctx.run_noalloc(1)

@@ -379,14 +379,14 @@ def cpu_best(steps=10):
 
 
 def kernel_name(info, precision, method, short=False):
-    """The pair kernel (or the time-blocked method's short-list kernel) a
+    """The pair kernel (or the time-blocked method's per-step short-list kernel) a
     context of the bench's workload launches (order 4, rotating frame, short
     rotation series)."""
     real = "double" if precision == 0 else "float"
     dyn = "true" if info["dynamic_items"] else "false"
     if method == 2:
         if short:
-            return f"pair_short<{real}, 4, false, false>"
+            return f"pair_list<{real}, 4, 1, false, false>"
         return f"pair_block<{real}, 4, {int(info.get('block_steps') or 8)}, false, {dyn}, false>"
     if precision == 0:
         return f"pair_past_f64<4, false, {dyn}, false>"

@@ -61,7 +61,8 @@ struct quest_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double2 *scratch = nullptr;  // this device's static scratch array (quest_run_noalloc)
-  int short_total = 0;         // time-blocked mode: pairs of delay 1 .. K - 1
+  int short_total = 0;         // time-blocked mode: pairs of delay 1 .. 7
+  int mid_total = 0;           // time-blocked mode, 16-step passes: pairs of delay 8 .. 15
   int slice_buf = 0;           // time-blocked mode: the current block's partial sums are in the
                                // context's buffer (0) or the static scratch (1)
   size_t hist_bytes = 0;       // time-blocked mode: the sample ring
@@ -471,9 +472,10 @@ int pair_sum(quest_ctx *c, DeviceState &d, bool timed)
 }
 int pair_sum(quest_ctx *c) { return pair_sum(c, c->d, true); }
 
-// Time-blocked method: the short list's sum of this step, bracketed by HIP
-// events when timing.
-int short_sum(quest_ctx *c, DeviceState &d, bool timed)
+// Time-blocked method: the list sums of step `step` -- the mid list's for 8
+// steps at the multiples of 8 (16-step passes), the short list's at every
+// step -- bracketed by HIP events when timing.
+int list_sums(quest_ctx *c, DeviceState &d, bool timed, int step)
 {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   timed = timed && c->timing;
@@ -484,7 +486,8 @@ int short_sum(quest_ctx *c, DeviceState &d, bool timed)
     c->ev_kind.push_back(1);
     HIPC(c, hipEventRecord(e0, c->stream));
   }
-  HIPC(c, launch_pair_short(c->k, d, c->P.precision, c->stream));
+  if(c->k.mid_steps && step % c->k.mid_steps == 0) HIPC(c, launch_pair_list(c->k, d, c->P.precision, true, c->stream));
+  HIPC(c, launch_pair_list(c->k, d, c->P.precision, false, c->stream));
   if(timed) HIPC(c, hipEventRecord(e1, c->stream));
   return QUEST_OK;
 }
@@ -497,7 +500,7 @@ int enqueue_step(quest_ctx *c, DeviceState &d, bool timed, int step)
 {
   int rc = QUEST_OK;
   if(!c->k.pair_blocked || step % c->k.block_steps == 0) rc = pair_sum(c, d, timed);
-  if(rc == QUEST_OK && c->k.pair_blocked) rc = short_sum(c, d, timed);
+  if(rc == QUEST_OK && c->k.pair_blocked) rc = list_sums(c, d, timed, step);
   if(rc != QUEST_OK) return rc;
   if(c->step_block > 0) {
     HIPC(c, launch_pc_step(c->k, d, c->P.precision, c->fc, c->P.history_window, c->step_block,
@@ -547,7 +550,8 @@ void free_device(quest_ctx *c)
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.io, d.sys_of, d.sys_range, d.sys_chunks,
                   d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff,
-                  d.step_map, d.near_lane, d.hist, d.rot, d.short_start, d.short_src, d.short_sum};
+                  d.step_map, d.near_lane, d.hist, d.rot, d.short_start, d.short_src, d.short_sum,
+                  d.mid_start, d.mid_src, d.mid_sum};
   for(void *p : ptrs)
     if(p) (void)hipFree(p);
   d = DeviceState{};
@@ -1011,15 +1015,45 @@ void free_blocked(quest_ctx *c)
 {
   DeviceState &d = c->d;
   for(void *p : {d.hist, d.rot, static_cast<void *>(d.short_start), static_cast<void *>(d.short_src),
-                 static_cast<void *>(d.short_sum)})
+                 static_cast<void *>(d.short_sum), static_cast<void *>(d.mid_start), static_cast<void *>(d.mid_src),
+                 static_cast<void *>(d.mid_sum)})
     if(p) (void)hipFree(p);
   d.hist = d.rot = nullptr;
-  d.short_start = d.short_src = nullptr;
-  d.short_sum = nullptr;
+  d.short_start = d.short_src = d.mid_start = d.mid_src = nullptr;
+  d.short_sum = d.mid_sum = nullptr;
   c->k.pair_blocked = 0;
   c->k.block_steps = 1;
+  c->k.mid_steps = 0;
   c->short_total = 0;
+  c->mid_total = 0;
   c->hist_bytes = 0;
+}
+
+// A list of every target's sources whose delay lies in [lo, hi) (the
+// precision's split), CSR by target: *start [n + 1] and *src on the device,
+// the number of pairs in *total.
+int build_list(quest_ctx *c, int lo, int hi, int **start_dev, int **src_dev, int *total)
+{
+  DeviceState &d = c->d;
+  const size_t nn = static_cast<size_t>(c->n);
+  HIPC(c, launch_delay_list(c->k, d, c->P.precision, lo, hi, false, nullptr, nullptr, nullptr, d.counts, c->stream));
+  std::vector<int> counts(nn);
+  HIPC(c, hipMemcpyAsync(counts.data(), d.counts, sizeof(int) * nn, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  std::vector<int> start(nn + 1, 0);
+  for(size_t i = 0; i < nn; ++i) {
+    if(start[i] > INT_MAX - counts[i]) return fail(c, QUEST_ERR_ARG, "time-blocked method: list beyond 2^31 pairs");
+    start[i + 1] = start[i] + counts[i];
+  }
+  *total = start[nn];
+  if(hipMalloc(reinterpret_cast<void **>(start_dev), sizeof(int) * (nn + 1)) != hipSuccess ||
+     hipMalloc(reinterpret_cast<void **>(src_dev), sizeof(int) * std::max(*total, 1)) != hipSuccess)
+    return fail(c, QUEST_ERR_NOMEM, "time-blocked method: list allocation failed");
+  HIPC(c, hipMemcpyAsync(*start_dev, start.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, launch_delay_list(c->k, d, c->P.precision, lo, hi, true, *start_dev, *src_dev, nullptr, d.counts,
+                            c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return QUEST_OK;
 }
 
 // Time-blocked mode: the steps per pass K, the rho01 sample ring, the
@@ -1037,7 +1071,7 @@ int enable_blocked(quest_ctx *c)
   const size_t r2 = f64 ? sizeof(double2) : sizeof(float2);
   const size_t nn = static_cast<size_t>(c->n);
   // steps per pass: the caller's, else 16 where the pairs of delay below 16
-  // (the short list of a 16-step pass) are under 2 % of all ordered pairs
+  // (the lists of a 16-step pass) are under 2 % of all ordered pairs
   std::vector<int> counts(nn);
   auto count_short = [&](int hi) -> int64_t {
     if(launch_delay_list(k, d, c->P.precision, 1, hi, false, nullptr, nullptr, nullptr, d.counts, c->stream) !=
@@ -1060,9 +1094,8 @@ int enable_blocked(quest_ctx *c)
     }
     K = static_cast<double>(short16) <= kLongPassShortFraction * ordered ? kBlockStepsLong : kBlockStepsShort;
   }
-  if(K != kBlockStepsLong || c->P.block_steps != 0)  // the counts above are for [1, 16)
-    if(count_short(K) < 0) return fail(c, QUEST_ERR_HIP, "time-blocked method: short-list scan failed");
   k.block_steps = K;
+  k.mid_steps = K == kBlockStepsLong ? kBlockStepsShort : 0;
   k.hist_slots = next_pow2(c->dmax_bound + M + K + 1);
   k.hist_mask = k.hist_slots - 1;
   k.hist_row = (k.hist_slots + K + M - 1 + 1) / 2 * 2;
@@ -1072,8 +1105,9 @@ int enable_blocked(quest_ctx *c)
                                       std::to_string(k.hist_row) + ") exceeds 4 GiB of samples");
   c->hist_bytes = (nn * k.hist_row * r2 + 15) / 16 * 16;
   if(hipMalloc(&d.hist, c->hist_bytes) != hipSuccess || hipMalloc(&d.rot, k.hist_slots * r2) != hipSuccess ||
-     hipMalloc(reinterpret_cast<void **>(&d.short_start), sizeof(int) * (nn + 1)) != hipSuccess ||
-     hipMalloc(reinterpret_cast<void **>(&d.short_sum), sizeof(double2) * std::max<size_t>(nn, 1)) != hipSuccess) {
+     hipMalloc(reinterpret_cast<void **>(&d.short_sum), sizeof(double2) * std::max<size_t>(nn, 1)) != hipSuccess ||
+     (k.mid_steps && hipMalloc(reinterpret_cast<void **>(&d.mid_sum),
+                               sizeof(double2) * k.mid_steps * std::max<size_t>(nn, 1)) != hipSuccess)) {
     free_blocked(c);
     return fail(c, QUEST_ERR_NOMEM, "time-blocked method: allocation failed");
   }
@@ -1092,16 +1126,14 @@ int enable_blocked(quest_ctx *c)
     else HIPC(c, hipMemcpyAsync(d.rot, hf.data(), sizeof(float2) * hf.size(), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));  // h, hf go out of scope
   }
-  std::vector<int> start(nn + 1, 0);
-  for(size_t i = 0; i < nn; ++i) start[i + 1] = start[i] + counts[i];
-  c->short_total = start[nn];
-  if(hipMalloc(reinterpret_cast<void **>(&d.short_src), sizeof(int) * std::max(c->short_total, 1)) != hipSuccess) {
+  // the short list (delays 1 .. 7); with 16-step passes the mid list (8 .. 15)
+  int rc = build_list(c, 1, kBlockStepsShort, &d.short_start, &d.short_src, &c->short_total);
+  if(rc == QUEST_OK && k.mid_steps) rc = build_list(c, kBlockStepsShort, K, &d.mid_start, &d.mid_src, &c->mid_total);
+  if(rc != QUEST_OK) {
+    const std::string msg = c->err;
     free_blocked(c);
-    return fail(c, QUEST_ERR_NOMEM, "time-blocked method: short list allocation failed");
+    return fail(c, rc, msg);
   }
-  HIPC(c, hipMemcpyAsync(d.short_start, start.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice, c->stream));
-  HIPC(c, launch_delay_list(k, d, c->P.precision, 1, K, true, d.short_start, d.short_src, nullptr, d.counts,
-                            c->stream));
   HIPC(c, launch_clear(d.hist, c->hist_bytes, c->stream));  // rho01 = 0 before step 0 (main.cpp:32-36)
   HIPC(c, hipStreamSynchronize(c->stream));
   k.pair_blocked = 1;
@@ -1247,7 +1279,7 @@ int quest_hist_evaluate(quest_ctx *c, int step, const double *rho01, double *out
     if(c->cur_step >= 0) HIPC(c, launch_hist_commit(c->k, c->d, c->P.precision, c->cur_step, c->stream));
     HIPC(c, launch_set_step(c->d, step, c->stream));
     int rc = (!c->k.pair_blocked || step % c->k.block_steps == 0) ? pair_sum(c) : QUEST_OK;
-    if(rc == QUEST_OK && c->k.pair_blocked) rc = short_sum(c, c->d, true);
+    if(rc == QUEST_OK && c->k.pair_blocked) rc = list_sums(c, c->d, true, step);
     if(rc != QUEST_OK) return rc;
     HIPC(c, launch_hist_reduce(c->k, c->d, c->stream));
     c->cur_step = step;

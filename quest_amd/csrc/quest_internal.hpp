@@ -93,6 +93,7 @@ struct Consts {
   int hist_slots;     // sample ring: power of two > max delay + order
   int hist_mask;
   int hist_row;       // entries per source row: hist_slots + K + order - 1, rounded up to even
+  int mid_steps;      // 8 with 16-step passes (the mid list, delays 8 .. 15, every 8 steps), else 0
 };
 
 struct PulseParams {
@@ -156,6 +157,9 @@ struct DeviceState {
   int *short_start = nullptr;          // [n + 1] short list (delays 1 .. K - 1), CSR by target
   int *short_src = nullptr;            // [short pairs]
   double2 *short_sum = nullptr;        // [n] the short list's sum at the current step
+  int *mid_start = nullptr;            // [n + 1] mid list (delays 8 .. 15, 16-step passes only), CSR by target
+  int *mid_src = nullptr;              // [mid pairs]
+  double2 *mid_sum = nullptr;          // [8][n] the mid list's sums for the 8 steps from its last launch
   // one-launch predictor-corrector step: thread -> dot (-1 idle), each group of
   // zero-delay partners within one block, all groups in the leading blocks
   int *step_map = nullptr;
@@ -207,8 +211,10 @@ hipError_t launch_near_count(const Consts &k, DeviceState &d, int precision, hip
 hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipStream_t s);
 hipError_t launch_delay_list(const Consts &k, DeviceState &d, int precision, int lo, int hi, bool fill,
                              const int *start, int *src, int *tgt, int *counts, hipStream_t s);
-// Time-blocked mode: the short list's sum at the current step into d.short_sum.
-hipError_t launch_pair_short(const Consts &k, DeviceState &d, int precision, hipStream_t s);
+// Time-blocked mode: the short list's sum at the current step into d.short_sum
+// (mid false), or the mid list's sums for the 8 steps from the current one
+// into d.mid_sum (mid true).
+hipError_t launch_pair_list(const Consts &k, DeviceState &d, int precision, bool mid, hipStream_t s);
 hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s);
 hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int precision, int32_t *out, int64_t first,
                               int64_t count, hipStream_t s);

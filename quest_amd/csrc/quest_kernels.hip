@@ -655,10 +655,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
 // of step t0 (steps < t0 finished) every pair with n >= K therefore holds all
 // it needs for the K steps t0 .. t0 + K - 1: its weights (geometry, Green's
 // function and Lagrange weights, the expensive part) are formed once and
-// applied to K consecutive windows of the source's rho01 samples.  The pairs
-// with 1 <= n < K (a short list, ~30 per dot at K = 8 and 512 um^-3) are
-// summed step by step (pair_short), and the zero-delay pairs stay on the near
-// list.
+// applied to K consecutive windows of the source's rho01 samples (K = 8 or
+// 16).  The pairs with 1 <= n < 8 (the short list, ~28 per dot at 512 um^-3)
+// are summed step by step, with K = 16 those with 8 <= n < 16 (the mid list,
+// ~207 per dot) in 8-step windows at every multiple of 8 (pair_list), and the
+// zero-delay pairs stay on the near list.
 //
 // Sample ring: rho01 of source b at finished step u in row b, entry
 // u & hist_mask, and again at hist_slots + (u & hist_mask) when that is below
@@ -893,43 +894,49 @@ __global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type 
   items.finish();
 }
 
-// The pairs of delay 1 <= n < K of every target (the short list, CSR by
-// target), at the current step: per pair the weights of block_weights and
-// the M + 1 samples it reads (all finished), summed in FP64.  kShortLanes
-// consecutive lanes share a target and take every kShortLanes-th pair of its
-// list (a chain of dependent loads per pair: list entry, source, samples),
-// then add their sums by a butterfly over the lanes.
-constexpr int kShortLanes = 8;
+// Listed pairs of every target (CSR by target) over KS steps from t0 (the
+// step counter): the short list (delays 1 .. 7) at every step (KS = 1), and
+// with 16-step passes the mid list (delays 8 .. 15) at every multiple of 8
+// (KS = 8).  Per pair the weights of block_weights once, then KS windows of
+// the source's samples (all finished: t0 + KS - 1 - n <= t0 - 1), summed in
+// FP64 into out[j n + a], j < KS.  kListLanes consecutive lanes share a
+// target and take every kListLanes-th pair of its list (a chain of dependent
+// loads per pair: list entry, source, samples), then add their sums by a
+// butterfly over the lanes.
+constexpr int kListLanes = 8;
 
-template <typename R, int M, bool LAB, bool GEN>
-__global__ __launch_bounds__(kBlock) void pair_short(const typename V4<R>::type *__restrict__ pd,
-                                                     const double4 *__restrict__ pos64,
-                                                     const typename V2<R>::type *__restrict__ hist,
-                                                     const typename V2<R>::type *__restrict__ rot,
-                                                     const int *__restrict__ short_start,
-                                                     const int *__restrict__ short_src, const int *step_ptr,
-                                                     double2 *__restrict__ out, Consts k)
+template <typename R, int M, int KS, bool LAB, bool GEN>
+__global__ __launch_bounds__(kBlock) void pair_list(const typename V4<R>::type *__restrict__ pd,
+                                                    const double4 *__restrict__ pos64,
+                                                    const typename V2<R>::type *__restrict__ hist,
+                                                    const typename V2<R>::type *__restrict__ rot,
+                                                    const int *__restrict__ list_start,
+                                                    const int *__restrict__ list_src, const int *step_ptr,
+                                                    double2 *__restrict__ out, Consts k)
 {
   using R2 = typename V2<R>::type;
   using R4 = typename V4<R>::type;
   constexpr bool F64 = sizeof(R) == sizeof(double);
-  static_assert(kWave % kShortLanes == 0, "a target's lanes lie in one wave");
+  static_assert(kWave % kListLanes == 0, "a target's lanes lie in one wave");
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int sub = gid % kShortLanes;
-  const bool valid = gid / kShortLanes < k.n;
-  const int a = valid ? gid / kShortLanes : k.n - 1;
+  const int sub = gid % kListLanes;
+  const bool valid = gid / kListLanes < k.n;
+  const int a = valid ? gid / kListLanes : k.n - 1;
   const PairK<R> Kc(k);
-  const int step = *step_ptr;
+  const int t0 = *step_ptr;
+  const int sm = (t0 - M) & k.hist_mask;  // window start of a delay-0 pair
   const R4 pa = pd[2 * a], da = pd[2 * a + 1];
   R4 dah = da;
   dah.x = R(0.5) * da.x;
   dah.y = R(0.5) * da.y;
   dah.z = R(0.5) * da.z;
   const double4 pa64 = pos64[a];
-  double accr = 0, acci = 0;
-  const int e1 = valid ? short_start[a + 1] : 0;
-  for(int e = short_start[a] + sub; e < e1; e += kShortLanes) {
-    const int b = short_src[e];
+  double accr[KS], acci[KS];
+#pragma unroll
+  for(int j = 0; j < KS; ++j) accr[j] = acci[j] = 0.0;
+  const int e1 = valid ? list_start[a + 1] : 0;
+  for(int e = list_start[a] + sub; e < e1; e += kListLanes) {
+    const int b = list_src[e];
     const R4 pb = pd[2 * b], db = pd[2 * b + 1];
     const R dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
     R xc, ir;
@@ -944,25 +951,42 @@ __global__ __launch_bounds__(kBlock) void pair_short(const typename V4<R>::type 
     const R2 rn = rot[n];
     R wr[M + 1], wim[M + 1];
     block_weights<R, M, LAB, GEN>(Kc, ir, dx, dy, dz, da, dah, db, xc, rn, true, wr, wim);
-    const R2 *row = hist + static_cast<size_t>(b) * k.hist_row;
-    R tr = R(0), ti = R(0);
+    // window of steps t0 - n - M .. t0 - n + KS - 1 (contiguous in the row)
+    const R2 *win = hist + static_cast<size_t>(b) * k.hist_row + ((sm - n) & k.hist_mask);
+    R tr[KS], ti[KS];
 #pragma unroll
-    for(int m = 0; m <= M; ++m) {
-      const R2 s = row[(step - n - m) & k.hist_mask];
-      tr = fmaT(wr[m], s.x, tr);
-      tr = fmaT(-wim[m], s.y, tr);
-      ti = fmaT(wr[m], s.y, ti);
-      ti = fmaT(wim[m], s.x, ti);
+    for(int j = 0; j < KS; ++j) tr[j] = ti[j] = R(0);
+#pragma unroll
+    for(int i = 0; i < KS + M; ++i) {
+      const R2 smp = win[i];
+#pragma unroll
+      for(int m = 0; m <= M; ++m) {
+        const int j = i + m - M;
+        if(j < 0 || j >= KS) continue;
+        tr[j] = fmaT(wr[m], smp.x, tr[j]);
+        tr[j] = fmaT(-wim[m], smp.y, tr[j]);
+        ti[j] = fmaT(wr[m], smp.y, ti[j]);
+        ti[j] = fmaT(wim[m], smp.x, ti[j]);
+      }
     }
-    accr += static_cast<double>(tr);
-    acci += static_cast<double>(ti);
+#pragma unroll
+    for(int j = 0; j < KS; ++j) {
+      accr[j] += static_cast<double>(tr[j]);
+      acci[j] += static_cast<double>(ti[j]);
+    }
   }
 #pragma unroll
-  for(int off = 1; off < kShortLanes; off <<= 1) {
-    accr += __shfl_xor(accr, off, kShortLanes);
-    acci += __shfl_xor(acci, off, kShortLanes);
-  }
-  if(valid && sub == 0) out[a] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
+  for(int j = 0; j < KS; ++j)
+#pragma unroll
+    for(int off = 1; off < kListLanes; off <<= 1) {
+      accr[j] += __shfl_xor(accr[j], off, kListLanes);
+      acci[j] += __shfl_xor(acci[j], off, kListLanes);
+    }
+  if(valid)
+#pragma unroll
+    for(int j = 0; j < KS; ++j)
+      if(j % kListLanes == sub)
+        out[static_cast<size_t>(j) * k.n + a] = make_double2(accr[j] * k.unit_scale, acci[j] * k.unit_scale);
 }
 
 // Past-only pair sum from the coefficient table (the reference's evaluate,
@@ -1047,13 +1071,15 @@ __device__ __forceinline__ double2 reduce_partials(const Consts &k, const Device
 
 // Field of the past pair sum at target a: the partials times -K e^{-i theta
 // step} (on-the-fly), as they are (the table holds complete coefficients), or
-// (time-blocked) the slice of this step plus the short list's sum, times -K
-// (rotating; the rotation is in the weights) or K (lab).
+// (time-blocked) the slice of this step plus the short list's sum (and, with
+// 16-step passes, the mid list's), times -K (rotating; the rotation is in the
+// weights) or K (lab).
 __device__ __forceinline__ double2 pair_field(const Consts &k, const DeviceState &d, int a, int step)
 {
   if(k.pair_blocked) {
     const double2 *slice = d.partial + static_cast<size_t>(step % k.block_steps) * (k.max_chunks + 1) * k.n;
-    const double2 sum = cadd(reduce_partials(k, d, slice, a), d.short_sum[a]);
+    double2 sum = cadd(reduce_partials(k, d, slice, a), d.short_sum[a]);
+    if(k.mid_steps) sum = cadd(sum, d.mid_sum[static_cast<size_t>(step % k.mid_steps) * k.n + a]);
     const double g = k.dyadic == QUEST_DYADIC_LAB ? k.k_green : -k.k_green;
     return make_double2(g * sum.x, g * sum.y);
   }
@@ -1756,8 +1782,9 @@ hipError_t launch_block_m(const Consts &k, DeviceState &d, hipStream_t s)
   return hipGetLastError();
 }
 
-template <typename R, int M>
-hipError_t launch_short_m(const Consts &k, DeviceState &d, hipStream_t s)
+template <typename R, int M, int KS>
+hipError_t launch_list_mk(const Consts &k, DeviceState &d, const int *start, const int *src, double2 *out,
+                          hipStream_t s)
 {
   using R2 = typename V2<R>::type;
   using R4 = typename V4<R>::type;
@@ -1767,17 +1794,24 @@ hipError_t launch_short_m(const Consts &k, DeviceState &d, hipStream_t s)
   else
     pd = d.pd32;
   const R2 *h = static_cast<const R2 *>(d.hist), *rt = static_cast<const R2 *>(d.rot);
-  const dim3 grid(static_cast<unsigned>((static_cast<size_t>(k.n) * kShortLanes + kBlock - 1) / kBlock));
+  const dim3 grid(static_cast<unsigned>((static_cast<size_t>(k.n) * kListLanes + kBlock - 1) / kBlock));
   if(k.dyadic == QUEST_DYADIC_LAB)
-    hipLaunchKernelGGL((pair_short<R, M, true, false>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt,
-                       d.short_start, d.short_src, d.step, d.short_sum, k);
+    hipLaunchKernelGGL((pair_list<R, M, KS, true, false>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt, start, src,
+                       d.step, out, k);
   else if(k.trig_poly)
-    hipLaunchKernelGGL((pair_short<R, M, false, false>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt,
-                       d.short_start, d.short_src, d.step, d.short_sum, k);
+    hipLaunchKernelGGL((pair_list<R, M, KS, false, false>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt, start, src,
+                       d.step, out, k);
   else
-    hipLaunchKernelGGL((pair_short<R, M, false, true>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt,
-                       d.short_start, d.short_src, d.step, d.short_sum, k);
+    hipLaunchKernelGGL((pair_list<R, M, KS, false, true>), grid, dim3(kBlock), 0, s, pd, d.pos64, h, rt, start, src,
+                       d.step, out, k);
   return hipGetLastError();
+}
+
+template <typename R, int M>
+hipError_t launch_list_m(const Consts &k, DeviceState &d, bool mid, hipStream_t s)
+{
+  return mid ? launch_list_mk<R, M, kBlockStepsShort>(k, d, d.mid_start, d.mid_src, d.mid_sum, s)
+             : launch_list_mk<R, M, 1>(k, d, d.short_start, d.short_src, d.short_sum, s);
 }
 
 template <int M>
@@ -1957,16 +1991,16 @@ int pair_blocks_per_cu(const Consts &k, int precision)
   }
 }
 
-hipError_t launch_pair_short(const Consts &k, DeviceState &d, int precision, hipStream_t s)
+hipError_t launch_pair_list(const Consts &k, DeviceState &d, int precision, bool mid, hipStream_t s)
 {
   const bool f64 = precision == QUEST_PREC_FP64;
   switch(k.order) {
-  case 1: return f64 ? launch_short_m<double, 1>(k, d, s) : launch_short_m<float, 1>(k, d, s);
-  case 2: return f64 ? launch_short_m<double, 2>(k, d, s) : launch_short_m<float, 2>(k, d, s);
-  case 3: return f64 ? launch_short_m<double, 3>(k, d, s) : launch_short_m<float, 3>(k, d, s);
-  case 4: return f64 ? launch_short_m<double, 4>(k, d, s) : launch_short_m<float, 4>(k, d, s);
-  case 5: return f64 ? launch_short_m<double, 5>(k, d, s) : launch_short_m<float, 5>(k, d, s);
-  case 6: return f64 ? launch_short_m<double, 6>(k, d, s) : launch_short_m<float, 6>(k, d, s);
+  case 1: return f64 ? launch_list_m<double, 1>(k, d, mid, s) : launch_list_m<float, 1>(k, d, mid, s);
+  case 2: return f64 ? launch_list_m<double, 2>(k, d, mid, s) : launch_list_m<float, 2>(k, d, mid, s);
+  case 3: return f64 ? launch_list_m<double, 3>(k, d, mid, s) : launch_list_m<float, 3>(k, d, mid, s);
+  case 4: return f64 ? launch_list_m<double, 4>(k, d, mid, s) : launch_list_m<float, 4>(k, d, mid, s);
+  case 5: return f64 ? launch_list_m<double, 5>(k, d, mid, s) : launch_list_m<float, 5>(k, d, mid, s);
+  case 6: return f64 ? launch_list_m<double, 6>(k, d, mid, s) : launch_list_m<float, 6>(k, d, mid, s);
   default: return hipErrorInvalidValue;
   }
 }

@@ -14,7 +14,7 @@ def test_profiled_traffic_matches_bench_labels():
     import bench
     # the N = 65536 line's kernel (the time-blocked method's pass), and the
     # per-step method's (the default pipelined source loop)
-    tb = bench.profiled_traffic(bench.kernel_name({"dynamic_items": 0, "pipelined": 1, "block_steps": 8}, 1, 2))
+    tb = bench.profiled_traffic(bench.kernel_name({"dynamic_items": 0, "pipelined": 1, "block_steps": 16}, 1, 2))
     assert tb is not None and 5e8 < tb < 1e10
     t32 = bench.profiled_traffic(bench.kernel_name({"dynamic_items": 0, "pipelined": 1}, 1, 0))
     assert t32 is not None and 5e8 < t32 < 1e10

@@ -242,7 +242,8 @@ class Context:
     def set_pair_method(self, method: int):
         """PAIR_ON_THE_FLY (default), PAIR_TABLE (the reference's per-pair
         coefficient table, FP64 only) or PAIR_TIME_BLOCKED (one pass over the
-        pairs of delay >= 8 per 8 steps); before the first step."""
+        pairs of delay >= K per K = 8 or 16 steps, `block_steps`); before the
+        first step."""
         self._check(self._L.quest_set_pair_method(self.ptr, int(method)))
 
     def autotune(self, launches: int = 3) -> int:
@@ -318,7 +319,9 @@ class Context:
         return ms.value, cnt.value
 
     def short_kernel_time(self, reset: bool = False):
-        """Time-blocked method: accumulated short-list kernel time (ms) and launches."""
+        """Time-blocked method: accumulated list-kernel time (ms: the short
+        list every step, the mid list every 8 steps with 16-step passes) and
+        the number of steps it served."""
         ms = C.c_double()
         cnt = C.c_int64()
         self._check(self._L.quest_short_kernel_time(self.ptr, C.byref(ms), C.byref(cnt), int(reset)))

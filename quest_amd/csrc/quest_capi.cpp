@@ -1057,8 +1057,9 @@ int build_list(quest_ctx *c, int lo, int hi, int **start_dev, int **src_dev, int
 }
 
 // Time-blocked mode: the steps per pass K, the rho01 sample ring, the
-// rotation table e^{-i theta (n + 1/2)}, and the short list (pairs of delay
-// 1 .. K - 1, the precision's own split, as the pair kernels classify them).  The ring holds
+// rotation table e^{-i theta (n + 1/2)}, the short list (pairs of delay
+// 1 .. 7) and with K = 16 the mid list (8 .. 15), both by the precision's own
+// split, as the pair kernels classify them.  The ring holds
 // more than the largest delay + order + K steps, so that the pass of the
 // current block can be formed again from it at any step of the block
 // (refill_block).

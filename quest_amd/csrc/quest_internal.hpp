@@ -154,7 +154,7 @@ struct DeviceState {
   // time-blocked mode only
   void *hist = nullptr;                // rho01 sample ring [n][hist_row] (double2 FP64, float2 mixed)
   void *rot = nullptr;                 // [hist_slots] e^{-i theta (n + 1/2)} (double2 FP64, float2 mixed)
-  int *short_start = nullptr;          // [n + 1] short list (delays 1 .. K - 1), CSR by target
+  int *short_start = nullptr;          // [n + 1] short list (delays 1 .. 7), CSR by target
   int *short_src = nullptr;            // [short pairs]
   double2 *short_sum = nullptr;        // [n] the short list's sum at the current step
   int *mid_start = nullptr;            // [n + 1] mid list (delays 8 .. 15, 16-step passes only), CSR by target

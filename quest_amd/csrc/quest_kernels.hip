@@ -901,8 +901,8 @@ __global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type 
 // the source's samples (all finished: t0 + KS - 1 - n <= t0 - 1), summed in
 // FP64 into out[j n + a], j < KS.  kListLanes consecutive lanes share a
 // target and take every kListLanes-th pair of its list (a chain of dependent
-// loads per pair: list entry, source, samples), then add their sums by a
-// butterfly over the lanes.
+// loads per pair: list entry, source, samples; U of them in flight per lane),
+// then add their sums by a butterfly over the lanes.
 constexpr int kListLanes = 8;
 
 template <typename R, int M, int KS, bool LAB, bool GEN>
@@ -934,45 +934,76 @@ __global__ __launch_bounds__(kBlock) void pair_list(const typename V4<R>::type *
   double accr[KS], acci[KS];
 #pragma unroll
   for(int j = 0; j < KS; ++j) accr[j] = acci[j] = 0.0;
+  // U pairs of the lane's share in flight at once: their list entries, then
+  // their sources, then their sample windows are loaded together, so a lane
+  // waits for U chains of dependent loads at a time instead of one.  The
+  // mid list (~23 pairs per lane at 512 um^-3) takes 2; the short list (~3
+  // per lane) 1: with 4 its ensemble kernel ran 58 -> 68 us per step (half
+  // the occupancy, a quarter of the slots empty)
+  constexpr int U = KS == 1 ? 1 : 2;
   const int e1 = valid ? list_start[a + 1] : 0;
-  for(int e = list_start[a] + sub; e < e1; e += kListLanes) {
-    const int b = list_src[e];
-    const R4 pb = pd[2 * b], db = pd[2 * b + 1];
-    const R dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-    R xc, ir;
-    int n;
-    if constexpr(F64) {
-      n = split_l64(dx, dy, dz, pa64, pos64, b, Kc.cdt, xc, ir);
-    } else {
-      const int code = split_mixed(dx, dy, dz, Kc.eps_rel, Kc.lim0, kRoundMagic, pa64, pos64, b, Kc.cdt,
-                                   Kc.icdt64, xc, ir);
-      n = kRoundMagicBits - code;
+  for(int e = list_start[a] + sub; e < e1; e += kListLanes * U) {
+    int b[U];
+    bool ok[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      ok[u] = e + u * kListLanes < e1;
+      b[u] = list_src[ok[u] ? e + u * kListLanes : e];  // a repeated entry where the share ends, not summed
     }
-    const R2 rn = rot[n];
-    R wr[M + 1], wim[M + 1];
-    block_weights<R, M, LAB, GEN>(Kc, ir, dx, dy, dz, da, dah, db, xc, rn, true, wr, wim);
-    // window of steps t0 - n - M .. t0 - n + KS - 1 (contiguous in the row)
-    const R2 *win = hist + static_cast<size_t>(b) * k.hist_row + ((sm - n) & k.hist_mask);
-    R tr[KS], ti[KS];
+    R4 pb[U], db[U];
 #pragma unroll
-    for(int j = 0; j < KS; ++j) tr[j] = ti[j] = R(0);
+    for(int u = 0; u < U; ++u) {
+      pb[u] = pd[2 * b[u]];
+      db[u] = pd[2 * b[u] + 1];
+    }
+    R dx[U], dy[U], dz[U], xc[U], ir[U];
+    int n[U];
 #pragma unroll
-    for(int i = 0; i < KS + M; ++i) {
-      const R2 smp = win[i];
-#pragma unroll
-      for(int m = 0; m <= M; ++m) {
-        const int j = i + m - M;
-        if(j < 0 || j >= KS) continue;
-        tr[j] = fmaT(wr[m], smp.x, tr[j]);
-        tr[j] = fmaT(-wim[m], smp.y, tr[j]);
-        ti[j] = fmaT(wr[m], smp.y, ti[j]);
-        ti[j] = fmaT(wim[m], smp.x, ti[j]);
+    for(int u = 0; u < U; ++u) {
+      dx[u] = pa.x - pb[u].x;
+      dy[u] = pa.y - pb[u].y;
+      dz[u] = pa.z - pb[u].z;
+      if constexpr(F64) {
+        n[u] = split_l64(dx[u], dy[u], dz[u], pa64, pos64, b[u], Kc.cdt, xc[u], ir[u]);
+      } else {
+        const int code = split_mixed(dx[u], dy[u], dz[u], Kc.eps_rel, Kc.lim0, kRoundMagic, pa64, pos64, b[u],
+                                     Kc.cdt, Kc.icdt64, xc[u], ir[u]);
+        n[u] = kRoundMagicBits - code;
       }
     }
+    // windows of steps t0 - n - M .. t0 - n + KS - 1 (contiguous in the row)
+    R2 rn[U], smp[U][KS + M];
 #pragma unroll
-    for(int j = 0; j < KS; ++j) {
-      accr[j] += static_cast<double>(tr[j]);
-      acci[j] += static_cast<double>(ti[j]);
+    for(int u = 0; u < U; ++u) {
+      rn[u] = rot[n[u]];
+      const R2 *win = hist + static_cast<size_t>(b[u]) * k.hist_row + ((sm - n[u]) & k.hist_mask);
+#pragma unroll
+      for(int i = 0; i < KS + M; ++i) smp[u][i] = win[i];
+    }
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      R wr[M + 1], wim[M + 1];
+      block_weights<R, M, LAB, GEN>(Kc, ir[u], dx[u], dy[u], dz[u], da, dah, db[u], xc[u], rn[u], true, wr, wim);
+      R tr[KS], ti[KS];
+#pragma unroll
+      for(int j = 0; j < KS; ++j) tr[j] = ti[j] = R(0);
+#pragma unroll
+      for(int i = 0; i < KS + M; ++i)
+#pragma unroll
+        for(int m = 0; m <= M; ++m) {
+          const int j = i + m - M;
+          if(j < 0 || j >= KS) continue;
+          tr[j] = fmaT(wr[m], smp[u][i].x, tr[j]);
+          tr[j] = fmaT(-wim[m], smp[u][i].y, tr[j]);
+          ti[j] = fmaT(wr[m], smp[u][i].y, ti[j]);
+          ti[j] = fmaT(wim[m], smp[u][i].x, ti[j]);
+        }
+      if(ok[u])
+#pragma unroll
+        for(int j = 0; j < KS; ++j) {
+          accr[j] += static_cast<double>(tr[j]);
+          acci[j] += static_cast<double>(ti[j]);
+        }
     }
   }
 #pragma unroll

@@ -53,7 +53,9 @@ struct Configuration {
   int interpolation_order = 0;
   std::string qd_path = "dots.cfg", pulse_path = "pulse.cfg";
   int precision = QUEST_PREC_FP64;  // arithmetic of the pair sum on the device
-  int pair_method = QUEST_PAIR_ON_THE_FLY;  // or QUEST_PAIR_TABLE (the reference's coefficient table)
+  // QUEST_PAIR_TIME_BLOCKED (sums of 8 or 16 steps per pass; the fastest), QUEST_PAIR_ON_THE_FLY
+  // (every pair every step) or QUEST_PAIR_TABLE (the reference's coefficient table, FP64)
+  int pair_method = QUEST_PAIR_TIME_BLOCKED;
   int accumulation = QUEST_ACC_BLOCK;       // mixed precision's 64-source block sums (or compensated)
   bool autotune = false;                    // DeviceIntegrator: quest_autotune before the first step
   int device = 0;

@@ -110,7 +110,6 @@ struct Options {
 Options parse_configs(int argc, char *argv[])
 {
   Options o;
-  config.pair_method = QUEST_PAIR_TIME_BLOCKED;  // the driver's default (--pair-method)
   for(int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto next = [&]() -> std::string {

@@ -189,7 +189,8 @@ def test_blocked_autotune_keeps_the_run(ref_weights, block_steps):
     assert rel_err(rb, ra) < 1e-13
 
 
-def test_blocked_batch_matches_single_systems(ref_weights):
+@PASS_STEPS
+def test_blocked_batch_matches_single_systems(ref_weights, block_steps):
     """Five systems of uneven size in one batch context against each run
     alone, both time-blocked (FP64)."""
     from quest_amd import Context, systems
@@ -198,14 +199,14 @@ def test_blocked_batch_matches_single_systems(ref_weights):
     syss = [dense_system(n, 0.12 * n ** (1 / 3), seed=100 + i) for i, n in enumerate(sizes)]
     pls = [short_pulse(area=a) for a in (0.5, 1.0, 1.5, 2.0, 2.5)]
     ps, cs, fc = ref_weights
-    with Context(syss, consts, pulse=pls) as ctx:
+    with Context(syss, consts, pulse=pls, block_steps=block_steps) as ctx:
         ctx.set_pc_weights(ps, cs, fc)
         ctx.set_pair_method(2)
         ctx.run(100)
         batch = ctx.state()[0]
     off = 0
     for s, pl, n in zip(syss, pls, sizes):
-        with _ctx(s, consts, ref_weights, 0, pulse=pl) as one:
+        with _ctx(s, consts, ref_weights, 0, pulse=pl, block_steps=block_steps) as one:
             one.run(100)
             r = one.state()[0]
         assert rel_err(batch[off:off + n], r) < 1e-13

@@ -178,11 +178,12 @@ def test_pulse_list_run_matches_oracle(oracle_lib, ref_weights, precision, tol):
     assert err < tol
 
 
-def _run_gpu(sysd, consts, pl, weights, steps, precision=0, chunks=1, accumulation=0):
+def _run_gpu(sysd, consts, pl, weights, steps, precision=0, chunks=1, accumulation=0, method=0):
     from quest_amd import Context
     ps, cs, fc = weights
     with Context(sysd, consts, precision=precision, pulse=pl, accumulation=accumulation) as ctx:
         ctx.set_pc_weights(ps, cs, fc)
+        ctx.set_pair_method(method)
         per = steps // chunks
         for _ in range(chunks):
             ctx.run(per)
@@ -402,8 +403,9 @@ def test_full_size_mixed_matches_fp64(ref_weights, name, method):
     assert err01 < 1e-6 and err00 < 1e-6
 
 
+@pytest.mark.parametrize("method", [0, 2], ids=["per_step", "time_blocked"])
 @pytest.mark.parametrize("dt", [1.3e-4, 1e-3])
-def test_large_laser_phase_per_step_matches_oracle(oracle_lib, ref_weights, dt):
+def test_large_laser_phase_per_step_matches_oracle(oracle_lib, ref_weights, dt, method):
     """theta = omega dt beyond the short rotation series (FP32 series up to
     0.28, FP64 up to 0.33): dt = 1.3e-4 ps (theta = 0.296: FP64 series, FP32
     library sincos) and 1e-3 ps (theta = 2.28: library sincos in both).  The
@@ -420,11 +422,12 @@ def test_large_laser_phase_per_step_matches_oracle(oracle_lib, ref_weights, dt):
     ps, cs, fc = ref_weights
     ref = oracle.solve(oracle_params(oracle, consts), sysd, oracle_pulse(oracle, pl), ps, cs, fc, steps, mode=1)
     assert ref["status"] == 0
-    r64, d64 = _run_gpu(sysd, consts, pl, ref_weights, steps, precision=0)
-    r32, _ = _run_gpu(sysd, consts, pl, ref_weights, steps, precision=1)
+    r64, d64 = _run_gpu(sysd, consts, pl, ref_weights, steps, precision=0, method=method)
+    r32, _ = _run_gpu(sysd, consts, pl, ref_weights, steps, precision=1, method=method)
     e64 = rel_err(r64, ref["y"])
     e32 = float(np.abs(r32 - r64).max())
-    print(f"dt={dt}: theta={consts['laser_omega'] * dt:.3f} fp64 rel err {e64:.2e}, mixed |d rho| {e32:.2e}, "
+    print(f"dt={dt} method {method}: theta={consts['laser_omega'] * dt:.3f} fp64 rel err {e64:.2e}, "
+          f"mixed |d rho| {e32:.2e}, "
           f"max|rho01| {np.abs(ref['y'][:, 1]).max():.3f}")
     assert np.abs(ref["y"][:, 1]).max() > 0.1, "the pulse did not drive the system"
     assert e64 < 1e-10 and e32 < 1e-6

@@ -45,10 +45,12 @@ def test_step_block_follows_largest_group(n_chain, block):
     assert info["step_block"] == block
 
 
+@pytest.mark.parametrize("method,block_steps", [(0, 0), (2, 8), (2, 16)], ids=["per_step", "K8", "K16"])
 @pytest.mark.parametrize("n_chain", [100, 200, 300])
-def test_grouped_run_matches_oracle(oracle_lib, ref_weights, n_chain):
+def test_grouped_run_matches_oracle(oracle_lib, ref_weights, n_chain, method, block_steps):
     """n_chain 100 / 200: one-launch step with 128- / 256-thread blocks and
-    padding; n_chain 300: per-iteration kernels."""
+    padding; n_chain 300: per-iteration kernels; each with the per-step and
+    the time-blocked pair sums (8- and 16-step passes)."""
     from quest_amd import Context, systems
     oracle = oracle_lib
     consts = systems.consts()
@@ -59,15 +61,17 @@ def test_grouped_run_matches_oracle(oracle_lib, ref_weights, n_chain):
     ref = oracle.solve(oracle_params(oracle, consts), sysd, oracle_pulse(oracle, pl), ps, cs, fc,
                        steps, mode=0)
     assert ref["status"] == 0
-    with Context(sysd, consts, precision=0, pulse=pl) as ctx:
+    with Context(sysd, consts, precision=0, pulse=pl, block_steps=block_steps) as ctx:
         ctx.set_pc_weights(ps, cs, fc)
-        ctx.run(steps // 2)
-        ctx.run(steps - steps // 2)
+        ctx.set_pair_method(method)
+        ctx.run(steps // 2 + 3)
+        ctx.run(steps - steps // 2 - 3)
         info = ctx.info()
         rho, drho = ctx.state()
     assert info["step_block"] == {100: 128, 200: 256, 300: 0}[n_chain]
     e_rho, e_d = rel_err(rho, ref["y"]), rel_err(drho, ref["yp"])
-    print(f"chain {n_chain}: block {info['step_block']}, rho rel err {e_rho:.3e}, drho {e_d:.3e}")
+    print(f"chain {n_chain} method {method} K {block_steps}: block {info['step_block']}, rho rel err "
+          f"{e_rho:.3e}, drho {e_d:.3e}")
     assert np.abs(ref["y"][:, 1]).max() > 0.1, "the pulse did not drive the system"
     assert e_rho < 1e-10 and e_d < 1e-9
 

@@ -798,7 +798,10 @@ __device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R d
 // counter) and every ordered pair of delay n >= K, the weighted windows of
 // the source's samples, into partial slice j = 0..K-1 (slice j, row r, target
 // a at (j (max_chunks + 1) + r) n + a).  Same work list, items and claiming
-// as pair_past_*; FP32 (mixed) sums are folded into FP64 every 64 sources.
+// as pair_past_*.  Each step keeps three sums of real products (the complex
+// product with three multiplications), combined into re and im at the end
+// (FP64) or when the FP32 (mixed) sums are folded into FP64, every 64
+// sources.
 template <typename R, int M, int K, bool LAB, bool DYN, bool GEN>
 __global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type *__restrict__ pd,
                                                      const double4 *__restrict__ pos64,
@@ -829,8 +832,8 @@ __global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type 
     dah.y = R(0.5) * da.y;
     dah.z = R(0.5) * da.z;
     const double4 pa64 = pos64[a];
-    // one source into the step sums (sr, si)
-    auto source = [&](int b, R(&sr)[K], R(&si)[K]) {
+    // one source into the three step sums p1..p3 (see below)
+    auto source = [&](int b, R(&p1)[K], R(&p2)[K], R(&p3)[K]) {
       const R4 pb = pd[2 * b], db = pd[2 * b + 1];
       const R dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
       R xc, ir;
@@ -852,35 +855,71 @@ __global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type 
       R wr[M + 1], wim[M + 1];
       block_weights<R, M, LAB, GEN>(Kc, ir, dx, dy, dz, da, dah, db, xc, rn, n >= K, wr, wim);
       // step t0 + j reads sample t0 + j - n - m = window entry j - m + M
+      if constexpr(LAB) {
+        // real weights on real samples (2 Re rho01; the imaginary parts are 0)
 #pragma unroll
-      for(int i = 0; i < K + M; ++i)
+        for(int i = 0; i < K + M; ++i)
 #pragma unroll
-        for(int m = 0; m <= M; ++m) {
-          const int j = i + m - M;
-          if(j < 0 || j >= K) continue;
-          sr[j] = fmaT(wr[m], smp[i].x, sr[j]);
-          sr[j] = fmaT(-wim[m], smp[i].y, sr[j]);
-          si[j] = fmaT(wr[m], smp[i].y, si[j]);
-          si[j] = fmaT(wim[m], smp[i].x, si[j]);
+          for(int m = 0; m <= M; ++m) {
+            const int j = i + m - M;
+            if(j < 0 || j >= K) continue;
+            p1[j] = fmaT(wr[m], smp[i].x, p1[j]);
+          }
+      } else {
+        // (wr + i wi)(x + i y) with three products (Gauss): the sums of wr x,
+        // wi y and (wr + wi)(x + y) give re = p1 - p2, im = p3 - p1 - p2,
+        // formed when the sums are folded: 3 FMAs per tap instead of 4
+        R ws[M + 1];
+#pragma unroll
+        for(int m = 0; m <= M; ++m) ws[m] = wr[m] + wim[m];
+#pragma unroll
+        for(int i = 0; i < K + M; ++i) {
+          const R ss = smp[i].x + smp[i].y;
+#pragma unroll
+          for(int m = 0; m <= M; ++m) {
+            const int j = i + m - M;
+            if(j < 0 || j >= K) continue;
+            p1[j] = fmaT(wr[m], smp[i].x, p1[j]);
+            p2[j] = fmaT(wim[m], smp[i].y, p2[j]);
+            p3[j] = fmaT(ws[m], ss, p3[j]);
+          }
         }
+      }
     };
     double accr[K], acci[K];
 #pragma unroll
     for(int j = 0; j < K; ++j) accr[j] = acci[j] = 0.0;
     if constexpr(F64) {
-      for(int b = wi.z; b < wi.w; ++b) source(b, accr, acci);
+      double acc3[K];
+#pragma unroll
+      for(int j = 0; j < K; ++j) acc3[j] = 0.0;
+      for(int b = wi.z; b < wi.w; ++b) source(b, accr, acci, acc3);
+      if constexpr(!LAB) {
+#pragma unroll
+        for(int j = 0; j < K; ++j) {
+          const double q1 = accr[j], q2 = acci[j];
+          accr[j] = q1 - q2;
+          acci[j] = acc3[j] - q1 - q2;
+        }
+      }
     } else {
       // FP32 sums of up to 64 sources, folded into FP64
       for(int bt = wi.z; bt < wi.w; bt += kChunk) {
         const int be = min(wi.w, bt + kChunk);
-        R sr[K], si[K];
+        R q1[K], q2[K], q3[K];
 #pragma unroll
-        for(int j = 0; j < K; ++j) sr[j] = si[j] = R(0);
-        for(int b = bt; b < be; ++b) source(b, sr, si);
+        for(int j = 0; j < K; ++j) q1[j] = q2[j] = q3[j] = R(0);
+        for(int b = bt; b < be; ++b) source(b, q1, q2, q3);
 #pragma unroll
         for(int j = 0; j < K; ++j) {
-          accr[j] += static_cast<double>(sr[j]);
-          acci[j] += static_cast<double>(si[j]);
+          const double d1 = q1[j];
+          if constexpr(LAB) {
+            accr[j] += d1;
+          } else {
+            const double d2 = q2[j];
+            accr[j] += d1 - d2;
+            acci[j] += static_cast<double>(q3[j]) - d1 - d2;
+          }
         }
       }
     }

@@ -48,19 +48,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 F_ALG = 105            # flop per ordered pair per step (algorithmic count of the per-step formulation)
-# Time-blocked formulation, per ordered pair: 190 flop once per pass (geometry,
-# dipole products, radial factors, the rotation e^{-i theta (n + x)}, the
-# complex q0/q1/q2 coefficients, and the weights on the M + 1 = 5 samples
-# through the monomial Lagrange basis), then 40 flop per step (5 complex
-# multiply-adds): 40 + 190 / K per pair-step.
-F_BLK_PAIR = 190
-F_BLK_STEP = 40
+# Time-blocked formulation with K steps per pass, per ordered pair: once per
+# pass 190 flop (geometry, dipole products, radial factors, the rotation
+# e^{-i theta (n + x)}, the complex q0/q1/q2 coefficients, the weights on the
+# M + 1 = 5 samples through the monomial Lagrange basis) plus the sums
+# wr + wi of the 5 weights and x + y of the K + 4 window samples; then 30 flop
+# per step (5 complex multiply-adds of three real products each):
+# 30 + (199 + K) / K per pair-step, 43.4 at K = 16 (with four real products
+# per complex multiply-add it would be 40 + 190 / K = 51.9).
+F_BLK_PAIR = 199
+F_BLK_STEP = 30
 
 
 def flop_per_pair_step(steps_per_launch: int) -> float:
     """Algorithmic flop per ordered pair and step of the pair kernel that
     forms `steps_per_launch` steps per launch (1: the per-step formulation)."""
-    return float(F_ALG) if steps_per_launch <= 1 else F_BLK_STEP + F_BLK_PAIR / steps_per_launch
+    k = steps_per_launch
+    return float(F_ALG) if k <= 1 else F_BLK_STEP + (F_BLK_PAIR + k) / k
 FP32_PEAK = 157.3      # TFLOP/s, MI355X vector FP32 (datasheet)
 FP64_PEAK = 78.6       # TFLOP/s, MI355X vector FP64
 # TFLOP/s: non-packed FP32 FMAs issue at one 64-lane instruction per cycle per

@@ -672,12 +672,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
 // pair_term is  A p + B p' + C p''/2  with complex A, B, C fixed by the
 // geometry (rotation e^{-i theta (n + x)} included).  So its weight on
 // sample s - m is W_m = A L_m(x) + B L_m'(x) + C L_m''(x)/2, formed from the
-// monomial coefficients of the L_m as W_m = sum_j l[j][m] Gamma_j with
-// Gamma_j = A x^j + j B x^(j-1) + j(j-1)/2 C x^(j-2).
+// monomial coefficients of the L_m in z = x - M/2 (nodes centred on 0) as
+// W_m = sum_j l[j][m] Gamma_j with
+// Gamma_j = A z^j + j B z^(j-1) + j(j-1)/2 C z^(j-2).
 
-// Monomial coefficients of the Lagrange basis on nodes 0..M:
-// L_m(x) = sum_j c[j][m] x^j, exact rationals (integer products over one
-// integer denominator) rounded once.
+// Monomial coefficients of the Lagrange basis on the centred nodes
+// m - M/2, m = 0..M: L_m(z) = sum_j c[j][m] z^j with z = x - M/2, exact
+// dyadic products over one integer denominator, rounded once.  The nodes are
+// symmetric about 0, so c[j][M - m] = (-1)^j c[j][m]: a weight and its mirror
+// share their even and odd parts (block_weights).
 template <int M> struct LagrangeMonomials {
   double c[M + 1][M + 1];
   constexpr LagrangeMonomials() : c{}
@@ -689,8 +692,9 @@ template <int M> struct LagrangeMonomials {
       int deg = 0;
       for(int j = 0; j <= M; ++j) {
         if(j == m) continue;
-        for(int e = deg + 1; e >= 1; --e) p[e] = p[e - 1] - j * p[e];  // times (x - j)
-        p[0] = -j * p[0];
+        const double node = j - 0.5 * M;
+        for(int e = deg + 1; e >= 1; --e) p[e] = p[e - 1] - node * p[e];  // times (z - node)
+        p[0] = -node * p[0];
         ++deg;
         den *= m - j;
       }
@@ -702,7 +706,7 @@ template <int M> struct LagrangeMonomials {
 // Weights W_m = wr[m] + i wi[m] of one pair on samples s - m, m = 0..M,
 // before the global factor -K (rotating) or K (lab) and the unit scale.
 // live false: all weights 0 (the pair belongs to another list).  rot_n =
-// e^{-i theta (n + 1/2)} (rotating frame), x = xc + 1/2.
+// e^{-i theta (n + 1/2)} (rotating frame), x = xc + 1/2 on nodes 0..M.
 template <typename R, int M, bool LAB, bool GEN>
 __device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R dy, R dz,
                                               const typename V4<R>::type &da, const typename V4<R>::type &dah,
@@ -752,11 +756,12 @@ __device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R d
     cr = rr * g1;
     ci = ri * g1;
   }
-  const R x = xc + R(0.5);
+  // z = x - M/2 with x = xc + 1/2 the fraction on nodes 0..M
+  const R z = xc + R(0.5 - 0.5 * M);
   R pw[M + 1];
   pw[0] = R(1);
 #pragma unroll
-  for(int j = 1; j <= M; ++j) pw[j] = j == 1 ? x : pw[j - 1] * x;
+  for(int j = 1; j <= M; ++j) pw[j] = j == 1 ? z : pw[j - 1] * z;
   R gr[M + 1], gi[M + 1];
   gr[0] = ar;
   gi[0] = ai;
@@ -771,13 +776,14 @@ __device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R d
       gi[j] = fmaT(ci, s, gi[j]);
     }
   }
+  // W_m = E_m + O_m and W_{M-m} = E_m - O_m, E and O the sums over even and
+  // odd j of c[j][m] Gamma_j: 11 terms at M = 4 instead of the 21 nonzero
+  // c[j][m] Gamma_j of all five weights
   constexpr LagrangeMonomials<M> L{};
-#pragma unroll
-  for(int m = 0; m <= M; ++m) {
-    R sr = R(0), si = R(0);
+  auto part = [&](int m, int parity, R &sr, R &si) {
     bool first = true;
 #pragma unroll
-    for(int j = 0; j <= M; ++j) {
+    for(int j = parity; j <= M; j += 2) {
       const double c = L.c[j][m];
       if(c == 0.0) continue;
       if(first) {
@@ -789,9 +795,19 @@ __device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R d
         si = fmaT(R(c), gi[j], si);
       }
     }
-    wr[m] = sr;
-    wi[m] = si;
+    if(first) sr = si = R(0);
+  };
+#pragma unroll
+  for(int m = 0; 2 * m < M; ++m) {
+    R er, ei, orr, oi;
+    part(m, 0, er, ei);
+    part(m, 1, orr, oi);
+    wr[m] = er + orr;
+    wi[m] = ei + oi;
+    wr[M - m] = er - orr;
+    wi[M - m] = ei - oi;
   }
+  if constexpr(M % 2 == 0) part(M / 2, 0, wr[M / 2], wi[M / 2]);  // the odd c[j][M/2] vanish
 }
 
 // The time-blocked pair sum: for the K steps t0 .. t0 + K - 1 (t0 = the step
@@ -801,9 +817,11 @@ __device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R d
 // as pair_past_*.  Each step keeps three sums of real products (the complex
 // product with three multiplications), combined into re and im at the end
 // (FP64) or when the FP32 (mixed) sums are folded into FP64, every 64
-// sources.
+// sources.  The FP64 16-step pass is held to three waves per SIMD (168
+// VGPRs, a few of them spilled): 2 % faster than two waves at 193 VGPRs; the
+// FP32 one loses at three (45 spilled) and keeps two.
 template <typename R, int M, int K, bool LAB, bool DYN, bool GEN>
-__global__ __launch_bounds__(kBlock) void pair_block(const typename V4<R>::type *__restrict__ pd,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(double) && K == 16 ? 3 : 1, 8))) void pair_block(const typename V4<R>::type *__restrict__ pd,
                                                      const double4 *__restrict__ pos64,
                                                      const typename V2<R>::type *__restrict__ hist,
                                                      const typename V2<R>::type *__restrict__ rot,

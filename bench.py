@@ -49,14 +49,16 @@ sys.path.insert(0, ROOT)
 
 F_ALG = 105            # flop per ordered pair per step (algorithmic count of the per-step formulation)
 # Time-blocked formulation with K steps per pass, per ordered pair: once per
-# pass 190 flop (geometry, dipole products, radial factors, the rotation
+# pass 158 flop (geometry, dipole products, radial factors, the rotation
 # e^{-i theta (n + x)}, the complex q0/q1/q2 coefficients, the weights on the
-# M + 1 = 5 samples through the monomial Lagrange basis) plus the sums
-# wr + wi of the 5 weights and x + y of the K + 4 window samples; then 30 flop
-# per step (5 complex multiply-adds of three real products each):
-# 30 + (199 + K) / K per pair-step, 43.4 at K = 16 (with four real products
-# per complex multiply-add it would be 40 + 190 / K = 51.9).
-F_BLK_PAIR = 199
+# M + 1 = 5 samples through the monomial Lagrange basis on centred nodes: 11
+# real-by-complex terms and 4 complex mirror sums) plus the sums wr + wi of
+# the 5 weights and x + y of the K + 4 window samples; then 30 flop per step
+# (5 complex multiply-adds of three real products each): 30 + (167 + K) / K
+# per pair-step, 41.4 at K = 16 (with four real products per complex
+# multiply-add and the 21 terms of the one-sided basis it would be
+# 40 + 190 / K = 51.9).
+F_BLK_PAIR = 167
 F_BLK_STEP = 30
 
 

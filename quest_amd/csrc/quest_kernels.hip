@@ -1041,25 +1041,39 @@ __global__ __launch_bounds__(kBlock) void pair_list(const typename V4<R>::type *
     for(int u = 0; u < U; ++u) {
       R wr[M + 1], wim[M + 1];
       block_weights<R, M, LAB, GEN>(Kc, ir[u], dx[u], dy[u], dz[u], da, dah, db[u], xc[u], rn[u], true, wr, wim);
-      R tr[KS], ti[KS];
+      // the three real products of pair_block's complex multiply-adds (one
+      // in the lab frame, where weights and samples are real)
+      R q1[KS], q2[KS], q3[KS];
 #pragma unroll
-      for(int j = 0; j < KS; ++j) tr[j] = ti[j] = R(0);
+      for(int j = 0; j < KS; ++j) q1[j] = q2[j] = q3[j] = R(0);
+      R ws[M + 1];
 #pragma unroll
-      for(int i = 0; i < KS + M; ++i)
+      for(int m = 0; m <= M; ++m) ws[m] = wr[m] + wim[m];
+#pragma unroll
+      for(int i = 0; i < KS + M; ++i) {
+        const R ss = smp[u][i].x + smp[u][i].y;
 #pragma unroll
         for(int m = 0; m <= M; ++m) {
           const int j = i + m - M;
           if(j < 0 || j >= KS) continue;
-          tr[j] = fmaT(wr[m], smp[u][i].x, tr[j]);
-          tr[j] = fmaT(-wim[m], smp[u][i].y, tr[j]);
-          ti[j] = fmaT(wr[m], smp[u][i].y, ti[j]);
-          ti[j] = fmaT(wim[m], smp[u][i].x, ti[j]);
+          q1[j] = fmaT(wr[m], smp[u][i].x, q1[j]);
+          if constexpr(!LAB) {
+            q2[j] = fmaT(wim[m], smp[u][i].y, q2[j]);
+            q3[j] = fmaT(ws[m], ss, q3[j]);
+          }
         }
+      }
       if(ok[u])
 #pragma unroll
         for(int j = 0; j < KS; ++j) {
-          accr[j] += static_cast<double>(tr[j]);
-          acci[j] += static_cast<double>(ti[j]);
+          const double d1 = q1[j];
+          if constexpr(LAB) {
+            accr[j] += d1;
+          } else {
+            const double d2 = q2[j];
+            accr[j] += d1 - d2;
+            acci[j] += static_cast<double>(q3[j]) - d1 - d2;
+          }
         }
     }
   }

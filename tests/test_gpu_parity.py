@@ -254,7 +254,8 @@ def test_mixed_delays_equal_fp64_full_size(workload):
     like the FP64 split (history_interaction.cpp:31-32): the integer delays of
     the mixed pair kernel's split equal the FP64 ones bit for bit -- all
     2.1e9 pairs at N = 65536, 128 seeded ranges of 2^19 pairs (67M) at
-    N = 131072 -- and so do the zero-delay lists (same number of pairs)."""
+    N = 131072 -- and so do the zero-delay lists (same number of pairs) and,
+    with the time-blocked method, the short lists and the pass length."""
     from quest_amd import Context, systems
     sysd, _ = systems.workload(workload)
     n = sysd["pos"].shape[0]
@@ -276,7 +277,13 @@ def test_mixed_delays_equal_fp64_full_size(workload):
             mism = int(np.count_nonzero(a != b))
             assert mism == 0, f"{mism} of pairs [{first}, +{count}) differ"
             checked += count
-    print(f"{workload}: {checked} pairs, mixed delays == FP64 delays; {near} zero-delay pairs in both")
+        c64.set_pair_method(2)
+        c32.set_pair_method(2)
+        i64, i32 = c64.info(), c32.info()
+        assert i32["short_pairs"] == i64["short_pairs"] > 0
+        assert i32["block_steps"] == i64["block_steps"] == 16
+    print(f"{workload}: {checked} pairs, mixed delays == FP64 delays; {near} zero-delay pairs and "
+          f"{i64['short_pairs']} short-list pairs in both")
 
 
 def test_unbounded_solution_reports_first_bad_step(oracle_lib, ref_weights):

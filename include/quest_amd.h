@@ -66,9 +66,10 @@ typedef struct {
   int device;          /* HIP device ordinal                                */
   int source_chunk;    /* sources per pair-kernel work item; 0 = default
                           (the largest system in 48 (mixed) or 24 (FP64)
-                          equal chunks of at least 128 / 256 sources,
-                          halved until the work list fills the GPU); see
-                          quest_autotune                                    */
+                          equal chunks -- 8 in both with
+                          QUEST_PAIR_TIME_BLOCKED, taken when the method is
+                          set -- of at least 128 / 256 sources, halved until
+                          the work list fills the GPU); see quest_autotune  */
   int item_claim;      /* how the pair kernel's waves take work items:
                           QUEST_CLAIM_AUTO (static until quest_autotune,
                           which times both and keeps the faster),
@@ -257,11 +258,13 @@ int quest_run(quest_ctx *ctx, int n_steps);
  * are those of quest_run, bit for bit.  QUEST_ERR_ARG if the partial sums do
  * not fit (at the default split: mixed above ~85k dots, FP64 above ~167k;
  * longer chunks via quest_params.source_chunk shrink them; with
- * QUEST_PAIR_TIME_BLOCKED there are 8 slices of them: mixed above ~10k dots,
- * FP64 above ~20k).  With QUEST_PAIR_TIME_BLOCKED, a call that starts inside
- * an 8-step pass whose partial sums the previous quest_run left in the
- * context's buffer forms that pass again in the static array first (and
- * quest_run does the reverse).  Every call on one GPU shares that GPU's array:
+ * QUEST_PAIR_TIME_BLOCKED, whose default split is 8 chunks, there are K = 8
+ * or 16 slices of 9 rows: above ~58k dots with 8-step passes, ~29k with
+ * 16-step ones, fewer where the split is halved to fill the GPU).  With
+ * QUEST_PAIR_TIME_BLOCKED, a call that starts inside a pass whose partial
+ * sums the previous quest_run left in the context's buffer forms that pass
+ * again in the static array first (and quest_run does the reverse).  Every
+ * call on one GPU shares that GPU's array:
  * calls that overlap in time on one GPU (from several host threads) race and
  * must be serialised by the caller; a call has finished with the array when
  * it returns.  The pair kernel is not timed (quest_set_timing). */

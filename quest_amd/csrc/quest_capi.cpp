@@ -347,12 +347,18 @@ int build_work(quest_ctx *c, int chunk_len)
 // faster except at N = 32768 and 131072 (7 % slower there; quest_autotune
 // finds those).  A chunk count that is not a multiple of 8 (N = 49152 cut
 // into 4096-source chunks: 12) left half the XCDs with half the work: 1.5x.
+// The time-blocked pass takes 8 chunks in both precisions: each chunk adds a
+// row of partial sums per step, written by the pass and read back by the step
+// kernel, and against 16 and 48 chunks (N = 16384-131072, both precisions,
+// one box) 8 was the best on average (N = 65536: 0.9 % faster mixed, 2.3 %
+// FP64, than 48; the worst case N = 131072 FP64 2.7 % slower than 48).
 int default_chunk(const quest_ctx *c)
 {
   int nmax = 2;
   for(int s = 0; s < c->n_sys; ++s) nmax = std::max(nmax, c->sys_begin[s + 1] - c->sys_begin[s]);
   const bool mixed = c->P.precision == QUEST_PREC_MIXED;
-  const int count = mixed ? 48 : 24, min_len = mixed ? 128 : 256;
+  const bool blocked = c->pair_method == QUEST_PAIR_TIME_BLOCKED;
+  const int count = blocked ? 8 : mixed ? 48 : 24, min_len = mixed ? 128 : 256;
   int len = std::max(min_len, (nmax + count - 1) / count);
   const int resident = c->cu_count * c->blocks_per_cu * (kBlock / kWave);  // waves
   while(len > 64 && count_work(c, len) < resident) len = std::max(64, (len + 1) / 2);
@@ -1352,7 +1358,9 @@ int quest_set_pair_method(quest_ctx *c, int method)
   }
   c->pair_method = method;
   c->blocks_per_cu = default_blocks_per_cu(c);
-  return build_work(c, c->chunk_len);
+  // the new method's default chunk unless the caller fixed one (a length
+  // quest_autotune chose was timed with the other method's kernel)
+  return build_work(c, c->P.source_chunk > 0 ? c->P.source_chunk : default_chunk(c));
 }
 
 int quest_pulse_evaluate(quest_ctx *c, int step, double *out)

@@ -817,11 +817,15 @@ __device__ __forceinline__ void block_weights(const PairK<R> &K, R ir, R dx, R d
 // as pair_past_*.  Each step keeps three sums of real products (the complex
 // product with three multiplications), combined into re and im at the end
 // (FP64) or when the FP32 (mixed) sums are folded into FP64, every 64
-// sources.  The FP64 16-step pass is held to three waves per SIMD (168
-// VGPRs, a few of them spilled): 2 % faster than two waves at 193 VGPRs; the
-// FP32 one loses at three (45 spilled) and keeps two.
+// sources.  The order-4 FP64 16-step pass with the series sincos is held to
+// three waves per SIMD (168 VGPRs, a few of them spilled): 2 % faster than
+// two waves at 193 VGPRs.  The FP32 one loses at three (45 spilled) and keeps
+// two, and so does the FP64 one with the library sincos (GEN), which would
+// spill hundreds of bytes per lane; other orders are left to the compiler.
 template <typename R, int M, int K, bool LAB, bool DYN, bool GEN>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(double) && K == 16 ? 3 : 1, 8))) void pair_block(const typename V4<R>::type *__restrict__ pd,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == sizeof(double) && K == 16 && M == 4 && !GEN ? 3 : 1, 8))) void
+pair_block(const typename V4<R>::type *__restrict__ pd,
                                                      const double4 *__restrict__ pos64,
                                                      const typename V2<R>::type *__restrict__ hist,
                                                      const typename V2<R>::type *__restrict__ rot,

@@ -61,8 +61,7 @@ struct quest_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double2 *scratch = nullptr;  // this device's static scratch array (quest_run_noalloc)
-  int short_total = 0;         // time-blocked mode: pairs of delay 1 .. 7
-  int mid_total = 0;           // time-blocked mode, 16-step passes: pairs of delay 8 .. 15
+  int short_total = 0;         // time-blocked mode: pairs of delay 1 .. 7 (short list and the 4-step window list)
   int slice_buf = 0;           // time-blocked mode: the current block's partial sums are in the
                                // context's buffer (0) or the static scratch (1)
   size_t hist_bytes = 0;       // time-blocked mode: the sample ring
@@ -478,9 +477,9 @@ int pair_sum(quest_ctx *c, DeviceState &d, bool timed)
 }
 int pair_sum(quest_ctx *c) { return pair_sum(c, c->d, true); }
 
-// Time-blocked method: the list sums of step `step` -- the mid list's for 8
-// steps at the multiples of 8 (16-step passes), the short list's at every
-// step -- bracketed by HIP events when timing.
+// Time-blocked method: the list sums of step `step` -- each window list's
+// for its w steps at the multiples of w (w = 8 with 16-step passes, and 4),
+// the short list's at every step -- bracketed by HIP events when timing.
 int list_sums(quest_ctx *c, DeviceState &d, bool timed, int step)
 {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -492,8 +491,10 @@ int list_sums(quest_ctx *c, DeviceState &d, bool timed, int step)
     c->ev_kind.push_back(1);
     HIPC(c, hipEventRecord(e0, c->stream));
   }
-  if(c->k.mid_steps && step % c->k.mid_steps == 0) HIPC(c, launch_pair_list(c->k, d, c->P.precision, true, c->stream));
-  HIPC(c, launch_pair_list(c->k, d, c->P.precision, false, c->stream));
+  for(int l = kWindowLists - 1; l >= 0; --l)
+    if(c->k.win_steps[l] && step % c->k.win_steps[l] == 0)
+      HIPC(c, launch_pair_list(c->k, d, c->P.precision, l, c->stream));
+  HIPC(c, launch_pair_list(c->k, d, c->P.precision, -1, c->stream));
   if(timed) HIPC(c, hipEventRecord(e1, c->stream));
   return QUEST_OK;
 }
@@ -556,10 +557,13 @@ void free_device(quest_ctx *c)
                   d.near_start, d.near_src, d.near_tgt, d.near_n, d.near_coef, d.pcw,
                   d.bad_step, d.counts, d.perm_dev, d.io, d.sys_of, d.sys_range, d.sys_chunks,
                   d.pulses, d.work, d.work_row, d.step, d.samp, d.table, d.table_n, d.sys_toff,
-                  d.step_map, d.near_lane, d.hist, d.rot, d.short_start, d.short_src, d.short_sum,
-                  d.mid_start, d.mid_src, d.mid_sum};
+                  d.step_map, d.near_lane, d.hist, d.rot, d.short_start, d.short_src, d.short_sum};
   for(void *p : ptrs)
     if(p) (void)hipFree(p);
+  for(int l = 0; l < kWindowLists; ++l)
+    for(void *p : {static_cast<void *>(d.win_start[l]), static_cast<void *>(d.win_src[l]),
+                   static_cast<void *>(d.win_sum[l])})
+      if(p) (void)hipFree(p);
   d = DeviceState{};
   if(c->h_stage) (void)hipHostFree(c->h_stage);
   c->h_stage = nullptr;
@@ -1021,17 +1025,22 @@ void free_blocked(quest_ctx *c)
 {
   DeviceState &d = c->d;
   for(void *p : {d.hist, d.rot, static_cast<void *>(d.short_start), static_cast<void *>(d.short_src),
-                 static_cast<void *>(d.short_sum), static_cast<void *>(d.mid_start), static_cast<void *>(d.mid_src),
-                 static_cast<void *>(d.mid_sum)})
+                 static_cast<void *>(d.short_sum)})
     if(p) (void)hipFree(p);
+  for(int l = 0; l < kWindowLists; ++l) {
+    for(void *p : {static_cast<void *>(d.win_start[l]), static_cast<void *>(d.win_src[l]),
+                   static_cast<void *>(d.win_sum[l])})
+      if(p) (void)hipFree(p);
+    d.win_start[l] = d.win_src[l] = nullptr;
+    d.win_sum[l] = nullptr;
+    c->k.win_steps[l] = 0;
+  }
   d.hist = d.rot = nullptr;
-  d.short_start = d.short_src = d.mid_start = d.mid_src = nullptr;
-  d.short_sum = d.mid_sum = nullptr;
+  d.short_start = d.short_src = nullptr;
+  d.short_sum = nullptr;
   c->k.pair_blocked = 0;
   c->k.block_steps = 1;
-  c->k.mid_steps = 0;
   c->short_total = 0;
-  c->mid_total = 0;
   c->hist_bytes = 0;
 }
 
@@ -1064,8 +1073,9 @@ int build_list(quest_ctx *c, int lo, int hi, int **start_dev, int **src_dev, int
 
 // Time-blocked mode: the steps per pass K, the rho01 sample ring, the
 // rotation table e^{-i theta (n + 1/2)}, the short list (pairs of delay
-// 1 .. 7) and with K = 16 the mid list (8 .. 15), both by the precision's own
-// split, as the pair kernels classify them.  The ring holds
+// 1 .. 3), the 4-step window list (4 .. 7) and with K = 16 the 8-step one
+// (8 .. 15), all by the precision's own split, as the pair kernels classify
+// them.  The ring holds
 // more than the largest delay + order + K steps, so that the pass of the
 // current block can be formed again from it at any step of the block
 // (refill_block).
@@ -1102,7 +1112,8 @@ int enable_blocked(quest_ctx *c)
     K = static_cast<double>(short16) <= kLongPassShortFraction * ordered ? kBlockStepsLong : kBlockStepsShort;
   }
   k.block_steps = K;
-  k.mid_steps = K == kBlockStepsLong ? kBlockStepsShort : 0;
+  k.win_steps[0] = kShortListEnd;
+  k.win_steps[1] = K == kBlockStepsLong ? kBlockStepsShort : 0;
   k.hist_slots = next_pow2(c->dmax_bound + M + K + 1);
   k.hist_mask = k.hist_slots - 1;
   k.hist_row = (k.hist_slots + K + M - 1 + 1) / 2 * 2;
@@ -1113,8 +1124,10 @@ int enable_blocked(quest_ctx *c)
   c->hist_bytes = (nn * k.hist_row * r2 + 15) / 16 * 16;
   if(hipMalloc(&d.hist, c->hist_bytes) != hipSuccess || hipMalloc(&d.rot, k.hist_slots * r2) != hipSuccess ||
      hipMalloc(reinterpret_cast<void **>(&d.short_sum), sizeof(double2) * std::max<size_t>(nn, 1)) != hipSuccess ||
-     (k.mid_steps && hipMalloc(reinterpret_cast<void **>(&d.mid_sum),
-                               sizeof(double2) * k.mid_steps * std::max<size_t>(nn, 1)) != hipSuccess)) {
+     (k.win_steps[0] && hipMalloc(reinterpret_cast<void **>(&d.win_sum[0]),
+                                  sizeof(double2) * k.win_steps[0] * std::max<size_t>(nn, 1)) != hipSuccess) ||
+     (k.win_steps[1] && hipMalloc(reinterpret_cast<void **>(&d.win_sum[1]),
+                                  sizeof(double2) * k.win_steps[1] * std::max<size_t>(nn, 1)) != hipSuccess)) {
     free_blocked(c);
     return fail(c, QUEST_ERR_NOMEM, "time-blocked method: allocation failed");
   }
@@ -1133,9 +1146,15 @@ int enable_blocked(quest_ctx *c)
     else HIPC(c, hipMemcpyAsync(d.rot, hf.data(), sizeof(float2) * hf.size(), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));  // h, hf go out of scope
   }
-  // the short list (delays 1 .. 7); with 16-step passes the mid list (8 .. 15)
-  int rc = build_list(c, 1, kBlockStepsShort, &d.short_start, &d.short_src, &c->short_total);
-  if(rc == QUEST_OK && k.mid_steps) rc = build_list(c, kBlockStepsShort, K, &d.mid_start, &d.mid_src, &c->mid_total);
+  // the short list (delays 1 .. 3), the window lists (4 .. 7; 8 .. 15 with
+  // 16-step passes)
+  int short_pairs = 0, win_pairs[kWindowLists] = {};
+  int rc = build_list(c, 1, kShortListEnd, &d.short_start, &d.short_src, &short_pairs);
+  if(rc == QUEST_OK)
+    rc = build_list(c, kShortListEnd, kBlockStepsShort, &d.win_start[0], &d.win_src[0], &win_pairs[0]);
+  if(rc == QUEST_OK && k.win_steps[1])
+    rc = build_list(c, kBlockStepsShort, K, &d.win_start[1], &d.win_src[1], &win_pairs[1]);
+  c->short_total = short_pairs + win_pairs[0];
   if(rc != QUEST_OK) {
     const std::string msg = c->err;
     free_blocked(c);

@@ -22,6 +22,12 @@ constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
 constexpr int kBlockStepsShort = 8;
 constexpr int kBlockStepsLong = 16;
 constexpr double kLongPassShortFraction = 0.02;
+// the pairs of delay below K: 1 .. 3 on the short list, summed at every step;
+// 4 .. 7 on a window list summed every 4 steps for 4 steps; with 16-step
+// passes 8 .. 15 on a window list summed every 8 steps for 8 steps (a pair of
+// delay n >= w has its samples for w steps)
+constexpr int kShortListEnd = 4;
+constexpr int kWindowLists = 2;
 // e^{i theta s} tables: 2^10 entries each for the low and middle ten bits of
 // the step, 2^11 for the high bits (steps below 2^31)
 constexpr int kPhaseLevel = 1024;
@@ -93,7 +99,7 @@ struct Consts {
   int hist_slots;     // sample ring: power of two > max delay + order
   int hist_mask;
   int hist_row;       // entries per source row: hist_slots + K + order - 1, rounded up to even
-  int mid_steps;      // 8 with 16-step passes (the mid list, delays 8 .. 15, every 8 steps), else 0
+  int win_steps[kWindowLists];  // window lists: 4 (delays 4 .. 7), and 8 with 16-step passes (8 .. 15), else 0
 };
 
 struct PulseParams {
@@ -154,12 +160,12 @@ struct DeviceState {
   // time-blocked mode only
   void *hist = nullptr;                // rho01 sample ring [n][hist_row] (double2 FP64, float2 mixed)
   void *rot = nullptr;                 // [hist_slots] e^{-i theta (n + 1/2)} (double2 FP64, float2 mixed)
-  int *short_start = nullptr;          // [n + 1] short list (delays 1 .. 7), CSR by target
+  int *short_start = nullptr;          // [n + 1] short list (delays 1 .. 3), CSR by target
   int *short_src = nullptr;            // [short pairs]
   double2 *short_sum = nullptr;        // [n] the short list's sum at the current step
-  int *mid_start = nullptr;            // [n + 1] mid list (delays 8 .. 15, 16-step passes only), CSR by target
-  int *mid_src = nullptr;              // [mid pairs]
-  double2 *mid_sum = nullptr;          // [8][n] the mid list's sums for the 8 steps from its last launch
+  int *win_start[kWindowLists] = {};   // [n + 1] window list l (k.win_steps[l] != 0), CSR by target
+  int *win_src[kWindowLists] = {};     // [its pairs]
+  double2 *win_sum[kWindowLists] = {}; // [win_steps[l]][n] its sums for the steps from its last launch
   // one-launch predictor-corrector step: thread -> dot (-1 idle), each group of
   // zero-delay partners within one block, all groups in the leading blocks
   int *step_map = nullptr;
@@ -212,9 +218,9 @@ hipError_t launch_near_fill(const Consts &k, DeviceState &d, int precision, hipS
 hipError_t launch_delay_list(const Consts &k, DeviceState &d, int precision, int lo, int hi, bool fill,
                              const int *start, int *src, int *tgt, int *counts, hipStream_t s);
 // Time-blocked mode: the short list's sum at the current step into d.short_sum
-// (mid false), or the mid list's sums for the 8 steps from the current one
-// into d.mid_sum (mid true).
-hipError_t launch_pair_list(const Consts &k, DeviceState &d, int precision, bool mid, hipStream_t s);
+// (list < 0), or window list `list`'s sums for the k.win_steps[list] steps
+// from the current one into d.win_sum[list].
+hipError_t launch_pair_list(const Consts &k, DeviceState &d, int precision, int list, hipStream_t s);
 hipError_t launch_near_coef(const Consts &k, DeviceState &d, int near_total, hipStream_t s);
 hipError_t launch_pair_delays(const Consts &k, DeviceState &d, int precision, int32_t *out, int64_t first,
                               int64_t count, hipStream_t s);

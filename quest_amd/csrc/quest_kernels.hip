@@ -656,10 +656,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
 // it needs for the K steps t0 .. t0 + K - 1: its weights (geometry, Green's
 // function and Lagrange weights, the expensive part) are formed once and
 // applied to K consecutive windows of the source's rho01 samples (K = 8 or
-// 16).  The pairs with 1 <= n < 8 (the short list, ~28 per dot at 512 um^-3)
-// are summed step by step, with K = 16 those with 8 <= n < 16 (the mid list,
-// ~207 per dot) in 8-step windows at every multiple of 8 (pair_list), and the
-// zero-delay pairs stay on the near list.
+// 16).  The pairs with 1 <= n < 4 (the short list, ~4 per dot at 512 um^-3)
+// are summed step by step, those with 4 <= n < 8 (~24 per dot) in 4-step
+// windows at every multiple of 4, with K = 16 those with 8 <= n < 16 (~207
+// per dot) in 8-step windows at every multiple of 8 (pair_list: a pair of
+// delay n >= w has its samples for w steps), and the zero-delay pairs stay on
+// the near list.
 //
 // Sample ring: rho01 of source b at finished step u in row b, entry
 // u & hist_mask, and again at hist_slots + (u & hist_mask) when that is below
@@ -956,9 +958,10 @@ pair_block(const typename V4<R>::type *__restrict__ pd,
 }
 
 // Listed pairs of every target (CSR by target) over KS steps from t0 (the
-// step counter): the short list (delays 1 .. 7) at every step (KS = 1), and
-// with 16-step passes the mid list (delays 8 .. 15) at every multiple of 8
-// (KS = 8).  Per pair the weights of block_weights once, then KS windows of
+// step counter): the short list (delays 1 .. 3) at every step (KS = 1), the
+// delays 4 .. 7 at every multiple of 4 (KS = 4) and with 16-step passes the
+// delays 8 .. 15 at every multiple of 8 (KS = 8).  Per pair the weights of
+// block_weights once, then KS windows of
 // the source's samples (all finished: t0 + KS - 1 - n <= t0 - 1), summed in
 // FP64 into out[j n + a], j < KS.  kListLanes consecutive lanes share a
 // target and take every kListLanes-th pair of its list (a chain of dependent
@@ -998,9 +1001,10 @@ __global__ __launch_bounds__(kBlock) void pair_list(const typename V4<R>::type *
   // U pairs of the lane's share in flight at once: their list entries, then
   // their sources, then their sample windows are loaded together, so a lane
   // waits for U chains of dependent loads at a time instead of one.  The
-  // mid list (~23 pairs per lane at 512 um^-3) takes 2; the short list (~3
-  // per lane) 1: with 4 its ensemble kernel ran 58 -> 68 us per step (half
-  // the occupancy, a quarter of the slots empty)
+  // window lists (~3 and ~26 pairs per lane at 512 um^-3) take 2; the short
+  // list 1 (with 4 the ensemble's list of delays 1 .. 7, ~3 pairs per lane,
+  // ran 58 -> 68 us per step: half the occupancy, a quarter of the slots
+  // empty)
   constexpr int U = KS == 1 ? 1 : 2;
   const int e1 = valid ? list_start[a + 1] : 0;
   for(int e = list_start[a] + sub; e < e1; e += kListLanes * U) {
@@ -1177,15 +1181,17 @@ __device__ __forceinline__ double2 reduce_partials(const Consts &k, const Device
 
 // Field of the past pair sum at target a: the partials times -K e^{-i theta
 // step} (on-the-fly), as they are (the table holds complete coefficients), or
-// (time-blocked) the slice of this step plus the short list's sum (and, with
-// 16-step passes, the mid list's), times -K (rotating; the rotation is in the
-// weights) or K (lab).
+// (time-blocked) the slice of this step plus the sums of the short list and
+// of the window lists for this step, times -K (rotating; the rotation is in
+// the weights) or K (lab).
 __device__ __forceinline__ double2 pair_field(const Consts &k, const DeviceState &d, int a, int step)
 {
   if(k.pair_blocked) {
     const double2 *slice = d.partial + static_cast<size_t>(step % k.block_steps) * (k.max_chunks + 1) * k.n;
     double2 sum = cadd(reduce_partials(k, d, slice, a), d.short_sum[a]);
-    if(k.mid_steps) sum = cadd(sum, d.mid_sum[static_cast<size_t>(step % k.mid_steps) * k.n + a]);
+#pragma unroll
+    for(int l = 0; l < kWindowLists; ++l)
+      if(k.win_steps[l]) sum = cadd(sum, d.win_sum[l][static_cast<size_t>(step % k.win_steps[l]) * k.n + a]);
     const double g = k.dyadic == QUEST_DYADIC_LAB ? k.k_green : -k.k_green;
     return make_double2(g * sum.x, g * sum.y);
   }
@@ -1624,7 +1630,7 @@ __global__ void advance_step_kernel(int *step) { *step += 1; }
 // Number (FILL = false) or list (FILL = true, in source order) of the sources
 // b != a of each target whose pair the pair kernels of this precision
 // classify with an integer delay in [lo, hi): the zero-delay list ([0, 1))
-// and the time-blocked method's short list ([1, K)).
+// and the time-blocked method's lists ([1, 4), [4, 8), [8, 16)).
 template <bool F64, bool FILL>
 __global__ __launch_bounds__(kBlock) void near_scan_kernel(Consts k, DeviceState d, int lo, int hi,
                                                            const int *__restrict__ start, int *__restrict__ src,
@@ -1914,10 +1920,13 @@ hipError_t launch_list_mk(const Consts &k, DeviceState &d, const int *start, con
 }
 
 template <typename R, int M>
-hipError_t launch_list_m(const Consts &k, DeviceState &d, bool mid, hipStream_t s)
+hipError_t launch_list_m(const Consts &k, DeviceState &d, int list, hipStream_t s)
 {
-  return mid ? launch_list_mk<R, M, kBlockStepsShort>(k, d, d.mid_start, d.mid_src, d.mid_sum, s)
-             : launch_list_mk<R, M, 1>(k, d, d.short_start, d.short_src, d.short_sum, s);
+  if(list < 0) return launch_list_mk<R, M, 1>(k, d, d.short_start, d.short_src, d.short_sum, s);
+  static_assert(kShortListEnd == 4 && kBlockStepsShort == 8, "window lists of 4 and 8 steps");
+  return k.win_steps[list] == kBlockStepsShort
+             ? launch_list_mk<R, M, kBlockStepsShort>(k, d, d.win_start[list], d.win_src[list], d.win_sum[list], s)
+             : launch_list_mk<R, M, kShortListEnd>(k, d, d.win_start[list], d.win_src[list], d.win_sum[list], s);
 }
 
 template <int M>
@@ -2097,16 +2106,16 @@ int pair_blocks_per_cu(const Consts &k, int precision)
   }
 }
 
-hipError_t launch_pair_list(const Consts &k, DeviceState &d, int precision, bool mid, hipStream_t s)
+hipError_t launch_pair_list(const Consts &k, DeviceState &d, int precision, int list, hipStream_t s)
 {
   const bool f64 = precision == QUEST_PREC_FP64;
   switch(k.order) {
-  case 1: return f64 ? launch_list_m<double, 1>(k, d, mid, s) : launch_list_m<float, 1>(k, d, mid, s);
-  case 2: return f64 ? launch_list_m<double, 2>(k, d, mid, s) : launch_list_m<float, 2>(k, d, mid, s);
-  case 3: return f64 ? launch_list_m<double, 3>(k, d, mid, s) : launch_list_m<float, 3>(k, d, mid, s);
-  case 4: return f64 ? launch_list_m<double, 4>(k, d, mid, s) : launch_list_m<float, 4>(k, d, mid, s);
-  case 5: return f64 ? launch_list_m<double, 5>(k, d, mid, s) : launch_list_m<float, 5>(k, d, mid, s);
-  case 6: return f64 ? launch_list_m<double, 6>(k, d, mid, s) : launch_list_m<float, 6>(k, d, mid, s);
+  case 1: return f64 ? launch_list_m<double, 1>(k, d, list, s) : launch_list_m<float, 1>(k, d, list, s);
+  case 2: return f64 ? launch_list_m<double, 2>(k, d, list, s) : launch_list_m<float, 2>(k, d, list, s);
+  case 3: return f64 ? launch_list_m<double, 3>(k, d, list, s) : launch_list_m<float, 3>(k, d, list, s);
+  case 4: return f64 ? launch_list_m<double, 4>(k, d, list, s) : launch_list_m<float, 4>(k, d, list, s);
+  case 5: return f64 ? launch_list_m<double, 5>(k, d, list, s) : launch_list_m<float, 5>(k, d, list, s);
+  case 6: return f64 ? launch_list_m<double, 6>(k, d, list, s) : launch_list_m<float, 6>(k, d, list, s);
   default: return hipErrorInvalidValue;
   }
 }

@@ -519,8 +519,9 @@ def main(argv=None) -> int:
                                f"2 rad/ps detuning, pi/2 pulse, dt=1e-4 ps, order 4, "
                                f"{full_steps}-step run", "n_dots": n,
                    "pair_method": ("time-blocked: one pass per {0} steps over the pairs of delay >= {0}, "
-                                   "a per-step short list for delays 1-{1}".format(
-                                       mixed["steps_per_launch"], mixed["steps_per_launch"] - 1)
+                                   "lists for delays 1-3 (every step), 4-7 (4-step windows){1}".format(
+                                       mixed["steps_per_launch"],
+                                       ", 8-15 (8-step windows)" if mixed["steps_per_launch"] == 16 else "")
                                    if method == PAIR_TIME_BLOCKED else "on-the-fly, one pass per step"),
                    "source_chunk": mixed["info"]["source_chunk"],
                    "source_chunks_per_tile": mixed["info"]["splits"],

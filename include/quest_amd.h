@@ -182,8 +182,9 @@ int quest_hist_evaluate(quest_ctx *ctx, int step, const double *rho01,
  *             >= K forms their sums for the K steps t0 .. t0 + K - 1: the
  *             pair's weights (geometry, Green's function, Lagrange weights)
  *             once, then K windows of the source's rho01 samples.  The pairs
- *             of delay 1 .. 7 are summed step by step from a short list, with
- *             K = 16 those of delay 8 .. 15 from a mid list in 8-step windows
+ *             of delay 1 .. 3 are summed step by step from a short list, those
+ *             of delay 4 .. 7 from a list in 4-step windows every 4 steps,
+ *             with K = 16 those of delay 8 .. 15 from a list in 8-step windows
  *             every 8 steps, the zero-delay pairs as in the other methods.
  *             Memory: a sample ring of n x (the largest delay + order + K, to
  *             a power of two, + K + order) complex values and K slices of the
@@ -310,10 +311,10 @@ int quest_set_timing(quest_ctx *ctx, int enable);
 int quest_pair_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
                            int reset);
 
-/* The same for the per-step short-list kernel of QUEST_PAIR_TIME_BLOCKED
- * (pairs of delay 1 .. 7, every step) and mid-list kernel (delays 8 .. 15,
- * every 8 steps, with 16-step passes), added up; launches counts the steps
- * they served.  Zero for the other methods. */
+/* The same for the list kernels of QUEST_PAIR_TIME_BLOCKED (pairs of delay
+ * 1 .. 3 every step, 4 .. 7 every 4 steps, 8 .. 15 every 8 steps with
+ * 16-step passes), added up; launches counts the steps they served.  Zero
+ * for the other methods. */
 int quest_short_kernel_time(quest_ctx *ctx, double *ms, int64_t *launches,
                             int reset);
 
@@ -337,7 +338,7 @@ int quest_autotune(quest_ctx *ctx, int launches, int *chunk_len);
  * pair-kernel items claimed dynamically (1) or by a static stride (0),
  * pipelined (1) or plain (0) source loop of the mixed-precision pair kernel,
  * pair method (QUEST_PAIR_*), steps per pass of the time-blocked method (0
- * for the others), its short-list pairs (delays 1 .. 7), its sample-ring
+ * for the others), its listed pairs of delay 1 .. 7, its sample-ring
  * slots. */
 int quest_ctx_info(const quest_ctx *ctx, int64_t *info);
 

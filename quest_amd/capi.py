@@ -319,9 +319,9 @@ class Context:
         return ms.value, cnt.value
 
     def short_kernel_time(self, reset: bool = False):
-        """Time-blocked method: accumulated list-kernel time (ms: the short
-        list every step, the mid list every 8 steps with 16-step passes) and
-        the number of steps it served."""
+        """Time-blocked method: accumulated list-kernel time (ms: delays 1-3
+        every step, 4-7 every 4 steps, 8-15 every 8 steps with 16-step passes)
+        and the number of steps it served."""
         ms = C.c_double()
         cnt = C.c_int64()
         self._check(self._L.quest_short_kernel_time(self.ptr, C.byref(ms), C.byref(cnt), int(reset)))

@@ -1,9 +1,10 @@
 """The time-blocked pair method (QUEST_PAIR_TIME_BLOCKED) against the oracle
 and against the per-step on-the-fly method.
 
-One pass at every step that is a multiple of K = 8 forms the past sums of the
-pairs of delay >= 8 for the next 8 steps, a short list the pairs of delay
-1..7 step by step; the sums are the reference's (history_interaction.cpp:47-70)
+One pass at every step that is a multiple of K = 8 (or 16) forms the past sums
+of the pairs of delay >= K for the next K steps, lists the pairs of delay 1..3
+step by step, 4..7 in 4-step windows and (K = 16) 8..15 in 8-step windows; the
+sums are the reference's (history_interaction.cpp:47-70)
 in another order, so FP64 runs meet the oracle at 1e-10 and mixed runs at
 1e-6, as the per-step method does.
 """

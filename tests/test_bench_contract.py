@@ -1,6 +1,7 @@
 """bench.py's contract pieces that need no GPU: the committed rocprof
 summaries it reads its HBM traffic from carry the kernel names it labels its
-line with, and a request for more than one GPU is refused with a message."""
+line with, a request for more than one GPU is refused with a message, and the
+timed region holds whole time-blocked passes."""
 import json
 import os
 import subprocess
@@ -29,3 +30,45 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+class _FakeContext:
+    """Records the step counts bench.timed_steps asks a context to run."""
+
+    def __init__(self, block_steps):
+        self.block_steps = block_steps
+        self.runs = []
+
+    def info(self):
+        return {"block_steps": self.block_steps}
+
+    def run(self, n):
+        self.runs.append(n)
+
+    def sync(self):
+        pass
+
+    def set_timing(self, on):
+        pass
+
+    def pair_kernel_time(self, reset=False):
+        return 0.0, 0
+
+    def short_kernel_time(self, reset=False):
+        return 0.0, 0
+
+
+def test_timed_region_holds_whole_passes():
+    """With K steps per pass the warm-up ends on a pass boundary and the timed
+    steps are a multiple of K: 20 requested steps with 16-step passes time 32
+    (two whole passes), never one pass for 20 steps."""
+    sys.path.insert(0, ROOT)
+    import bench
+    for k, warm, steps, want_warm, want_steps in ((16, 5, 20, 16, 32), (8, 5, 20, 8, 24),
+                                                    (16, 304, 192, 304, 192), (1, 5, 20, 5, 20),
+                                                    (16, 0, 16, 0, 16)):
+        ctx = _FakeContext(k)
+        out = bench.timed_steps(ctx, warm, steps)
+        assert out["warmup"] == want_warm and out["steps"] == want_steps
+        runs = ctx.runs if want_warm else [0] + ctx.runs
+        assert runs == [want_warm, want_steps, 16, want_steps]

@@ -845,8 +845,8 @@ int create(quest_ctx *c, const quest_params *p, int n_sys, const int *sizes, con
   HIPC(c, alloc(reinterpret_cast<void **>(&d.counts), sizeof(int) * (nn + 1)));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.pcw), sizeof(double) * 4 * kMaxWindow));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.bad_step), sizeof(int)));
-  HIPC(c, alloc(reinterpret_cast<void **>(&d.step), 4 * sizeof(int)));
-  HIPC(c, launch_clear(d.step, 4 * sizeof(int), c->stream));  // step and the pair kernels' item counters
+  HIPC(c, alloc(reinterpret_cast<void **>(&d.step), kStepCounterInts * sizeof(int)));
+  HIPC(c, launch_clear(d.step, kStepCounterInts * sizeof(int), c->stream));  // step and every counter
   HIPC(c, alloc(reinterpret_cast<void **>(&d.perm_dev), sizeof(int) * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.io), sizeof(double2) * 2 * nn));
   HIPC(c, alloc(reinterpret_cast<void **>(&d.sys_of), sizeof(int) * nn));

@@ -16,6 +16,14 @@ constexpr int kMaxOrder = 6;       // interpolation orders 1..6 are instantiated
 constexpr int kMaxWindow = 64;     // PC history window limit
 constexpr int kBlock = 256;        // threads per block of the per-dot and pair kernels
 constexpr int kWave = 64;          // wavefront size; a pair-kernel work item is one wave of targets
+// The step kernel counts its finished blocks in kStepGroups counters (block b
+// in counter b % kStepGroups), each on its own 128-B line after the first
+// line of the step counter array, and the last block of each group counts
+// the group at step[3]: blocks that finish together spread their atomics
+// over many lines instead of queueing on one.
+constexpr int kStepGroups = 64;
+constexpr int kCounterStride = 32;  // ints per 128-B line
+constexpr int kStepCounterInts = kCounterStride * (1 + kStepGroups);
 constexpr int kNumCorrectorSteps = 10;  // integrator.cpp:3
 // steps of one pass of the time-blocked pair sum: 8, or 16 where the pairs of
 // delay below 16 are few (quest_params.block_steps)
@@ -148,9 +156,10 @@ struct DeviceState {
   PulseParams *pulses = nullptr;       // [systems][QUEST_MAX_PULSES], used ones first
   int4 *work = nullptr;                // [n_work] (target begin, target end, source begin, source end)
   int *work_row = nullptr;             // [n_work] partial row written by the item
-  int *step = nullptr;                 // [4]: the step being integrated (graph replays read it), the
-                                       // pair kernels' item counters, the step kernel's finished
-                                       // blocks (all three zero between launches)
+  int *step = nullptr;                 // [kStepCounterInts]: the step being integrated (graph replays
+                                       // read it), the pair kernels' item counters, the step kernel's
+                                       // finished groups, then its finished blocks per group, one
+                                       // counter per 128-B line (all counters zero between launches)
   // coefficient-table mode only
   double2 *samp = nullptr;             // rho01 sample ring [n][samp_slots] (finished steps)
   double2 *table = nullptr;            // [order + 1][table_plane]: system s, source b, target a at

@@ -157,9 +157,10 @@ struct DeviceState {
   int4 *work = nullptr;                // [n_work] (target begin, target end, source begin, source end)
   int *work_row = nullptr;             // [n_work] partial row written by the item
   int *step = nullptr;                 // [kStepCounterInts]: the step being integrated (graph replays
-                                       // read it), the pair kernels' item counters, the step kernel's
-                                       // finished groups, then its finished blocks per group, one
-                                       // counter per 128-B line (all counters zero between launches)
+                                       // read it), the pair kernels' claim counter ([1]), the step
+                                       // kernel's finished groups ([3]), then its finished blocks per
+                                       // group, one counter per 128-B line (all counters zero between
+                                       // launches)
   // coefficient-table mode only
   double2 *samp = nullptr;             // rho01 sample ring [n][samp_slots] (finished steps)
   double2 *table = nullptr;            // [order + 1][table_plane]: system s, source b, target a at

@@ -453,43 +453,39 @@ __device__ __forceinline__ int items_stride() { return static_cast<int>(gridDim.
 
 // Work items of the persistent pair kernels, one wave per item.  Static
 // (DYN false): wave j of block i takes items 4i + j, 4i + j + 4 grid, ...
-// (XCD-aware with the list's chunk grouping).  Dynamic: each wave claims the
-// next item from a device counter, so uneven items (the systems of a batch)
-// balance; every wave makes exactly one failing claim and then counts itself
-// out, and the last wave out resets both counters for the next launch.
-// ctr: [0] next item, [1] waves out (both 0 between launches).
+// (XCD-aware with the list's chunk grouping).  Dynamic: every wave starts
+// with its static first item, then claims item W + c (W the grid's waves, c
+// from a device counter), so uneven items (the systems of a batch) balance
+// without the whole grid's first claims queueing on the counter at once.
+// Each wave that takes an item makes exactly one failing claim, so the
+// claims number n_work in all: the one that draws n_work - 1 is the last and
+// resets the counter for the next launch.
+// ctr: [0] claims drawn (0 between launches).
 template <bool DYN>
 struct ItemCursor {
   static constexpr bool dyn = DYN;
   int *ctr;
+  int n_work;
   int w;
-  __device__ explicit ItemCursor(int *c) : ctr(c), w(0) {}
+  __device__ ItemCursor(int *c, int n) : ctr(c), n_work(n), w(0) {}
   __device__ int claim()
   {
     int t = 0;
-    if((threadIdx.x & (kWave - 1)) == 0) t = atomicAdd(ctr, 1);
-    return __builtin_amdgcn_readfirstlane(t);
+    if((threadIdx.x & (kWave - 1)) == 0) {
+      t = atomicAdd(ctr, 1);
+      if(t == n_work - 1) atomicExch(ctr, 0);
+    }
+    return items_stride() + __builtin_amdgcn_readfirstlane(t);
   }
   __device__ int first()
   {
-    w = dyn ? claim() : first_item();
+    w = first_item();
     return w;
   }
   __device__ int next()
   {
     w = dyn ? claim() : w + items_stride();
     return w;
-  }
-  // No fence: a wave's failing claim returned before it counts itself out,
-  // which is all the reset needs (a device-scope fence writes back the L2).
-  __device__ void finish()
-  {
-    if(!dyn || (threadIdx.x & (kWave - 1)) != 0) return;
-    const int waves = static_cast<int>(gridDim.x) * (kBlock / kWave);
-    if(atomicAdd(ctr + 1, 1) == waves - 1) {
-      atomicExch(ctr, 0);
-      atomicExch(ctr + 1, 0);
-    }
   }
 };
 
@@ -504,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
   const PairK<double> K(k);
   const int step = *step_ptr;
   const size_t E = static_cast<size_t>(k.n) * k.ring_slots;  // entries per coefficient plane
-  ItemCursor<DYN> items(step_ptr + 1);
+  ItemCursor<DYN> items(step_ptr + 1, k.n_work);
   for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
@@ -533,7 +529,6 @@ __global__ __launch_bounds__(kBlock) void pair_past_f64(
     if(tgt < wi.y)
       partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
   }
-  items.finish();
 }
 
 // SGPR budget: at 82 SGPRs (what the compiler picks for this loop nest) the
@@ -573,7 +568,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
   // (the host keeps a plane below 4 GiB), formed as (slot * 16) | (b ring_slots * 16)
   // since slot < ring_slots, a power of two; the 8-B plane's offset is half of it
   const unsigned row_step16 = static_cast<unsigned>(k.ring_slots) << 4;
-  ItemCursor<DYN> items(step_ptr + 1);
+  ItemCursor<DYN> items(step_ptr + 1, k.n_work);
   for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
@@ -644,7 +639,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_num_sgpr(64))) void p
     if(tgt < wi.y)
       partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr * k.unit_scale, acci * k.unit_scale);
   }
-  items.finish();
 }
 
 // ------------------------------------------------- time-blocked pair sums
@@ -845,7 +839,7 @@ pair_block(const typename V4<R>::type *__restrict__ pd,
   const size_t slice = static_cast<size_t>(k.max_chunks + 1) * k.n;
   const float ms = kRoundMagic + static_cast<float>(sm);
   const int code0 = kRoundMagicBits + sm;  // n = code0 - code (FP32 split)
-  ItemCursor<DYN> items(step_ptr + 1);
+  ItemCursor<DYN> items(step_ptr + 1, k.n_work);
   for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int tgt = wi.x + static_cast<int>(threadIdx.x % kWave);
@@ -954,7 +948,6 @@ pair_block(const typename V4<R>::type *__restrict__ pd,
         out[j * slice] = make_double2(accr[j] * k.unit_scale, acci[j] * k.unit_scale);
     }
   }
-  items.finish();
 }
 
 // Listed pairs of every target (CSR by target) over KS steps from t0 (the
@@ -1116,7 +1109,7 @@ __global__ __launch_bounds__(kBlock) void pair_table_f64(
 {
   const int step = *step_ptr;
   const size_t P = static_cast<size_t>(k.table_plane);
-  ItemCursor<false> items(step_ptr + 1);  // table items are uniform: static stride
+  ItemCursor<false> items(step_ptr + 1, k.n_work);  // table items are uniform: static stride
   for(int w = items.first(); w < k.n_work; w = items.next()) {
     const int4 wi = work[w];
     const int s = sys_of[wi.x];
@@ -1141,7 +1134,6 @@ __global__ __launch_bounds__(kBlock) void pair_table_f64(
     }
     if(tgt < wi.y) partial[static_cast<size_t>(work_row[w]) * k.n + tgt] = make_double2(accr, acci);
   }
-  items.finish();
 }
 
 // ------------------------------------------------------- per-dot kernels

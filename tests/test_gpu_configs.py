@@ -15,7 +15,9 @@ tests/test_golden.py):
   populations within 1e-6 absolute.
 
 Both tests run the per-step pair method (QUEST_PAIR_ON_THE_FLY) and the
-time-blocked one (QUEST_PAIR_TIME_BLOCKED) to the same bounds.
+time-blocked one (QUEST_PAIR_TIME_BLOCKED) to the same bounds.  The N = 65536
+and N = 131072 workloads also run for their full lengths (2000 and 1000
+steps) in both precisions, mixed against FP64.
 """
 import math
 import os
@@ -107,4 +109,38 @@ def test_ensemble_256_systems_5000_steps_both_precisions(method):
     print(f"ensemble at step {steps}: max|rho01| {np.abs(final[0][:, 1]).max():.3f}; mixed vs FP64 "
           f"rho01 {e01:.2e} (worst system, relative), rho00 {e00:.2e} absolute")
     assert np.abs(final[0][:, 1]).max() > 0.1
+    assert e01 < 1e-6 and e00 < 1e-6
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("name", ["n65536", "n131072"])
+def test_large_workload_full_length_mixed_vs_fp64(name):
+    """The N = 65536 workload for all 2000 steps and the N = 131072 one for all
+    1000, in both precisions with the time-blocked method (the bench's): the
+    state stays finite at every record, and at the last step the mixed rho01
+    is within 1e-6 of the FP64 rho01 (norm-wise relative; measured 1.0e-8 at
+    N = 65536, 7.2e-9 at N = 131072) and the populations within 1e-6
+    absolute (measured 2e-15).  The pi/2 pulse peaks at 0.5 ps, after these
+    runs end, so |rho01| reaches only its leading tail (1.4e-3 and 3.8e-5 at
+    the last step); the accuracy under a strong drive is the 100-step test in
+    test_gpu_parity.py."""
+    from quest_amd import Context, pc_weights, systems
+    sysd, steps = systems.workload(name)
+    every = 500
+    final = {}
+    for precision in (0, 1):
+        with Context(sysd, systems.consts(), precision=precision, pulse=systems.pulse(math.pi / 2)) as ctx:
+            ctx.set_pc_weights(*pc_weights(18, 22, 3.15))
+            ctx.set_pair_method(2)
+            for _ in range(steps // every):
+                ctx.run(every)
+                rho, _ = ctx.state()
+                assert np.all(np.isfinite(rho))
+            assert ctx.step == steps
+            final[precision] = rho
+    e01 = rel_err(final[1][:, 1], final[0][:, 1])
+    e00 = float(np.abs(final[1][:, 0] - final[0][:, 0]).max())
+    print(f"{name} x {steps} steps: max|rho01| {np.abs(final[0][:, 1]).max():.3e}; mixed vs FP64 rho01 {e01:.2e} "
+          f"(relative), rho00 {e00:.2e} (absolute)")
+    assert np.abs(final[0][:, 1]).max() > 1e-9
     assert e01 < 1e-6 and e00 < 1e-6

@@ -4,10 +4,13 @@
 
 One step = one full predictor-corrector timestep of every dot (pair sum,
 pulse, predictor, 10 corrector iterations, Bloch RHS), as Integrator::solve
-does it (src/integrator.cpp:120-185).  With a method that forms K steps per
-pass, the warm-up and the timed steps are rounded up to multiples of K, so the
-timed region holds whole passes; `steps` is the number timed and
-`steps_requested` the number asked for.  The pair sums use the time-blocked
+does it (src/integrator.cpp:120-185).  The warm-up is at least the steps
+that fill the ring's active window with computed history (the largest delay
++ order + 2), so the timed steps run in a long run's steady state; with a
+method that forms K steps per pass, the warm-up and the timed steps are
+rounded up to multiples of K, so the timed region holds whole passes.
+`warmup` and `steps` are the numbers run, `warmup_requested` and
+`steps_requested` the numbers asked for.  The pair sums use the time-blocked
 method by default (QUEST_PAIR_TIME_BLOCKED: one pass per 8 or 16 steps over
 the pairs whose delay is at least that long, list kernels for the shorter
 delays); the per-step method's figures follow in `per_step_method`.  The
@@ -90,13 +93,15 @@ def fill_warmup(ctx, order=4):
 
 
 def timed_steps(ctx, warmup, steps):
-    """Warm-up (warmup < 0: fill_warmup), then the timed region: `steps` steps
+    """Warm-up (at least fill_warmup), then the timed region: `steps` steps
     launched kernel by kernel with HIP events on the context's stream around
     every pair-kernel launch (wall time and the pair kernel's mean duration
     over the same steps); then `steps` steps replayed as one captured graph
     per step (captured outside the timed loop)."""
-    if warmup < 0:
-        warmup = fill_warmup(ctx)
+    # At least the ring-filling warm-up, whatever was asked for: with fewer
+    # steps most pairs still read the zero prehistory and the GPU holds a
+    # higher clock than in the steady state of a run (same instructions).
+    warmup = max(warmup, fill_warmup(ctx))
     # The time-blocked method forms the pair sums of K steps in one pass launched
     # at multiples of K: the timed region starts on a pass boundary and holds
     # whole passes (the requested steps rounded up to a multiple of K), so it
@@ -180,7 +185,7 @@ def ensemble_main(args):
     out = {
         "metric": "pair-interactions/s (sum of N^2 * steps / s), 256 batched systems",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": mixed["steps"],
-        "steps_requested": args.steps,
+        "steps_requested": args.steps, "warmup_requested": args.warmup,
         "warmup": mixed["warmup"], "ms_per_step": 1e3 * mixed["seconds"] / mixed["steps"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 pair arithmetic + f64 accumulation/state", "data": "synthetic",
@@ -461,8 +466,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=192,
                     help="timed steps (default 192: whole passes of the time-blocked method)")
     ap.add_argument("--warmup", type=int, default=-1,
-                    help="untimed steps before the timed ones; -1 (default): enough to fill the "
-                         "interpolant ring's active window (max delay + order + 2 steps)")
+                    help="untimed steps before the timed ones, at least (and by default) enough to fill "
+                         "the ring's active window with computed history (max delay + order + 2 steps)")
     ap.add_argument("--workload", default="n65536")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
@@ -523,7 +528,7 @@ def main(argv=None) -> int:
     out = {
         "metric": f"pair-interactions/s (N²·steps/s), mixed vs FP64 on GPU at N={n}, ρ err≤1e-6",
         "value": value, "unit": "pair-interactions/s", "n_gpus": 1, "steps": mixed["steps"],
-        "steps_requested": args.steps,
+        "steps_requested": args.steps, "warmup_requested": args.warmup,
         "warmup": mixed["warmup"], "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 pair arithmetic + f64 accumulation/state",
         "data": "synthetic",

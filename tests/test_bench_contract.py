@@ -1,7 +1,7 @@
 """bench.py's contract pieces that need no GPU: the committed rocprof
 summaries it reads its HBM traffic from carry the kernel names it labels its
 line with, a request for more than one GPU is refused with a message, and the
-timed region holds whole time-blocked passes."""
+warm-up fills the ring and the timed region holds whole time-blocked passes."""
 import json
 import os
 import subprocess
@@ -40,7 +40,7 @@ class _FakeContext:
         self.runs = []
 
     def info(self):
-        return {"block_steps": self.block_steps}
+        return {"block_steps": self.block_steps, "max_delay_bound": 40}
 
     def run(self, n):
         self.runs.append(n)
@@ -59,16 +59,17 @@ class _FakeContext:
 
 
 def test_timed_region_holds_whole_passes():
-    """With K steps per pass the warm-up ends on a pass boundary and the timed
-    steps are a multiple of K: 20 requested steps with 16-step passes time 32
-    (two whole passes), never one pass for 20 steps."""
+    """The warm-up fills the ring (largest delay 40 + order 4 + 2 = 46 steps,
+    rounded up to 48) whatever is asked for, and with K steps per pass it ends
+    on a pass boundary and the timed steps are a multiple of K: 20 requested
+    steps with 16-step passes time 32 (two whole passes), never one pass for
+    20 steps."""
     sys.path.insert(0, ROOT)
     import bench
-    for k, warm, steps, want_warm, want_steps in ((16, 5, 20, 16, 32), (8, 5, 20, 8, 24),
-                                                    (16, 304, 192, 304, 192), (1, 5, 20, 5, 20),
-                                                    (16, 0, 16, 0, 16)):
+    for k, warm, steps, want_warm, want_steps in ((16, 5, 20, 48, 32), (8, 5, 20, 48, 24),
+                                                    (16, 304, 192, 304, 192), (1, 5, 20, 48, 20),
+                                                    (16, -1, 16, 48, 16), (8, 100, 20, 104, 24)):
         ctx = _FakeContext(k)
         out = bench.timed_steps(ctx, warm, steps)
         assert out["warmup"] == want_warm and out["steps"] == want_steps
-        runs = ctx.runs if want_warm else [0] + ctx.runs
-        assert runs == [want_warm, want_steps, 16, want_steps]
+        assert ctx.runs == [want_warm, want_steps, 16, want_steps]

@@ -197,7 +197,8 @@ int count_work(const quest_ctx *c, int chunk_len)
 // chunk (the waves of one block; for systems of >= 256 dots a 256-target
 // tile).  The partial row an item writes is its chunk's index.
 //   Dynamic claiming: tile groups in order, the chunks of a tile group
-// consecutive; waves claim items in list order.
+// consecutive; wave w of the grid takes item w first, then the waves claim
+// the items after the first grid's worth in list order.
 //   Static stride (block i's waves take items 4i..4i+3, then 4(i + grid)..,
 // and block i runs on XCD i mod 8 under round-robin dispatch): the unit of
 // placement is a (system, chunk) pair -- all tile groups of the system against
